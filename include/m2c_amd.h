@@ -1,0 +1,275 @@
+/*
+ * m2c_amd.h -- C ABI of the MI355X explicit finite-volume step.
+ *
+ * This header is the drop-in boundary between M2C-style host code and the
+ * gfx950 HIP kernels in libm2c_amd.so.  Every entry point takes plain
+ * pointers and sizes; no HIP, torch or C++ types cross it.  Each function
+ * names the reference interface it replaces (kevinwgy/m2c, file:line).
+ *
+ * Conventions
+ *  - Return 0 on success, a negative M2CX_E* code on a fatal error (the
+ *    reference would have called exit()), a positive count for warnings
+ *    (e.g. Riemann failures, SpaceOperator.cpp:2031-2034).  The library never
+ *    exits; m2cx_last_error() gives the message.
+ *  - Host arrays use the reference's SpaceVariable3D layout: a ghosted brick of
+ *    (nz+2) x (ny+2) x (nx+2) cells, one ghost layer, AoS with `dof` doubles per
+ *    cell, i fastest: index ((k+1)*(ny+2) + (j+1))*(nx+2) + (i+1), times dof,
+ *    plus p (SpaceVariable.h:86-96, SpaceVariable.cpp:55-162).  Material IDs
+ *    are int32 in the same brick (the reference stores them as doubles).
+ *  - On the device everything is structure-of-arrays FP64 with an x-padded
+ *    pitch; the AoS<->SoA transpose happens only in upload/download.
+ *  - One host thread per handle; all work is ordered on the handle's stream.
+ */
+#ifndef M2C_AMD_H
+#define M2C_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Enumerations, numbered as in the reference's IoData / VarFcnBase.   */
+/* ------------------------------------------------------------------ */
+
+/* VarFcnBase::Type (VarFcnBase.h:40-43) */
+enum { M2CX_EOS_SG = 0, M2CX_EOS_NASG = 1, M2CX_EOS_MG = 2, M2CX_EOS_DUMMY = 8 };
+
+/* MeshData::BcType (IoData.h:283-284) */
+enum { M2CX_BC_NONE = 0, M2CX_BC_INLET = 1, M2CX_BC_INLET2 = 2, M2CX_BC_SLIPWALL = 3,
+       M2CX_BC_STICKWALL = 4, M2CX_BC_SYMMETRY = 5, M2CX_BC_OVERSET = 6, M2CX_BC_OUTLET = 7 };
+
+/* LevelSetSchemeData::BcType (IoData.h:814) */
+enum { M2CX_LSBC_ZERO_NEUMANN = 1, M2CX_LSBC_LINEAR_EXTRAPOLATION = 2, M2CX_LSBC_NON_NEGATIVE = 3 };
+
+/* SchemeData::Flux (IoData.h:710) */
+enum { M2CX_FLUX_ROE = 0, M2CX_FLUX_LLF = 1, M2CX_FLUX_HLLC = 2 };
+
+/* ReconstructionData (IoData.h:662-671) */
+enum { M2CX_REC_CONSTANT = 0, M2CX_REC_LINEAR = 1 };
+enum { M2CX_LIM_NONE = 0, M2CX_LIM_GENERALIZED_MINMOD = 1, M2CX_LIM_VANALBADA = 2 };
+enum { M2CX_SLOPE_ZERO = 0, M2CX_SLOPE_NONZERO = 1 };
+enum { M2CX_VAR_PRIMITIVE = 0 };
+
+/* ExplicitData::Type (IoData.h:931) */
+enum { M2CX_TI_FORWARD_EULER = 0, M2CX_TI_RK2 = 1, M2CX_TI_RK3 = 2 };
+
+/* MultiPhaseData (IoData.h:893-903) */
+enum { M2CX_MPFLUX_EXACT = 0, M2CX_MPFLUX_NUMERICAL = 1, M2CX_MPFLUX_LLF = 2 };
+enum { M2CX_MPREC_CONSTANT = 0, M2CX_MPREC_LINEAR = 1 };
+enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0 };
+enum { M2CX_RNORMAL_MESH = 1 };
+
+/* Initial guess of the interface Riemann solve. */
+enum { M2CX_GUESS_ACOUSTIC = 0,   /* FindInitialFeasiblePointsByAcousticTheory */
+       M2CX_GUESS_MLP = 1 };      /* learned p* guess, acoustic fallback       */
+
+/* Error codes */
+enum { M2CX_OK = 0, M2CX_E_ARG = -1, M2CX_E_HIP = -2, M2CX_E_OOM = -3,
+       M2CX_E_HYPERBOLICITY = -4, M2CX_E_RIEMANN_FATAL = -5, M2CX_E_UNRESOLVED = -6,
+       M2CX_E_UNSUPPORTED = -7, M2CX_E_CAPACITY = -8 };
+
+/* ------------------------------------------------------------------ */
+/* Problem description                                                 */
+/* ------------------------------------------------------------------ */
+
+/* One material: MaterialModelData (IoData.cpp:1040-1080) with the SG, NASG,
+ * MG and dummy parameter sets (VarFcnSG.h:129-144, VarFcnNASG.h:100-113,
+ * VarFcnMG.h:104-127, VarFcnDummy.h:23-26). */
+typedef struct {
+  int32_t eos;                 /* M2CX_EOS_*                                  */
+  int32_t reserved;
+  double gamma, pc;            /* SG and NASG: specific heat ratio, p_c        */
+  double b, q;                 /* NASG: covolume, energy constant              */
+  double rho0, c0, Gamma0, s, e0; /* MG                                        */
+  double rhomin, pmin, rhomax, pmax; /* clipping (DensityCutOff, ...)          */
+  double failsafe_density;
+  double dummy_rho, dummy_p, dummy_e;  /* VarFcnDummy state                    */
+} m2cx_material;
+
+/* Uniform Cartesian mesh (MeshGenerator::ComputeMesh1DUniform,
+ * MeshGenerator.cpp:161-173) and its boundary conditions. */
+typedef struct {
+  int32_t nx, ny, nz, reserved;
+  double x0, xmax, y0, ymax, z0, zmax;
+  int32_t bc[6];               /* x0, xmax, y0, ymax, z0, zmax: M2CX_BC_*      */
+  double inlet[5];             /* BoundaryConditions.Inlet  (rho,u,v,w,p)      */
+  double inlet2[5];            /* BoundaryConditions.Inlet2                    */
+  int32_t inlet_id, inlet2_id; /* their material IDs                           */
+} m2cx_grid;
+
+/* Numerical scheme: Space.NavierStokes, Reconstruction, Time, MultiPhase,
+ * ExactRiemannSolution and LevelSet[0] blocks (defaults: IoData.cpp:1265-1274,
+ * 1370-1375, 1513-1535, 1622-1639, 1694-1713, 1771-1852). */
+typedef struct {
+  int32_t flux;                /* M2CX_FLUX_*                                  */
+  int32_t rec_type;            /* M2CX_REC_*                                   */
+  int32_t limiter;             /* M2CX_LIM_*                                   */
+  int32_t var_type;            /* M2CX_VAR_PRIMITIVE                           */
+  int32_t slope_near_interface;/* M2CX_SLOPE_*                                 */
+  int32_t integrator;          /* M2CX_TI_*                                    */
+  double gmm_coeff;            /* GeneralizedMinModCoefficient (1.2)           */
+  double entropy_fix;          /* EntropyFixCoefficient (0.2)                  */
+  double cfl;                  /* CFL (0.5)                                    */
+  double max_time;             /* MaxTime; <= 0 means no clamp                 */
+  int32_t mp_flux;             /* M2CX_MPFLUX_*                                */
+  int32_t mp_recon;            /* M2CX_MPREC_*                                 */
+  int32_t mp_phase_change;     /* M2CX_PHASECHANGE_*                           */
+  int32_t riemann_normal;      /* M2CX_RNORMAL_MESH                            */
+  int32_t riem_maxits_main;    /* MaxIts (200)                                 */
+  int32_t riem_maxits_bracket; /* MaxItsBracketing (100)                       */
+  int32_t riem_maxits_shock;   /* MaxItsShock (200)                            */
+  int32_t riem_steps_rarefaction; /* IntegrationSteps (200)                    */
+  double riem_tol_main;        /* Tolerance (1e-5)                             */
+  double riem_tol_shock;       /* ToleranceShock (1e-12)                       */
+  double riem_tol_rarefaction; /* ToleranceRarefaction (1e-8)                  */
+  double riem_min_pressure;    /* -1e8                                         */
+  double riem_failure_threshold; /* 0.2                                        */
+  double riem_pressure_at_failure; /* 1e-8                                     */
+  int32_t n_levelsets;         /* 0 or 1                                       */
+  int32_t ls_materialid;       /* LevelSet[0].MaterialID                       */
+  int32_t ls_bc[6];            /* LevelSet[0].BoundaryCondition* (M2CX_LSBC_*) */
+  int32_t riem_guess;          /* M2CX_GUESS_*                                 */
+  int32_t reserved;
+} m2cx_scheme;
+
+/* Fields held on the device by a solver handle. */
+enum {
+  M2CX_FIELD_V = 0,   /* primitive state V (5 dof)            */
+  M2CX_FIELD_V1 = 1,  /* RK stage state V1                    */
+  M2CX_FIELD_V2 = 2,  /* RK stage state V2                    */
+  M2CX_FIELD_R = 3,   /* residual R (5 dof)                   */
+  M2CX_FIELD_PHI = 4, /* level set phi (1 dof)                */
+  M2CX_FIELD_ID = 5   /* material id (int32, 1 dof)           */
+};
+
+/* Counters accumulated since the last m2cx_reset_counters(). */
+typedef struct {
+  int64_t riemann_calls;       /* interface faces solved                        */
+  int64_t riemann_trivial;     /* faces that took the trivial branch (:134-139) */
+  int64_t riemann_errors;      /* nonzero returns (SpaceOperator.cpp:2031)      */
+  int64_t riemann_main_iters;  /* secant main-loop iterations, summed           */
+  int64_t riemann_fevals;      /* f(p) evaluations (pairs of ComputeRhoUStar)   */
+  int64_t riemann_guess_fallbacks; /* learned guess rejected -> acoustic        */
+  int64_t clipped_cells;       /* ClipDensityAndPressure counts                 */
+  int64_t id_changed_cells;    /* cells whose material ID changed               */
+  int64_t steps;               /* completed time steps                          */
+} m2cx_counters;
+
+typedef struct m2cx_solver m2cx_solver;
+
+/* ------------------------------------------------------------------ */
+/* Lifetime                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Replaces the construction in Main.cpp:85-491 (VarFcn, FluxFcn,
+ * ExactRiemannSolverBase, SpaceOperator, LevelSetOperator, MultiPhaseOperator,
+ * TimeIntegrator) for the explicit Euler + level-set path.  Allocates every
+ * device field; `device` selects the HIP device. */
+int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat,
+                const m2cx_scheme* scheme, int device, m2cx_solver** out);
+/* Replaces the Destroy() calls in Main.cpp:793-855. */
+int m2cx_destroy(m2cx_solver* h);
+/* Message of the last failure on this handle (empty string if none). */
+const char* m2cx_last_error(const m2cx_solver* h);
+
+/* ------------------------------------------------------------------ */
+/* State transfer (SpaceVariable3D GetDataPointer / Restore..., SpaceVariable.h:86-96) */
+/* ------------------------------------------------------------------ */
+
+/* Copies a host field (reference layout, see above) to the device field.
+ * For M2CX_FIELD_ID `host` points to int32.  Ghosts are copied as given. */
+int m2cx_upload(m2cx_solver* h, int field, const void* host);
+/* Copies a device field to the host (reference layout, ghosts included). */
+int m2cx_download(m2cx_solver* h, int field, void* host);
+/* Simulation time / step counter held on the device (Main.cpp:653-712). */
+int m2cx_set_time(m2cx_solver* h, double t, int time_step);
+int m2cx_get_time(m2cx_solver* h, double* t, double* dt, int* time_step);
+
+/* ------------------------------------------------------------------ */
+/* SpaceOperator (SpaceOperator.h:107-149)                             */
+/* ------------------------------------------------------------------ */
+
+/* SpaceOperator::ComputeResidual (SpaceOperator.h:144-149,
+ * SpaceOperator.cpp:2573-2641): R = -(sum of face fluxes)/vol on interior
+ * cells for state field `v_field` (V, V1 or V2), current ID and phi.
+ * Interface faces use ExactRiemannSolverBase (multi-material); when
+ * `capture_riemann` is nonzero the star states are kept for the end-of-step
+ * update, as RiemannSolutions (RiemannSolutions.h:12-30).  Returns the number
+ * of Riemann failures (>=0) or a negative error. */
+int m2cx_compute_residual(m2cx_solver* h, int v_field, int capture_riemann);
+
+/* SpaceOperator::ComputeTimeStepSize (SpaceOperator.h:135-136,
+ * SpaceOperator.cpp:1298-1330): dt = cfl * min over cells and active axes of
+ * dx_a/(|u_a|+c). Synchronises to return dt and cfl. */
+int m2cx_compute_time_step(m2cx_solver* h, double* dt, double* cfl);
+
+/* ConservativeToPrimitive / PrimitiveToConservative / ClipDensityAndPressure /
+ * ApplyBoundaryConditions (SpaceOperator.h:107-123) on a device field pair.
+ * m2cx_clip returns the number of clipped cells. */
+int m2cx_apply_boundary_conditions(m2cx_solver* h, int v_field);
+int m2cx_clip(m2cx_solver* h, int v_field);
+
+/* LevelSetOperator::ComputeResidual + ApplyBoundaryConditions
+ * (LevelSetOperator.h:80-107, LevelSetOperator.cpp:768-870, :408-508)
+ * for level set 0 with velocities from `v_field`; result in the device R_phi
+ * buffer (download with m2cx_download_ls_residual). */
+int m2cx_levelset_residual(m2cx_solver* h, int v_field);
+int m2cx_download_ls_residual(m2cx_solver* h, double* host);
+
+/* ------------------------------------------------------------------ */
+/* TimeIntegrator (TimeIntegrator.h:77-88, TimeIntegrator.cpp:167-843) */
+/* ------------------------------------------------------------------ */
+
+/* TimeIntegratorBase::AdvanceOneTimeStep for the scheme's FE/RK2/RK3 with a
+ * host-given dt (the caller did ComputeTimeStepSize), including the end-of-
+ * step material update (UpdateSolutionAfterTimeStepping, :720-843). `time`
+ * is the time at the END of the step, as Main.cpp:709-712 passes it. */
+int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_step);
+
+/* The Main.cpp:653-712 loop body `nsteps` times with dt computed on the
+ * device (CFL + MaxTime clamp) and no host synchronisation between steps.
+ * Stops early at max_time.  Returns Riemann failures seen, or an error. */
+int m2cx_run_steps(m2cx_solver* h, int nsteps);
+
+/* Counters (riemann iterations per face etc.). */
+int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out);
+int m2cx_reset_counters(m2cx_solver* h);
+
+/* ------------------------------------------------------------------ */
+/* Point kernels, batched (device does the work; host arrays in/out)   */
+/* ------------------------------------------------------------------ */
+
+/* FluxFcnBase::ComputeNumericalFluxAtCellInterface (FluxFcnBase.h:65-69) with
+ * the scheme's flux (FluxFcnHLLC.h:197-245, FluxFcnGenRoe.h:175-212,
+ * FluxFcnLLF.h:82-111) for n faces: Vm, Vp are n x 5, id n, F n x 5. */
+int m2cx_numerical_flux_batch(m2cx_solver* h, int n, int dir, const double* Vm,
+                              const double* Vp, const int32_t* id, double* F);
+
+/* ExactRiemannSolverBase::ComputeRiemannSolution (ExactRiemannSolverBase.h:42-47,
+ * ExactRiemannSolverBase.cpp:59-347) for n problems: dir n x 3, Vm/Vp n x 5,
+ * idm/idp n; outputs Vs n x 5, id n, Vsm/Vsp n x 5, err n, and per problem
+ * the secant main-loop iteration count and f(p) evaluation count. */
+int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir,
+                       const double* Vm, const int32_t* idm,
+                       const double* Vp, const int32_t* idp,
+                       double* Vs, int32_t* id, double* Vsm, double* Vsp,
+                       int32_t* err, int32_t* iters, int32_t* fevals);
+
+/* ------------------------------------------------------------------ */
+/* Measurement                                                         */
+/* ------------------------------------------------------------------ */
+
+/* When enabled, every launch of the named kernels is bracketed by HIP events on
+ * the handle's stream; m2cx_kernel_time returns the summed milliseconds and
+ * launch count for kernel `name` ("stage", "riemann", "dt", "levelset", ...). */
+int m2cx_profile(m2cx_solver* h, int enable);
+int m2cx_kernel_time(m2cx_solver* h, const char* name, double* ms, int64_t* launches);
+int m2cx_synchronize(m2cx_solver* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* M2C_AMD_H */

@@ -1,0 +1,132 @@
+"""ctypes mirror of include/m2c_amd.h (the C ABI of libm2c_amd.so).
+
+The structures and enumerations below follow the header field by field.  The
+defaults of `default_scheme()` are the reference's IoData defaults
+(IoData.cpp:1265-1274, 1370-1375, 1513-1535, 1622-1639, 1694-1713, 1771-1852).
+"""
+import ctypes as C
+
+EOS_SG, EOS_NASG, EOS_MG, EOS_DUMMY = 0, 1, 2, 8
+BC_NONE, BC_INLET, BC_INLET2, BC_SLIPWALL, BC_STICKWALL, BC_SYMMETRY, BC_OVERSET, BC_OUTLET = range(8)
+LSBC_ZERO_NEUMANN, LSBC_LINEAR_EXTRAPOLATION, LSBC_NON_NEGATIVE = 1, 2, 3
+FLUX_ROE, FLUX_LLF, FLUX_HLLC = 0, 1, 2
+REC_CONSTANT, REC_LINEAR = 0, 1
+LIM_NONE, LIM_GMM, LIM_VANALBADA = 0, 1, 2
+SLOPE_ZERO, SLOPE_NONZERO = 0, 1
+TI_FE, TI_RK2, TI_RK3 = 0, 1, 2
+MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
+MPREC_CONSTANT, MPREC_LINEAR = 0, 1
+GUESS_ACOUSTIC, GUESS_MLP = 0, 1
+FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
+
+DBL_MAX = 1.7976931348623157e308
+
+
+class Material(C.Structure):
+    _fields_ = [("eos", C.c_int32), ("reserved", C.c_int32),
+                ("gamma", C.c_double), ("pc", C.c_double), ("b", C.c_double), ("q", C.c_double),
+                ("rho0", C.c_double), ("c0", C.c_double), ("Gamma0", C.c_double), ("s", C.c_double),
+                ("e0", C.c_double),
+                ("rhomin", C.c_double), ("pmin", C.c_double), ("rhomax", C.c_double), ("pmax", C.c_double),
+                ("failsafe_density", C.c_double),
+                ("dummy_rho", C.c_double), ("dummy_p", C.c_double), ("dummy_e", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("reserved", C.c_int32),
+                ("x0", C.c_double), ("xmax", C.c_double), ("y0", C.c_double), ("ymax", C.c_double),
+                ("z0", C.c_double), ("zmax", C.c_double),
+                ("bc", C.c_int32 * 6), ("inlet", C.c_double * 5), ("inlet2", C.c_double * 5),
+                ("inlet_id", C.c_int32), ("inlet2_id", C.c_int32)]
+
+
+class Scheme(C.Structure):
+    _fields_ = [("flux", C.c_int32), ("rec_type", C.c_int32), ("limiter", C.c_int32), ("var_type", C.c_int32),
+                ("slope_near_interface", C.c_int32), ("integrator", C.c_int32),
+                ("gmm_coeff", C.c_double), ("entropy_fix", C.c_double), ("cfl", C.c_double),
+                ("max_time", C.c_double),
+                ("mp_flux", C.c_int32), ("mp_recon", C.c_int32), ("mp_phase_change", C.c_int32),
+                ("riemann_normal", C.c_int32),
+                ("riem_maxits_main", C.c_int32), ("riem_maxits_bracket", C.c_int32),
+                ("riem_maxits_shock", C.c_int32), ("riem_steps_rarefaction", C.c_int32),
+                ("riem_tol_main", C.c_double), ("riem_tol_shock", C.c_double),
+                ("riem_tol_rarefaction", C.c_double), ("riem_min_pressure", C.c_double),
+                ("riem_failure_threshold", C.c_double), ("riem_pressure_at_failure", C.c_double),
+                ("n_levelsets", C.c_int32), ("ls_materialid", C.c_int32), ("ls_bc", C.c_int32 * 6),
+                ("riem_guess", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "riemann_calls", "riemann_trivial", "riemann_errors", "riemann_main_iters", "riemann_fevals",
+        "riemann_guess_fallbacks", "clipped_cells", "id_changed_cells", "steps")]
+
+
+def material(eos="SG", **kw):
+    """MaterialModelData with the reference defaults (IoData.cpp:1040-1053)."""
+    m = Material()
+    m.eos = {"SG": EOS_SG, "NASG": EOS_NASG, "MG": EOS_MG, "DUMMY": EOS_DUMMY}[eos]
+    m.gamma, m.pc = 1.4, 0.0
+    m.rhomin, m.pmin, m.rhomax, m.pmax = 0.0, -DBL_MAX, DBL_MAX, DBL_MAX
+    m.failsafe_density = 1.0e-10
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def materials(*mats):
+    arr = (Material * len(mats))()
+    for i, m in enumerate(mats):
+        arr[i] = m
+    return arr
+
+
+def grid(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), bc=(BC_OUTLET,) * 6, inlet=None, inlet2=None,
+         inlet_id=0, inlet2_id=0):
+    g = Grid()
+    g.nx, g.ny, g.nz = n
+    g.x0, g.y0, g.z0 = lo
+    g.xmax, g.ymax, g.zmax = hi
+    for i in range(6):
+        g.bc[i] = bc[i]
+    for i in range(5):
+        g.inlet[i] = 0.0 if inlet is None else inlet[i]
+        g.inlet2[i] = 0.0 if inlet2 is None else inlet2[i]
+    g.inlet_id, g.inlet2_id = inlet_id, inlet2_id
+    return g
+
+
+def default_scheme(**kw):
+    s = Scheme()
+    s.flux = FLUX_HLLC
+    s.rec_type = REC_LINEAR
+    s.limiter = LIM_GMM
+    s.var_type = 0
+    s.slope_near_interface = SLOPE_NONZERO
+    s.integrator = TI_RK2
+    s.gmm_coeff = 1.2
+    s.entropy_fix = 0.2
+    s.cfl = 0.5
+    s.max_time = -1.0
+    s.mp_flux = MPFLUX_NUMERICAL
+    s.mp_recon = MPREC_CONSTANT
+    s.mp_phase_change = 0
+    s.riemann_normal = 1
+    s.riem_maxits_main = 200
+    s.riem_maxits_bracket = 100
+    s.riem_maxits_shock = 200
+    s.riem_steps_rarefaction = 200
+    s.riem_tol_main = 1.0e-5
+    s.riem_tol_shock = 1.0e-12
+    s.riem_tol_rarefaction = 1.0e-8
+    s.riem_min_pressure = -1.0e8
+    s.riem_failure_threshold = 0.2
+    s.riem_pressure_at_failure = 1.0e-8
+    s.n_levelsets = 0
+    s.ls_materialid = 1
+    for i in range(6):
+        s.ls_bc[i] = LSBC_ZERO_NEUMANN
+    s.riem_guess = GUESS_ACOUSTIC
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s

@@ -1,0 +1,199 @@
+// Oracle (test infrastructure only): extern "C" entry points used by tests/
+// (through ctypes) and by bench.py's cpu_baseline leg.  Never linked into the
+// product library.
+#include <cstring>
+#include <memory>
+#include "solver.h"
+
+using namespace orc;
+
+struct orc_solver {
+  std::unique_ptr<Solver> s;
+};
+
+static void copy_field(const std::vector<double>& src, double* dst) { std::memcpy(dst, src.data(), src.size() * 8); }
+
+extern "C" {
+
+orc_solver* orc_create(const m2cx_grid* g, const m2cx_material* mats, int nmat, const m2cx_scheme* sch) {
+  auto* h = new orc_solver;
+  h->s.reset(new Solver(*g, mats, nmat, *sch));
+  return h;
+}
+
+void orc_destroy(orc_solver* h) { delete h; }
+
+const char* orc_error(orc_solver* h) { return h->s->msg.c_str(); }
+
+long orc_ncells(orc_solver* h) { return h->s->m.ncells(); }
+
+int orc_set_state(orc_solver* h, const double* V, const int* ID, const double* Phi) {
+  Solver& s = *h->s;
+  long n = s.m.ncells();
+  if (V) std::memcpy(s.V.data(), V, n * 5 * 8);
+  if (ID) std::memcpy(s.ID.data(), ID, n * 4);
+  if (Phi) std::memcpy(s.Phi.data(), Phi, n * 8);
+  return 0;
+}
+
+int orc_get_state(orc_solver* h, double* V, int* ID, double* Phi) {
+  Solver& s = *h->s;
+  long n = s.m.ncells();
+  if (V) copy_field(s.V, V);
+  if (ID) std::memcpy(ID, s.ID.data(), n * 4);
+  if (Phi) copy_field(s.Phi, Phi);
+  return 0;
+}
+
+int orc_get_field(orc_solver* h, const char* name, double* out) {
+  Solver& s = *h->s;
+  std::vector<double>* f = nullptr;
+  if (!strcmp(name, "V")) f = &s.V;
+  else if (!strcmp(name, "V1")) f = &s.V1;
+  else if (!strcmp(name, "V2")) f = &s.V2;
+  else if (!strcmp(name, "R")) f = &s.R;
+  else if (!strcmp(name, "U1")) f = &s.U1;
+  else if (!strcmp(name, "Un")) f = &s.Un;
+  else if (!strcmp(name, "Vl")) f = &s.Vl;
+  else if (!strcmp(name, "Vr")) f = &s.Vr;
+  else if (!strcmp(name, "Vb")) f = &s.Vb;
+  else if (!strcmp(name, "Vt")) f = &s.Vt;
+  else if (!strcmp(name, "Vk")) f = &s.Vk;
+  else if (!strcmp(name, "Vf")) f = &s.Vf;
+  else if (!strcmp(name, "Phi")) f = &s.Phi;
+  else if (!strcmp(name, "Phi1")) f = &s.Phi1;
+  else if (!strcmp(name, "Rls")) f = &s.Rls;
+  else if (!strcmp(name, "vol")) f = &s.vol;
+  if (!f) return -1;
+  copy_field(*f, out);
+  return 0;
+}
+
+// SpaceOperator::ComputeResidual on the current V; R written to `R` (full
+// ghosted brick, 5 dof).
+int orc_residual(orc_solver* h, double* R, int capture) {
+  Solver& s = *h->s;
+  s.ComputeResidual(s.V, s.R, capture != 0);
+  if (s.fatal) return -1;
+  if (R) copy_field(s.R, R);
+  return (int)s.cnt.riemann_errors;
+}
+
+int orc_apply_bc(orc_solver* h) { h->s->ApplyBoundaryConditions(h->s->V); return 0; }
+int orc_update_ghost_ids(orc_solver* h) { h->s->UpdateMaterialIDAtGhostNodes(); return 0; }
+int orc_ls_apply_bc(orc_solver* h) { h->s->LSApplyBoundaryConditions(h->s->Phi); return 0; }
+
+int orc_ls_residual(orc_solver* h, double* out) {
+  Solver& s = *h->s;
+  s.LSComputeResidual(s.V, s.Phi, s.Rls);
+  if (out) copy_field(s.Rls, out);
+  return 0;
+}
+
+int orc_time_step(orc_solver* h, double* dt, double* cfl) {
+  h->s->ComputeTimeStepSize(h->s->V, *dt, *cfl);
+  return h->s->fatal ? -1 : 0;
+}
+
+int orc_advance_one(orc_solver* h, double time, double dt) {
+  Solver& s = *h->s;
+  s.time_step++;
+  s.AdvanceOneTimeStep(time, dt);
+  return s.fatal ? -1 : 0;
+}
+
+int orc_run_steps(orc_solver* h, int nsteps) { return h->s->RunSteps(nsteps); }
+
+int orc_get_time(orc_solver* h, double* t, double* dt, int* step) {
+  *t = h->s->t;
+  *dt = h->s->dt_last;
+  *step = h->s->time_step;
+  return 0;
+}
+
+int orc_set_time(orc_solver* h, double t, int step) {
+  h->s->t = t;
+  h->s->time_step = step;
+  return 0;
+}
+
+// counters: calls, trivial, errors, iters, fevals, fallbacks, clipped, id_changed, steps
+int orc_counters(orc_solver* h, long* out) {
+  Counters& c = h->s->cnt;
+  long v[9] = {c.riemann_calls, c.riemann_trivial, c.riemann_errors, c.riemann_iters, c.riemann_fevals,
+               c.riemann_fallbacks, c.clipped, c.id_changed, c.steps};
+  std::memcpy(out, v, sizeof v);
+  return 0;
+}
+
+// Point kernels -----------------------------------------------------------
+
+int orc_flux_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, int dir, const double* Vm,
+                   const double* Vp, const int* id, double* F) {
+  std::vector<Eos> vf;
+  for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
+  Flux fl;
+  fl.vf = &vf;
+  fl.kind = sch->flux;
+  fl.del = sch->entropy_fix;
+  for (int q = 0; q < n; q++) fl.Numerical(dir, Vm + 5 * q, Vp + 5 * q, id[q], F + 5 * q);
+  return 0;
+}
+
+int orc_riemann_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, const double* dir,
+                      const double* Vm, const int* idm, const double* Vp, const int* idp, double* Vs, int* id,
+                      double* Vsm, double* Vsp, int* err, int* iters, int* fevals, int* flags) {
+  std::vector<Eos> vf;
+  for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
+  ExactRiemann rs(vf, RiemannParams::from(*sch));
+  for (int q = 0; q < n; q++) {
+    rs.st = RiemannStats();
+    int mid = -1;
+    err[q] = rs.Solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q,
+                      Vsp + 5 * q);
+    id[q] = mid;
+    if (iters) iters[q] = rs.st.it_main;
+    if (fevals) fevals[q] = rs.st.fevals;
+    if (flags) flags[q] = (rs.st.trivial ? 1 : 0) | (rs.st.fallback ? 2 : 0) | (rs.st.fatal ? 4 : 0);
+  }
+  return 0;
+}
+
+// One Riemann problem in the PRINT_RIEMANN_SOLUTION build: fills `rows`
+// (xi, rho, u, p, id) up to `cap` rows and returns the row count.
+int orc_riemann_print(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, const double* Vm, int idm,
+                      const double* Vp, int idp, double* rows, int cap, double* star, int nonadaptive) {
+  std::vector<Eos> vf;
+  for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
+  ExactRiemann rs(vf, RiemannParams::from(*sch));
+  rs.print_mode = true;
+  rs.nonadaptive = nonadaptive != 0;
+  double dir[3] = {1.0, 0.0, 0.0};
+  double Vs[5], Vsm[5], Vsp[5];
+  int mid = -1;
+  rs.Solve(dir, Vm, idm, Vp, idp, Vs, mid, Vsm, Vsp);
+  int nr = (int)rs.sol1d.size();
+  for (int r = 0; r < nr && r < cap; r++)
+    for (int c = 0; c < 5; c++) rows[r * 5 + c] = rs.sol1d[r][c];
+  if (star) {
+    star[0] = Vsm[4]; star[1] = Vsm[1]; star[2] = Vsm[0]; star[3] = Vsp[0];
+  }
+  return nr;
+}
+
+// Wave curve (ComputeRhoUStar) from a fresh path, for closed-form tests.
+int orc_wave(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int wave, double rho, double u, double p,
+             double ps, int idd, double rhos0, double rhos1, double* rhos, double* us) {
+  std::vector<Eos> vf;
+  for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
+  ExactRiemann rs(vf, RiemannParams::from(*sch));
+  ExactRiemann::Path path;
+  path.push_back({p, rho, u});
+  return rs.RhoUStar(wave, path, rho, u, p, ps, idd, rhos0, rhos1, *rhos, *us) ? 1 : 0;
+}
+
+double orc_cr_root(double x, int n) { return m2c::cr_root(x, n); }
+
+void orc_set_ck_textbook(int on) { orc::g_ck_textbook = on != 0; }
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Oracle (test infrastructure only): CPU restatement of the two-material exact
+// Riemann solver, ExactRiemannSolverBase (ExactRiemannSolverBase.h:21-158,
+// ExactRiemannSolverBase.cpp:59-1896).  The adaptive solver is restated in
+// full: safeguarded secant main loop (:219-311), bracketing (:675-839,
+// :1003-1158), wave curves (:1213-1624) with the adaptive Cash-Karp
+// rarefaction integrator and its cached integration path (:1245-1262,
+// :1773-1896), and the xi = 0 sampling (:352-558).  `print_mode` reproduces the
+// PRINT_RIEMANN_SOLUTION build that wrote Tests/RiemannProblems/*/exact.
+// The non-adaptive fallback class (:2243-2989) is restated in riemann.cpp.
+#pragma once
+#include <array>
+#include <vector>
+#include <algorithm>
+#include <cstdint>
+#include "eos.h"
+#include "toms748.h"
+#include "../../m2c_amd/csrc/m2c_crmath.h"
+
+namespace orc {
+
+// Test-only switch: evaluate the third Cash-Karp stage at p0 + 3/10 dp (the
+// published tableau) instead of the reference's p1 + 3/10 dp = p0 + dp/2
+// (ExactRiemannSolverBase.cpp:1814).  Off by default; used only to pin the
+// rest of the integrator against closed-form isentropes.
+extern bool g_ck_textbook;
+
+struct RiemannParams {
+  int maxIts_main = 200, maxIts_bracket = 100, maxIts_shock = 200, numSteps_rarefaction = 200;
+  double tol_main = 1e-5, tol_shock = 1e-12, tol_rarefaction = 1e-8;
+  double min_pressure = -1e8, failure_threshold = 0.2, pressure_at_failure = 1e-8;
+  static RiemannParams from(const m2cx_scheme& s) {
+    RiemannParams p;
+    p.maxIts_main = s.riem_maxits_main; p.maxIts_bracket = s.riem_maxits_bracket;
+    p.maxIts_shock = s.riem_maxits_shock; p.numSteps_rarefaction = s.riem_steps_rarefaction;
+    p.tol_main = s.riem_tol_main; p.tol_shock = s.riem_tol_shock; p.tol_rarefaction = s.riem_tol_rarefaction;
+    p.min_pressure = s.riem_min_pressure; p.failure_threshold = s.riem_failure_threshold;
+    p.pressure_at_failure = s.riem_pressure_at_failure;
+    return p;
+  }
+};
+
+struct RiemannStats {
+  int it_main = 0;     // secant main-loop iterations
+  int fevals = 0;      // f(p) evaluations (1-wave calls outside FinalizeSolution)
+  bool trivial = false;
+  bool fallback = false;  // the non-adaptive solver was invoked
+  bool fatal = false;     // the reference would have exited
+};
+
+class ExactRiemann {
+ public:
+  using Path = std::vector<std::array<double, 3>>;  // (p, rho, u)
+  const std::vector<Eos>& vf;
+  RiemannParams P;
+  bool nonadaptive = false;   // ExactRiemannSolverNonAdaptive behaviour
+  bool print_mode = false;
+  std::vector<std::array<double, 5>> sol1d;  // (xi, rho, u, p, id) in print mode
+  RiemannStats st;
+
+  ExactRiemann(const std::vector<Eos>& v, const RiemannParams& p) : vf(v), P(p) {}
+
+  int Solve(const double* dir, const double* Vm, int idl, const double* Vp, int idr, double* Vs, int& id,
+            double* Vsm, double* Vsp);
+
+  // exposed for tests
+  bool RhoUStar(int wave, Path& path, double rho, double u, double p, double ps, int idd, double rhos0,
+                double rhos1, double& rhos, double& us, bool* trans_rare = nullptr, double* Vrare = nullptr);
+
+ private:
+  Path path1, path3;
+  bool RK45(int wave, int idd, double rho_0, double u_0, double p_0, double dp, double& rho, double& u,
+            double& p, double& xi, double& uErr, double& rhoErr);
+  bool RK4(int wave, int idd, double rho_0, double u_0, double p_0, double dp, double& rho, double& u,
+           double& p, double& xi);
+  bool RhoUStarNonAdaptive(int wave, Path& path, double rho, double u, double p, double ps, int idd,
+                           double rhos0, double rhos1, double& rhos, double& us, bool* trans_rare,
+                           double* Vrare);
+  bool Shock(double rho, double u, double p, double ps, int idd, double rhos0, double rhos1, int wave,
+             double& rhos, double& us);
+  bool FindInitialInterval(double rhol, double ul, double pl, double el, double cl, int idl, double rhor,
+                           double ur, double pr, double er, double cr, int idr, double& p0, double& rhol0,
+                           double& rhor0, double& ul0, double& ur0, double& p1, double& rhol1, double& rhor1,
+                           double& ul1, double& ur1);
+  bool FindInitialFeasiblePoints(double rhol, double ul, double pl, double el, double cl, int idl, double rhor,
+                                 double ur, double pr, double er, double cr, int idr, double& p0, double& rhol0,
+                                 double& rhor0, double& ul0, double& ur0, double& p1, double& rhol1,
+                                 double& rhor1, double& ul1, double& ur1);
+  int Acoustic(double rhol, double ul, double pl, double cl, int idl, double rhor, double ur, double pr, double cr,
+               int idr, double& p0, double& rhol0, double& rhor0, double& ul0, double& ur0, double& p1,
+               double& rhol1, double& rhor1, double& ul1, double& ur1);
+  void Finalize(const double* dir, const double* Vm, const double* Vp, double rhol, double ul, double pl, int idl,
+                double rhor, double ur, double pr, int idr, double rhol2, double rhor2, double u2, double p2,
+                bool trans_rare, const double* Vrare, double* Vs, int& id, double* Vsm, double* Vsp);
+  bool Wave(int wave, Path& path, double rho, double u, double p, double ps, int idd, double rhos0, double rhos1,
+            double& rhos, double& us, bool* tr = nullptr, double* Vr = nullptr, bool count = true) {
+    if (count && wave == 1) st.fevals++;
+    return nonadaptive ? RhoUStarNonAdaptive(wave, path, rho, u, p, ps, idd, rhos0, rhos1, rhos, us, tr, Vr)
+                       : RhoUStar(wave, path, rho, u, p, ps, idd, rhos0, rhos1, rhos, us, tr, Vr);
+  }
+  void Push(double xi, double rho, double u, double p, int idd) {
+    if (print_mode) sol1d.push_back({xi, rho, u, p, (double)idd});
+  }
+};
+
+}  // namespace orc

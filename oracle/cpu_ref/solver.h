@@ -1,0 +1,108 @@
+// Oracle (test infrastructure only): CPU restatement of the explicit
+// finite-volume step on one rank (no domain decomposition: the reference's
+// PETSc halo exchanges are identities on a single subdomain).
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+#include <array>
+#include "eos.h"
+#include "flux.h"
+#include "riemann.h"
+
+namespace orc {
+
+using V5 = std::array<double, 5>;
+
+struct Mesh {
+  int NX, NY, NZ, PX, PY, PZ;
+  std::vector<double> x, y, z, dx, dy, dz;  // indexed i+1 (ghosts included)
+  long idx(int i, int j, int k) const { return ((long)(k + 1) * PY + (j + 1)) * PX + (i + 1); }
+  long ncells() const { return (long)PX * PY * PZ; }
+  bool inside(int i, int j, int k) const { return i >= 0 && i < NX && j >= 0 && j < NY && k >= 0 && k < NZ; }
+  double X(int i) const { return x[i + 1]; }
+  double Y(int j) const { return y[j + 1]; }
+  double Z(int k) const { return z[k + 1]; }
+  double DX(int i) const { return dx[i + 1]; }
+  double DY(int j) const { return dy[j + 1]; }
+  double DZ(int k) const { return dz[k + 1]; }
+};
+
+// A "face" ghost of the outer ghost layer (GhostPoint with type FACE,
+// SpaceOperator.cpp:472-593).
+struct Ghost {
+  int i, j, k, ii, jj, kk, side, bc, lsbc;  // side: 0 L,1 R,2 B,3 T,4 K,5 F
+};
+
+struct RiemannStore {  // RiemannSolutions.h:12-30, keyed by cell linear index
+  std::map<long, std::pair<V5, int>> left, right, bottom, top, back, front;
+  void clear() { left.clear(); right.clear(); bottom.clear(); top.clear(); back.clear(); front.clear(); }
+};
+
+struct Counters {
+  long riemann_calls = 0, riemann_trivial = 0, riemann_errors = 0, riemann_iters = 0, riemann_fevals = 0;
+  long riemann_fallbacks = 0, clipped = 0, id_changed = 0, steps = 0;
+};
+
+class Solver {
+ public:
+  Mesh m;
+  m2cx_grid grid;
+  m2cx_scheme sch;
+  std::vector<Eos> vf;
+  int nmat, inactive_id;
+  Flux flux;
+  ExactRiemann riem;
+  // fields (AoS like SpaceVariable3D, ghost width 1)
+  std::vector<double> V, V1, V2, Un, U1, R, Vl, Vr, Vb, Vt, Vk, Vf, vol;
+  std::vector<double> A3, B3;  // Reconstructor CoeffA/CoeffB (3 per cell)
+  std::vector<int> K3;         // CoeffK
+  std::vector<int> ID, IDn;
+  std::vector<double> Phi, Phi1, Phi_tmp, Rls;
+  std::vector<double> cxL, cxR, cyL, cyR, czL, czR;  // FD3 coefficients, 4 per cell
+  std::vector<Ghost> ghosts;
+  RiemannStore rs;
+  Counters cnt;
+  double t = 0.0, dt_last = 0.0, dts = 0.0;
+  int time_step = 0;
+  bool fatal = false;
+  std::string msg;
+
+  Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat, const m2cx_scheme& s);
+
+  void Fail(const std::string& s) { if (!fatal) { fatal = true; msg = s; } }
+
+  // SpaceOperator
+  void Reconstruct(const std::vector<double>& Vin);
+  void CheckReconstructedStates(const std::vector<double>& Vin);
+  void ComputeAdvectionFluxes(const std::vector<double>& Vin, std::vector<double>& F, bool capture);
+  void ComputeResidual(const std::vector<double>& Vin, std::vector<double>& Rout, bool capture);
+  void ConservativeToPrimitive(const std::vector<double>& U, std::vector<double>& Vout);
+  void PrimitiveToConservative(const std::vector<double>& Vin, std::vector<double>& U);
+  int ClipDensityAndPressure(std::vector<double>& Vio);
+  void ApplyBoundaryConditions(std::vector<double>& Vio);
+  void ComputeTimeStepSize(const std::vector<double>& Vin, double& dt, double& cfl);
+  // LevelSetOperator
+  void LSApplyBoundaryConditions(std::vector<double>& phi);
+  void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
+  // MultiPhaseOperator
+  void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
+  void UpdateMaterialIDAtGhostNodes();
+  int UpdateStateVariablesByRiemannSolutions();
+  // TimeIntegrator
+  void AdvanceOneTimeStep(double time, double dt);
+  void UpdateSolutionAfterTimeStepping();
+  // Main.cpp:653-712
+  int RunSteps(int nsteps);
+
+ private:
+  void SetupMesh();
+  void SetupGhosts();
+  void SetupFD3();
+  int Riemann(int dir, const double* Vm, int idm, const double* Vp, int idp, double* Vs, int& mid, double* Vsm,
+              double* Vsp);
+  bool LocalUpdate(int i, int j, int k, int id, bool upwind);
+  void FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved);
+};
+
+}  // namespace orc

@@ -1,0 +1,125 @@
+"""Turn a parsed reference input deck (tests/golden/riemann_problems.json) into
+ABI structures and the initial fields, as Main.cpp:85-382 and
+SpaceInitializer.cpp:98-482,1390-1514 set them up (uniform mesh, plane
+geometries, default state = Inlet, level set by signed distance)."""
+import numpy as np
+
+from m2c_amd import abi
+
+BC_NAMES = {"Inlet": abi.BC_INLET, "Farfield": abi.BC_INLET, "Inlet2": abi.BC_INLET2,
+            "SlipWall": abi.BC_SLIPWALL, "StickWall": abi.BC_STICKWALL, "Symmetry": abi.BC_SYMMETRY,
+            "Outlet": abi.BC_OUTLET}
+FLUX = {"Roe": abi.FLUX_ROE, "LocalLaxFriedrichs": abi.FLUX_LLF, "HLLC": abi.FLUX_HLLC}
+LIM = {"None": abi.LIM_NONE, "GeneralizedMinMod": abi.LIM_GMM, "VanAlbada": abi.LIM_VANALBADA}
+TI = {"ForwardEuler": abi.TI_FE, "RungeKutta2": abi.TI_RK2, "RungeKutta3": abi.TI_RK3}
+
+
+def _mat(md):
+    eos = md.get("EquationOfState", "StiffenedGas")
+    if eos == "StiffenedGas":
+        sg = md.get("StiffenedGasModel", {})
+        return abi.material("SG", gamma=sg.get("SpecificHeatRatio", 1.4), pc=sg.get("PressureConstant", 0.0))
+    if eos == "NobleAbelStiffenedGas":
+        sg = md.get("NobleAbelStiffenedGasModel", {})
+        return abi.material("NASG", gamma=sg.get("SpecificHeatRatio", 1.4), pc=sg.get("PressureConstant", 0.0),
+                            b=sg.get("VolumeConstant", 0.0), q=sg.get("EnergyConstant", 0.0))
+    if eos == "MieGruneisen":
+        mg = md.get("MieGruneisenModel", {})  # defaults IoData.cpp:708-724
+        return abi.material("MG", rho0=mg.get("ReferenceDensity", 8.96e-3), c0=mg.get("BulkSpeedOfSound", 3.933e6),
+                            Gamma0=mg.get("ReferenceGamma", 1.99), s=mg.get("HugoniotSlope", 1.5),
+                            e0=mg.get("ReferenceSpecificInternalEnergy", 0.0))
+    raise NotImplementedError(eos)
+
+
+def supported_eos(deck):
+    mats = deck["Equations"]
+    return all(mats[k].get("EquationOfState", "StiffenedGas") in ("StiffenedGas", "MieGruneisen",
+                                                                   "NobleAbelStiffenedGas") for k in mats)
+
+
+def materials(deck):
+    mats = deck["Equations"]
+    keys = sorted(mats, key=lambda k: int(k.split("[")[1][:-1]))
+    return abi.materials(*[_mat(mats[k]) for k in keys]), len(keys)
+
+
+def state(sd):
+    return [sd.get("Density", 1.0), sd.get("VelocityX", 0.0), sd.get("VelocityY", 0.0),
+            sd.get("VelocityZ", 0.0), sd.get("Pressure", 1.0)], int(sd.get("MaterialID", 0))
+
+
+def setup(deck):
+    """Returns (grid, mats, nmat, scheme, V, ID, Phi) with V/ID/Phi in the
+    reference's ghosted AoS layout."""
+    me = deck["Mesh"]
+    n = (int(me["NumberOfCellsX"]), int(me["NumberOfCellsY"]), int(me["NumberOfCellsZ"]))
+    bcs = [BC_NAMES[me.get(k, "Outlet")] for k in ("BoundaryConditionX0", "BoundaryConditionXmax",
+                                                   "BoundaryConditionY0", "BoundaryConditionYmax",
+                                                   "BoundaryConditionZ0", "BoundaryConditionZmax")]
+    inlet, inlet_id = state(deck.get("BoundaryConditions", {}).get("Inlet", {}))
+    inlet2, inlet2_id = state(deck.get("BoundaryConditions", {}).get("Inlet2", {}))
+    g = abi.grid(n, (me["X0"], me["Y0"], me["Z0"]), (me["Xmax"], me["Ymax"], me["Zmax"]), bcs, inlet, inlet2,
+                 inlet_id, inlet2_id)
+    mats, nmat = materials(deck)
+    ns = deck["Space"]["NavierStokes"]
+    rec = ns.get("Reconstruction", {})
+    mp = deck.get("MultiPhase", {})
+    ti = deck["Time"]
+    s = abi.default_scheme()
+    s.flux = FLUX[ns.get("Flux", "HLLC")]
+    s.rec_type = abi.REC_LINEAR if rec.get("Type", "Linear") == "Linear" else abi.REC_CONSTANT
+    s.limiter = LIM[rec.get("Limiter", "GeneralizedMinMod")]
+    s.gmm_coeff = rec.get("GeneralizedMinModCoefficient", 1.2)
+    s.slope_near_interface = abi.SLOPE_ZERO if rec.get("SlopeNearInterface", "NonZero") == "Zero" else abi.SLOPE_NONZERO
+    s.entropy_fix = ns.get("EntropyFixCoefficient", 0.2)
+    s.integrator = TI[ti.get("Explicit", {}).get("Type", "RungeKutta2")]
+    s.cfl = ti.get("CFL", 0.5)
+    s.max_time = ti.get("MaxTime", -1.0)
+    s.mp_flux = {"Exact": abi.MPFLUX_EXACT, "Numerical": abi.MPFLUX_NUMERICAL,
+                 "LocalLaxFriedrichs": abi.MPFLUX_LLF}[mp.get("Flux", "Numerical")]
+    s.mp_recon = abi.MPREC_LINEAR if mp.get("ReconstructionAtInterface", "Constant") == "Linear" else abi.MPREC_CONSTANT
+    ls = [k for k in deck["Space"] if k.startswith("LevelSet")]
+    s.n_levelsets = len(ls)
+    if ls:
+        s.ls_materialid = int(deck["Space"][ls[0]]["MaterialID"])
+    # initial condition
+    px, py, pz = n[0] + 2, n[1] + 2, n[2] + 2
+    V = np.zeros((pz, py, px, 5))
+    ID = np.zeros((pz, py, px), dtype=np.int32)
+    V[...] = inlet
+    ID[...] = inlet_id
+    x = _axis(n[0], me["X0"], me["Xmax"])
+    y = _axis(n[1], me["Y0"], me["Ymax"])
+    z = _axis(n[2], me["Z0"], me["Zmax"])
+    diag = np.sqrt((me["Xmax"] - me["X0"]) ** 2 + (me["Ymax"] - me["Y0"]) ** 2 + (me["Zmax"] - me["Z0"]) ** 2)
+    Phi = np.full((pz, py, px), diag)
+    geo = deck.get("InitialCondition", {}).get("GeometricEntities", {})
+    X, Y, Z = np.meshgrid(x[1:-1], y[1:-1], z[1:-1], indexing="ij")
+    X, Y, Z = X.transpose(2, 1, 0), Y.transpose(2, 1, 0), Z.transpose(2, 1, 0)
+    for key in sorted(geo):
+        if not key.startswith("Plane"):
+            raise NotImplementedError(key)
+        pl = geo[key]
+        x0 = np.array([pl.get("Point_x", 0.0), pl.get("Point_y", 0.0), pl.get("Point_z", 0.0)])
+        d = np.array([pl.get("Normal_x", 0.0), pl.get("Normal_y", 0.0), pl.get("Normal_z", 0.0)])
+        d = d * (1.0 / np.sqrt(d @ d))
+        dist = (X - x0[0]) * d[0] + (Y - x0[1]) * d[1] + (Z - x0[2]) * d[2]
+        st, mid = state(pl["InitialState"])
+        sel = dist > 0
+        V[1:-1, 1:-1, 1:-1][sel] = st
+        ID[1:-1, 1:-1, 1:-1][sel] = mid
+        if ls and mid == s.ls_materialid:
+            inner = Phi[1:-1, 1:-1, 1:-1]
+            upd = np.abs(dist) < np.abs(inner)
+            inner[upd] = -dist[upd]
+    return g, mats, nmat, s, V, ID, Phi
+
+
+def _axis(N, a0, a1):
+    h = (a1 - a0) / N
+    c = np.empty(N + 2)
+    for i in range(N):
+        c[i + 1] = a0 + 0.5 * h + i * h
+    c[0] = c[1] - h
+    c[N + 1] = c[N] + h
+    return c

@@ -1,0 +1,197 @@
+"""ctypes binding of oracle/liboracle.so, the CPU restatement of the
+reference's explicit step.  Test infrastructure only: used as the checker."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from m2c_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_lib = None
+P = C.c_void_p
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_error.restype = C.c_char_p
+        L.orc_error.argtypes = [P]
+        L.orc_ncells.restype = C.c_long
+        L.orc_ncells.argtypes = [P]
+        L.orc_set_state.argtypes = [P, D, I, D]
+        L.orc_get_state.argtypes = [P, D, I, D]
+        L.orc_get_field.argtypes = [P, C.c_char_p, D]
+        L.orc_residual.argtypes = [P, D, C.c_int]
+        L.orc_apply_bc.argtypes = [P]
+        L.orc_update_ghost_ids.argtypes = [P]
+        L.orc_ls_apply_bc.argtypes = [P]
+        L.orc_ls_residual.argtypes = [P, D]
+        L.orc_time_step.argtypes = [P, D, D]
+        L.orc_advance_one.argtypes = [P, C.c_double, C.c_double]
+        L.orc_run_steps.argtypes = [P, C.c_int]
+        L.orc_get_time.argtypes = [P, D, D, I]
+        L.orc_set_time.argtypes = [P, C.c_double, C.c_int]
+        L.orc_counters.argtypes = [P, C.POINTER(C.c_long)]
+        L.orc_flux_batch.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, C.c_int,
+                                     D, D, I, D]
+        L.orc_riemann_batch.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, D, D, I,
+                                        D, I, D, I, D, D, I, I, I, I]
+        L.orc_riemann_print.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), D, C.c_int, D,
+                                        C.c_int, D, C.c_int, D, C.c_int]
+        L.orc_wave.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, C.c_double,
+                               C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, D, D]
+        L.orc_cr_root.restype = C.c_double
+        L.orc_cr_root.argtypes = [C.c_double, C.c_int]
+        L.orc_set_ck_textbook.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    return a.ctypes.data_as(I)
+
+
+class Oracle:
+    """One CPU restatement instance on a grid (single rank)."""
+
+    def __init__(self, grid, mats, nmat, scheme):
+        self.L = lib()
+        self.grid, self.mats, self.nmat, self.scheme = grid, mats, nmat, scheme
+        self.h = self.L.orc_create(C.byref(grid), mats, nmat, C.byref(scheme))
+        self.shape = (grid.nz + 2, grid.ny + 2, grid.nx + 2)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def error(self):
+        return self.L.orc_error(self.h).decode()
+
+    def set_state(self, V=None, ID=None, Phi=None):
+        V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        ID = None if ID is None else np.ascontiguousarray(ID, dtype=np.int32)
+        Phi = None if Phi is None else np.ascontiguousarray(Phi, dtype=np.float64)
+        self.L.orc_set_state(self.h, None if V is None else dp(V), None if ID is None else ip(ID),
+                             None if Phi is None else dp(Phi))
+
+    def get_state(self):
+        V = np.zeros(self.shape + (5,))
+        ID = np.zeros(self.shape, dtype=np.int32)
+        Phi = np.zeros(self.shape)
+        self.L.orc_get_state(self.h, dp(V), ip(ID), dp(Phi))
+        return V, ID, Phi
+
+    def field(self, name):
+        dof = 1 if name in ("Phi", "Phi1", "Rls", "vol") else 5
+        out = np.zeros(self.shape + ((dof,) if dof > 1 else ()))
+        assert self.L.orc_get_field(self.h, name.encode(), dp(out)) == 0
+        return out
+
+    def residual(self, capture=False):
+        R = np.zeros(self.shape + (5,))
+        rc = self.L.orc_residual(self.h, dp(R), int(capture))
+        if rc < 0:
+            raise RuntimeError(self.error())
+        return R
+
+    def apply_bc(self):
+        self.L.orc_apply_bc(self.h)
+
+    def update_ghost_ids(self):
+        self.L.orc_update_ghost_ids(self.h)
+
+    def ls_apply_bc(self):
+        self.L.orc_ls_apply_bc(self.h)
+
+    def ls_residual(self):
+        out = np.zeros(self.shape)
+        self.L.orc_ls_residual(self.h, dp(out))
+        return out
+
+    def time_step(self):
+        dt, cfl = C.c_double(), C.c_double()
+        if self.L.orc_time_step(self.h, C.byref(dt), C.byref(cfl)) < 0:
+            raise RuntimeError(self.error())
+        return dt.value, cfl.value
+
+    def run_steps(self, n):
+        if self.L.orc_run_steps(self.h, n) < 0:
+            raise RuntimeError(self.error())
+
+    def advance_one(self, time, dt):
+        if self.L.orc_advance_one(self.h, time, dt) < 0:
+            raise RuntimeError(self.error())
+
+    def time(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_int()
+        self.L.orc_get_time(self.h, C.byref(t), C.byref(dt), C.byref(st))
+        return t.value, dt.value, st.value
+
+    def counters(self):
+        out = (C.c_long * 9)()
+        self.L.orc_counters(self.h, out)
+        keys = ["riemann_calls", "riemann_trivial", "riemann_errors", "riemann_iters", "riemann_fevals",
+                "riemann_fallbacks", "clipped", "id_changed", "steps"]
+        return dict(zip(keys, list(out)))
+
+
+def flux_batch(mats, nmat, scheme, d, Vm, Vp, ids):
+    n = len(Vm)
+    Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+    Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    F = np.zeros((n, 5))
+    lib().orc_flux_batch(mats, nmat, C.byref(scheme), n, d, dp(Vm), dp(Vp), ip(ids), dp(F))
+    return F
+
+
+def riemann_batch(mats, nmat, scheme, dirs, Vm, idm, Vp, idp):
+    n = len(Vm)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (dirs, Vm, Vp)]
+    idm = np.ascontiguousarray(idm, dtype=np.int32)
+    idp = np.ascontiguousarray(idp, dtype=np.int32)
+    Vs, Vsm, Vsp = np.zeros((n, 5)), np.zeros((n, 5)), np.zeros((n, 5))
+    mid, err, it, fe, fl = (np.zeros(n, dtype=np.int32) for _ in range(5))
+    lib().orc_riemann_batch(mats, nmat, C.byref(scheme), n, dp(a[0]), dp(a[1]), ip(idm), dp(a[2]), ip(idp),
+                            dp(Vs), ip(mid), dp(Vsm), dp(Vsp), ip(err), ip(it), ip(fe), ip(fl))
+    return dict(Vs=Vs, id=mid, Vsm=Vsm, Vsp=Vsp, err=err, iters=it, fevals=fe, flags=fl)
+
+
+def riemann_print(mats, nmat, scheme, Vm, idm, Vp, idp, cap=4096, nonadaptive=False):
+    Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+    Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+    rows = np.zeros((cap, 5))
+    star = np.zeros(4)
+    n = lib().orc_riemann_print(mats, nmat, C.byref(scheme), dp(Vm), idm, dp(Vp), idp, dp(rows), cap, dp(star), int(nonadaptive))
+    return rows[:n], star
+
+
+def wave(mats, nmat, scheme, w, rho, u, p, ps, idd, rhos0, rhos1):
+    rhos, us = C.c_double(), C.c_double()
+    ok = lib().orc_wave(mats, nmat, C.byref(scheme), w, rho, u, p, ps, idd, rhos0, rhos1, C.byref(rhos),
+                        C.byref(us))
+    return bool(ok), rhos.value, us.value
+
+
+def set_ck_textbook(on):
+    lib().orc_set_ck_textbook(int(on))
+
+
+def cr_root(x, n):
+    return lib().orc_cr_root(x, n)
