@@ -1,0 +1,12 @@
+#!/bin/bash
+# Builds libm2c_amd.so (gfx950) in-tree.  No FMA contraction and no fast-math:
+# results must match the reference's non-contracted x86-64 arithmetic bit for
+# bit.  Packed-FP32 instructions are disabled (see the build notes).
+set -e
+cd "$(dirname "$0")"
+HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
+$HIPCC --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
+  -ffp-contract=off -fno-fast-math \
+  -Xclang -target-feature -Xclang -packed-fp32-ops \
+  -Wall -Wno-unused-variable -Wno-unused-but-set-variable -Wno-unused-value -Wno-unused-result \
+  -I../include -o libm2c_amd.so csrc/m2c_capi.hip "$@"

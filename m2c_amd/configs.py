@@ -1,0 +1,119 @@
+"""Initial-condition generators for the benchmark configurations, in the
+reference's SpaceVariable3D layout (ghosted AoS, see include/m2c_amd.h).
+
+These restate what SpaceInitializer does for planes and spheres
+(SpaceInitializer.cpp:237-482: default state everywhere, then geometries in
+order; level set by signed distance, SpaceInitializer.cpp:1390-1514, negative
+inside the material it tracks), followed by the boundary fill.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+
+def _axis(N, a0, a1):
+    """Cell centres with one ghost each side (MeshGenerator.cpp:161-173,
+    SpaceOperator.cpp:259-327)."""
+    h = (a1 - a0) / N
+    c = np.empty(N + 2)
+    c[1:-1] = a0 + 0.5 * h + np.arange(N) * h
+    c[0] = c[1] - h
+    c[-1] = c[-2] + h
+    return c
+
+
+def fill_ghosts(V, ID, Phi, grid):
+    """ApplyBoundaryConditions (SpaceOperator.cpp:738-821) for OUTLET /
+    SYMMETRY-type faces, UpdateMaterialIDAtGhostNodes and the level-set
+    ZeroNeumann fill, on the host, for initial states."""
+    bcs = list(grid.bc)
+    for ax in range(3):  # array axes: V[k, j, i]
+        arr_ax = 2 - ax
+        for side in (0, 1):
+            bc = bcs[2 * ax + side]
+            g = 0 if side == 0 else -1
+            im = 1 if side == 0 else -2
+            sl_g = [slice(1, -1)] * 3
+            sl_i = [slice(1, -1)] * 3
+            sl_g[arr_ax] = g
+            sl_i[arr_ax] = im
+            sl_g, sl_i = tuple(sl_g), tuple(sl_i)
+            ID[sl_g] = ID[sl_i]
+            if Phi is not None:
+                Phi[sl_g] = Phi[sl_i]
+            if bc == abi.BC_INLET:
+                V[sl_g] = list(grid.inlet)
+            elif bc == abi.BC_INLET2:
+                V[sl_g] = list(grid.inlet2)
+            else:
+                V[sl_g] = V[sl_i]
+                if bc in (abi.BC_SYMMETRY, abi.BC_SLIPWALL):
+                    V[sl_g + (ax + 1,)] *= -1.0
+                elif bc == abi.BC_STICKWALL:
+                    V[sl_g + (slice(1, 4),)] *= -1.0
+
+
+def c1_sod(n=(128, 32, 32)):
+    """3D Sod shock tube on the unit box, single stiffened-gas material
+    (gamma 1.4, p_c 0), MUSCL primitive + van Albada, HLLC, RK2, CFL 0.5,
+    transmissive (Outlet) boundaries."""
+    g = abi.grid(n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (abi.BC_OUTLET,) * 6)
+    mats = abi.materials(abi.material("SG", gamma=1.4, pc=0.0))
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK2, cfl=0.5)
+    x = _axis(n[0], 0.0, 1.0)
+    V = np.zeros((n[2] + 2, n[1] + 2, n[0] + 2, 5))
+    V[..., 0] = np.where(x < 0.5, 1.0, 0.125)[None, None, :]
+    V[..., 4] = np.where(x < 0.5, 1.0, 0.1)[None, None, :]
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    fill_ghosts(V, ID, None, g)
+    return g, mats, 1, s, V, ID, None
+
+
+WATER_SG = dict(gamma=6.12, pc=3.43e8)
+AIR_SG = dict(gamma=1.4, pc=0.0)
+
+
+def sg_post_shock(rho1, p1, gam, pc, mach):
+    """Rankine-Hugoniot state behind a right-moving shock of Mach `mach`
+    entering quiescent stiffened gas (ideal-gas relations in p + p_c)."""
+    c1 = math.sqrt(gam * (p1 + pc) / rho1)
+    m2 = mach * mach
+    rho2 = rho1 * (gam + 1) * m2 / ((gam - 1) * m2 + 2)
+    p2 = (p1 + pc) * (1 + 2 * gam / (gam + 1) * (m2 - 1)) - pc
+    u2 = mach * c1 * (1 - rho1 / rho2)
+    return rho2, u2, p2
+
+
+def c2_shock_bubble(n=256, L=1.0, radius=0.15, mach=1.5, shock_x=0.2, tol=1e-10):
+    """Two-material shock-bubble: stiffened-gas water with one centred ideal-gas
+    (SG, gamma 1.4) air bubble of radius 0.15 L, hit by a planar Mach-1.5 shock
+    placed at x = 0.2 L travelling +x.  Exact interface Riemann solver with
+    tolerance 1e-10, HLLC elsewhere, MUSCL primitive van Albada, RK3, CFL 0.4,
+    one level set (material 1 = air) with ZeroNeumann boundaries, transmissive
+    boundaries."""
+    g = abi.grid((n, n, n), (0.0, 0.0, 0.0), (L, L, L), (abi.BC_OUTLET,) * 6)
+    mats = abi.materials(abi.material("SG", **WATER_SG), abi.material("SG", **AIR_SG))
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK3, cfl=0.4,
+                           n_levelsets=1, ls_materialid=1, riem_tol_main=tol)
+    x = _axis(n, 0.0, L)
+    rho_w, p_w = 1000.0, 1.0e5
+    rho2, u2, p2 = sg_post_shock(rho_w, p_w, WATER_SG["gamma"], WATER_SG["pc"], mach)
+    V = np.empty((n + 2, n + 2, n + 2, 5))
+    V[...] = [rho_w, 0.0, 0.0, 0.0, p_w]
+    post = (x < shock_x * L)
+    V[:, :, post, :] = [rho2, u2, 0.0, 0.0, p2]
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    c = 0.5 * L
+    R = radius * L
+    X = x[None, None, :]
+    Y = x[None, :, None]
+    Z = x[:, None, None]
+    dist = np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2) - R
+    Phi = np.ascontiguousarray(np.broadcast_to(dist, V.shape[:3]))
+    inside = dist < 0
+    V[inside] = [1.2, 0.0, 0.0, 0.0, p_w]
+    ID[inside] = 1
+    fill_ghosts(V, ID, Phi, g)
+    return g, mats, 2, s, V, ID, Phi

@@ -1,0 +1,180 @@
+// Grid layout and stencil kernels of the explicit step for gfx950.
+//
+// Layout: every field is structure-of-arrays FP64 (one array per component)
+// over the ghosted brick (nx+2) x (ny+2) x (nz+2), i fastest; one ghost layer
+// suffices because face states of ghost cells come from the boundary
+// conditions, never from a limiter (Reconstructor.cpp:807-889).
+//
+// The stage kernel fuses, per cell, what the reference does in separate full
+// passes for one RK stage: MUSCL reconstruction (Reconstructor.cpp:335-918),
+// the reconstructed-state check (SpaceOperator.cpp:2420-2569), the face-flux
+// loop (SpaceOperator.cpp:1432-2076), R /= -vol (:2628-2632), the RK
+// combination (TimeIntegrator.cpp:319-715), C2P + clipping and the boundary
+// fill of the new state (SpaceOperator.cpp:597-821).
+#pragma once
+#include "m2c_physics.h"
+
+namespace m2c {
+
+constexpr int TX = 16, TY = 4, TZ = 4, NT = TX * TY * TZ;  // stage tile: 256 threads
+
+struct DevGrid {
+  int NX, NY, NZ, SX, SY, SZ;
+  long S;        // plane stride SX*SY
+  long ncells;   // SX*SY*SZ
+  // per-axis coordinates and spacings with ghosts (index i+1)
+  const double *x, *y, *z, *dx, *dy, *dz;
+  // Reconstructor CoeffA / CoeffB / CoeffK per axis for interior cells (index i)
+  const double *Ax, *Ay, *Az, *Bx, *By, *Bz;
+  const int *Kx, *Ky, *Kz;
+  int bc[6];
+  double inlet[5], inlet2[5];
+};
+
+struct Scheme {
+  int flux, rec_type, limiter, slope_zero, integrator;
+  int mp_flux, mp_recon, nmat, inactive;
+  double gmm, del, cfl;
+};
+
+__device__ __forceinline__ long gidx(const DevGrid& g, int i, int j, int k) {
+  return ((long)(k + 1) * g.SY + (j + 1)) * g.SX + (i + 1);
+}
+
+struct F5 {
+  double* c[5];
+};
+struct CF5 {
+  const double* c[5];
+};
+
+__device__ __forceinline__ void load5(const CF5& f, long c, double* v) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) v[p] = f.c[p][c];
+}
+__device__ __forceinline__ void store5(const F5& f, long c, const double* v) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) f.c[p][c] = v[p];
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructed face state of an interior cell q along axis D (side 0: the
+// minus face, vl/vb/vk; side 1: the plus face, vr/vt/vf).
+//
+// The reference replaces only the FIRST failing face of a cell by the cell
+// value (Reconstructor.cpp:722-778) and then clips every face, reverting
+// clipped ones to the cell value and aborting on any remaining failure
+// (SpaceOperator.cpp:2420-2569).  In every run that does not abort, this is
+// the same as: a face reverts to the cell value iff it fails CheckState or
+// clipping would modify it.  `fail`/`fixable` report the two conditions so
+// the owning cell can detect the abort case.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void face_state(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
+                                           const unsigned char* __restrict__ ID, int i, int j, int k, int side,
+                                           double* out, bool& fail, bool& unfixable) {
+  const long c = gidx(g, i, j, k);
+  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+  double v[5];
+  load5(V, c, v);
+  fail = false;
+  unfixable = false;
+  const int myid = ID[c];
+  if (s.rec_type == M2CX_REC_CONSTANT || myid == s.inactive) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = v[p];
+    if (s.rec_type == M2CX_REC_CONSTANT && myid != s.inactive) {
+      const DevEos& E = M.m[myid];
+      bool f = eos_check5(E, out), w = eos_would_clip(E, out);
+      fail = f;
+      unfixable = f && !w;
+    }
+    return;
+  }
+  const int idm = ID[c - st], idp = ID[c + st];
+  double vm[5], vp[5];
+  load5(V, c - st, vm);
+  load5(V, c + st, vp);
+  const int a = (D == 0) ? i : (D == 1 ? j : k);
+  const double A = (D == 0) ? g.Ax[a] : (D == 1 ? g.Ay[a] : g.Az[a]);
+  const double B = (D == 0) ? g.Bx[a] : (D == 1 ? g.By[a] : g.Bz[a]);
+  const int K = (D == 0) ? g.Kx[a] : (D == 1 ? g.Ky[a] : g.Kz[a]);
+  const bool zero = s.slope_zero && (myid != idm || myid != idp);
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    double d0 = (idm == s.inactive) ? 0.0 : v[p] - vm[p];
+    double d1 = (idp == s.inactive) ? 0.0 : vp[p] - v[p];
+    double sig;
+    if (s.limiter == M2CX_LIM_GENERALIZED_MINMOD) sig = lim_gmm(A, B, s.gmm, d0, d1);
+    else if (s.limiter == M2CX_LIM_VANALBADA) sig = lim_va(A, B, K, d0, d1);
+    else sig = 0.5 * (d0 + d1);
+    if (zero) sig = 0.0;
+    out[p] = side ? v[p] + 0.5 * sig : v[p] - 0.5 * sig;
+  }
+  const DevEos& E = M.m[myid];
+  bool f = eos_check5(E, out);
+  bool w = eos_would_clip(E, out);
+  fail = f;
+  unfixable = f && !w;
+  if (f || w) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = v[p];
+  }
+}
+
+// Face state on the ghost side of a domain boundary (Reconstructor.cpp:807-889):
+// INLET/INLET2/OVERSET take the ghost value, OUTLET copies the image's face,
+// SYMMETRY/SLIPWALL flip the normal velocity, STICKWALL flips all velocities.
+__device__ __forceinline__ void ghost_face(int bc, int D, const double* vghost, const double* fimage, double* out) {
+  if (bc == M2CX_BC_INLET || bc == M2CX_BC_INLET2 || bc == M2CX_BC_OVERSET) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = vghost[p];
+  } else if (bc == M2CX_BC_SYMMETRY || bc == M2CX_BC_SLIPWALL) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = (p == D + 1) ? -fimage[p] : fimage[p];
+  } else if (bc == M2CX_BC_STICKWALL) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = (p >= 1 && p <= 3) ? -fimage[p] : fimage[p];
+  } else {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = fimage[p];
+  }
+}
+
+// Both states of the face between cell (i,j,k) - e_D and (i,j,k), i.e. the
+// minus face of (i,j,k).  `bits_m`/`bits_p` return fail/unfixable bits for the
+// two owning cells (bit 0 fail, bit 1 unfixable).
+template <int D>
+__device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
+                                          const unsigned char* __restrict__ ID, int i, int j, int k, double* Vm,
+                                          double* Vp, int& bits_m, int& bits_p) {
+  const int im = (D == 0) ? i - 1 : i, jm = (D == 1) ? j - 1 : j, km = (D == 2) ? k - 1 : k;
+  const int N = (D == 0) ? g.NX : (D == 1 ? g.NY : g.NZ);
+  const int a = (D == 0) ? i : (D == 1 ? j : k);
+  bool f, u;
+  bits_m = bits_p = 0;
+  if (a == 0) {
+    // ghost on the minus side
+    double fp[5], vg[5];
+    face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+    bits_p = (f ? 1 : 0) | (u ? 2 : 0);
+    load5(V, gidx(g, im, jm, km), vg);
+    for (int p = 0; p < 5; p++) fp[p] = Vp[p];
+    ghost_face(g.bc[2 * D], D, vg, fp, Vm);
+  } else if (a == N) {
+    // ghost on the plus side
+    double fm[5], vg[5];
+    face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
+    bits_m = (f ? 1 : 0) | (u ? 2 : 0);
+    load5(V, gidx(g, i, j, k), vg);
+    for (int p = 0; p < 5; p++) fm[p] = Vm[p];
+    ghost_face(g.bc[2 * D + 1], D, vg, fm, Vp);
+  } else {
+    face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
+    bits_m = (f ? 1 : 0) | (u ? 2 : 0);
+    face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+    bits_p = (f ? 1 : 0) | (u ? 2 : 0);
+  }
+}
+
+}  // namespace m2c

@@ -1,0 +1,429 @@
+// Kernels of the explicit step (gfx950).  See m2c_grid.h for the layout.
+#pragma once
+#include "m2c_grid.h"
+
+namespace m2c {
+
+// Device-side counters and flags of one solver.
+struct DevCounters {
+  unsigned long long clipped;      // ClipDensityAndPressure hits (all stages)
+  unsigned long long stage_clipped;  // hits in the current intermediate stage
+  unsigned long long riemann_calls, riemann_trivial, riemann_errors, riemann_iters, riemann_fevals;
+  unsigned long long riemann_fallbacks, guess_fallbacks, id_changed, steps;
+  int error;          // bit 0 hyperbolicity (abort), bit 1 Riemann fatal, bit 2 capacity, bit 3 unresolved
+  int done;           // time loop reached MaxTime
+  unsigned int dt_ticket;
+  int nif;            // interface faces in the current list
+  int n_remaining, n_changed, n_unresolved, pad;
+  double t, dt, cfl, dts, tmax;
+  int time_step, pad2;
+};
+
+enum StageMode { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_RK2_FINAL = 2, MODE_RK3_STAGE2 = 3, MODE_RK3_FINAL = 4,
+                 MODE_FE = 5 };
+
+// Interface-face list produced once per time step (face between cell
+// `cell - e_dir` and `cell`).  Fluxes are refreshed every stage; the star
+// states are the stage-1 capture (RiemannSolutions, RiemannSolutions.h:12-30).
+struct IfaceList {
+  int* cell;      // linear index of the plus-side cell
+  int* dir;
+  double* f1;     // [5][cap]: flux applied to the minus-side cell (localflux1)
+  double* f2;     // [5][cap]: flux applied to the plus-side cell (localflux2)
+  double* vsm;    // [5][cap]: captured star state, minus side
+  double* vsp;    // [5][cap]
+  int* err;       // Riemann error of the capture
+  int* stats;     // [2][cap]: main-loop iterations, f(p) evaluations (last solve)
+  int cap;
+};
+
+struct StageArgs {
+  DevGrid g;
+  Scheme s;
+  CF5 Vin;
+  F5 Vout, Un, U1, R;
+  const unsigned char* ID;
+  const int* fidx[3];
+  IfaceList L;
+  int mode;
+  DevCounters* cnt;
+};
+
+__device__ __forceinline__ void write_ghosts(const DevGrid& g, const F5& V, int i, int j, int k, const double* v) {
+  int side = -1;
+  for (int pass = 0; pass < 6; pass++) {
+    int gi = i, gj = j, gk = k;
+    if (pass == 0) { if (i != 0) continue; gi = -1; }
+    else if (pass == 1) { if (i != g.NX - 1) continue; gi = g.NX; }
+    else if (pass == 2) { if (j != 0) continue; gj = -1; }
+    else if (pass == 3) { if (j != g.NY - 1) continue; gj = g.NY; }
+    else if (pass == 4) { if (k != 0) continue; gk = -1; }
+    else { if (k != g.NZ - 1) continue; gk = g.NZ; }
+    side = pass;
+    const int bc = g.bc[side];
+    double w[5];
+    if (bc == M2CX_BC_INLET) { for (int p = 0; p < 5; p++) w[p] = g.inlet[p]; }
+    else if (bc == M2CX_BC_INLET2) { for (int p = 0; p < 5; p++) w[p] = g.inlet2[p]; }
+    else if (bc == M2CX_BC_SYMMETRY || bc == M2CX_BC_SLIPWALL) {
+      for (int p = 0; p < 5; p++) w[p] = v[p];
+      w[side / 2 + 1] *= -1.0;
+    } else if (bc == M2CX_BC_STICKWALL) {
+      w[0] = v[0]; w[1] = -1.0 * v[1]; w[2] = -1.0 * v[2]; w[3] = -1.0 * v[3]; w[4] = v[4];
+    } else if (bc == M2CX_BC_OUTLET) {
+      for (int p = 0; p < 5; p++) w[p] = v[p];
+    } else {
+      continue;
+    }
+    store5(V, gidx(g, gi, gj, gk), w);
+  }
+}
+
+// Flux through the minus face (axis D) of cell (i,j,k); returns f1 (applied to
+// the minus-side cell) and sets cell_bits_m/p for the reconstruction check.
+template <int D>
+__device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, int i, int j, int k, double* f1,
+                                          int& bm, int& bp, int& err) {
+  const DevGrid& g = a.g;
+  const Scheme& s = a.s;
+  const long c = gidx(g, i, j, k);
+  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+  const int myid = a.ID[c], nid = a.ID[c - st];
+  double Vm[5], Vp[5];
+  face_pair<D>(g, s, M, a.Vin, a.ID, i, j, k, Vm, Vp, bm, bp);
+  if (nid == myid) {
+    if (myid != s.inactive) flux_num(M.m[myid], s.flux, s.del, D, Vm, Vp, f1, err);
+    else for (int p = 0; p < 5; p++) f1[p] = 0.0;
+  } else if (nid == s.inactive || myid == s.inactive) {
+    for (int p = 0; p < 5; p++) f1[p] = 0.0;
+  } else if (s.mp_flux == M2CX_MPFLUX_LLF) {
+    double vm[5], vp[5];
+    if (s.mp_recon == M2CX_MPREC_CONSTANT) { load5(a.Vin, c - st, vm); load5(a.Vin, c, vp); }
+    else for (int p = 0; p < 5; p++) { vm[p] = Vm[p]; vp[p] = Vp[p]; }
+    flux_llf(M.m[nid], M.m[myid], D, vm, vp, f1, err);
+  } else {
+    const int slot = a.fidx[D][c];
+    for (int p = 0; p < 5; p++) f1[p] = a.L.f1[(long)p * a.L.cap + slot];
+  }
+}
+
+__constant__ DevMats c_mats;
+
+template <int MODE>
+__global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
+  __shared__ double sFx[5][TZ * TY * (TX + 1)];
+  __shared__ double sFy[5][TZ * (TY + 1) * TX];
+  __shared__ double sFz[5][(TZ + 1) * TY * TX];
+  __shared__ int sMask[NT];
+  const DevGrid& g = a.g;
+  const Scheme& s = a.s;
+  const DevMats& M = c_mats;
+  const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
+  const int t = threadIdx.x;
+  const int ib = blockIdx.x * TX, jb = blockIdx.y * TY, kb = blockIdx.z * TZ;
+  const int i = ib + tx, j = jb + ty, k = kb + tz;
+  const bool inside = (i < g.NX) && (j < g.NY) && (k < g.NZ);
+  sMask[t] = 0;
+  __syncthreads();
+  int err = 0;
+  double f[5];
+  int bm, bp;
+  // own minus faces of this thread's cell position (also for the first
+  // position past the domain end, whose minus face closes the last cell)
+  if (i <= g.NX && j < g.NY && k < g.NZ) {
+    face_flux<0>(a, M, i, j, k, f, bm, bp, err);
+    const int fi = (tz * TY + ty) * (TX + 1) + tx;
+    for (int p = 0; p < 5; p++) sFx[p][fi] = f[p];
+    if (tx > 0 && bm) atomicOr(&sMask[t - 1], bm << 2);        // vr of cell tx-1: face slot 1
+    if (bp) atomicOr(&sMask[t], bp << 0);                       // vl of this cell: face slot 0
+  }
+  if (i < g.NX && j <= g.NY && k < g.NZ) {
+    face_flux<1>(a, M, i, j, k, f, bm, bp, err);
+    const int fi = (tz * (TY + 1) + ty) * TX + tx;
+    for (int p = 0; p < 5; p++) sFy[p][fi] = f[p];
+    if (ty > 0 && bm) atomicOr(&sMask[t - TX], bm << 6);
+    if (bp) atomicOr(&sMask[t], bp << 4);
+  }
+  if (i < g.NX && j < g.NY && k <= g.NZ) {
+    face_flux<2>(a, M, i, j, k, f, bm, bp, err);
+    const int fi = (tz * TY + ty) * TX + tx;
+    for (int p = 0; p < 5; p++) sFz[p][fi] = f[p];
+    if (tz > 0 && bm) atomicOr(&sMask[t - TX * TY], bm << 10);
+    if (bp) atomicOr(&sMask[t], bp << 8);
+  }
+  // the plus faces of the tile's last layer: one direction per wave
+  if (t < 64) {  // y: (tx, tz) -> face at j = jb + TY
+    const int ex = t % TX, ez = t / TX;
+    const int ei = ib + ex, ej = jb + TY, ek = kb + ez;
+    if (ei < g.NX && ej <= g.NY && ek < g.NZ) {
+      face_flux<1>(a, M, ei, ej, ek, f, bm, bp, err);
+      const int fi = (ez * (TY + 1) + TY) * TX + ex;
+      for (int p = 0; p < 5; p++) sFy[p][fi] = f[p];
+      if (bm) atomicOr(&sMask[ex + TX * ((TY - 1) + TY * ez)], bm << 6);
+    }
+  } else if (t < 128) {  // z: (tx, ty) -> face at k = kb + TZ
+    const int e = t - 64;
+    const int ex = e % TX, ey = e / TX;
+    const int ei = ib + ex, ej = jb + ey, ek = kb + TZ;
+    if (ei < g.NX && ej < g.NY && ek <= g.NZ) {
+      face_flux<2>(a, M, ei, ej, ek, f, bm, bp, err);
+      const int fi = (TZ * TY + ey) * TX + ex;
+      for (int p = 0; p < 5; p++) sFz[p][fi] = f[p];
+      if (bm) atomicOr(&sMask[ex + TX * (ey + TY * (TZ - 1))], bm << 10);
+    }
+  } else if (t < 128 + TY * TZ) {  // x: (ty, tz) -> face at i = ib + TX
+    const int e = t - 128;
+    const int ey = e % TY, ez = e / TY;
+    const int ei = ib + TX, ej = jb + ey, ek = kb + ez;
+    if (ei <= g.NX && ej < g.NY && ek < g.NZ) {
+      face_flux<0>(a, M, ei, ej, ek, f, bm, bp, err);
+      const int fi = (ez * TY + ey) * (TX + 1) + TX;
+      for (int p = 0; p < 5; p++) sFx[p][fi] = f[p];
+      if (bm) atomicOr(&sMask[(TX - 1) + TX * (ey + TY * ez)], bm << 2);
+    }
+  }
+  __syncthreads();
+  if (err) atomicOr(&a.cnt->error, 1);
+  if (!inside) return;
+
+  // reconstruction check of this cell's six faces (abort condition)
+  {
+    const int m = sMask[t];
+    int fails = 0, unf = 0;
+    for (int q = 0; q < 6; q++) {
+      if (m & (1 << (2 * q))) fails |= 1 << q;
+      if (m & (2 << (2 * q))) unf |= 1 << q;
+    }
+    if (fails) {
+      const int rest = fails & (fails - 1);
+      if (rest & unf) atomicOr(&a.cnt->error, 1);
+    }
+  }
+
+  const long c = gidx(g, i, j, k);
+  const int myid = a.ID[c];
+  const double ax = g.dy[j + 1] * g.dz[k + 1];
+  const double ay = g.dx[i + 1] * g.dz[k + 1];
+  const double az = g.dx[i + 1] * g.dy[j + 1];
+  const int fxm = (tz * TY + ty) * (TX + 1) + tx, fxp = fxm + 1;
+  const int fym = (tz * (TY + 1) + ty) * TX + tx, fyp = fym + TX;
+  const int fzm = (tz * TY + ty) * TX + tx, fzp = fzm + TY * TX;
+  // minus faces of an interface take localflux2 from the Riemann list
+  const bool mp = s.mp_flux != M2CX_MPFLUX_LLF;
+  const int idx_m = a.ID[c - 1], idy_m = a.ID[c - g.SX], idz_m = a.ID[c - g.S];
+  const bool ifx = mp && idx_m != myid && idx_m != s.inactive && myid != s.inactive;
+  const bool ify = mp && idy_m != myid && idy_m != s.inactive && myid != s.inactive;
+  const bool ifz = mp && idz_m != myid && idz_m != s.inactive && myid != s.inactive;
+  const int slx = ifx ? a.fidx[0][c] : 0, sly = ify ? a.fidx[1][c] : 0, slz = ifz ? a.fidx[2][c] : 0;
+  double r[5];
+  const double cst = 1.0 / (-(g.dx[i + 1] * g.dy[j + 1] * g.dz[k + 1]));
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const double Fm = ifx ? a.L.f2[(long)p * a.L.cap + slx] : sFx[p][fxm];
+    const double Gm = ify ? a.L.f2[(long)p * a.L.cap + sly] : sFy[p][fym];
+    const double Hm = ifz ? a.L.f2[(long)p * a.L.cap + slz] : sFz[p][fzm];
+    double acc = 0.0 - ax * Fm;
+    acc = acc - ay * Gm;
+    acc = acc - az * Hm;
+    acc = acc + ax * sFx[p][fxp];
+    acc = acc + ay * sFy[p][fyp];
+    acc = acc + az * sFz[p][fzp];
+    r[p] = acc * cst;
+  }
+
+  if (MODE == MODE_RESIDUAL) {
+    store5(a.R, c, r);
+    return;
+  }
+  const DevEos& E = M.m[myid];
+  const double dt = a.cnt->dt;
+  double u[5], v[5];
+  if (MODE == MODE_STAGE1 || MODE == MODE_FE) {
+    double v0[5], un[5];
+    load5(a.Vin, c, v0);
+    eos_p2c(E, v0, un);
+    if (MODE == MODE_STAGE1) store5(a.Un, c, un);
+#pragma unroll
+    for (int p = 0; p < 5; p++) u[p] = un[p] + dt * r[p];
+  } else {
+    double u1[5], un[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) { u1[p] = a.U1.c[p][c]; un[p] = a.Un.c[p][c]; }
+    if (MODE == MODE_RK2_FINAL) {
+      const double h = 0.5 * dt;
+#pragma unroll
+      for (int p = 0; p < 5; p++) { u[p] = 0.5 * u1[p] + 0.5 * un[p]; u[p] = u[p] + h * r[p]; }
+    } else if (MODE == MODE_RK3_STAGE2) {
+      const double h = 0.25 * dt;
+#pragma unroll
+      for (int p = 0; p < 5; p++) { u[p] = 0.25 * u1[p] + 0.75 * un[p]; u[p] = u[p] + h * r[p]; }
+    } else {
+      const double h = 2.0 / 3.0 * dt;
+#pragma unroll
+      for (int p = 0; p < 5; p++) { u[p] = 2.0 / 3.0 * u1[p] + 1.0 / 3.0 * un[p]; u[p] = u[p] + h * r[p]; }
+    }
+  }
+  eos_c2p(E, u, v);
+  const bool clipped = eos_clip(E, v);
+  if (clipped) {
+    atomicAdd(&a.cnt->clipped, 1ull);
+    if (MODE == MODE_STAGE1 || MODE == MODE_RK3_STAGE2) atomicAdd(&a.cnt->stage_clipped, 1ull);
+  }
+  if (eos_check5(E, v)) atomicOr(&a.cnt->error, 1);
+  if (MODE == MODE_STAGE1 || MODE == MODE_RK3_STAGE2) store5(a.U1, c, u);
+  store5(a.Vout, c, v);
+  write_ghosts(g, a.Vout, i, j, k, v);
+}
+
+// U = P2C(V) on interior cells, only if the preceding stage clipped a cell
+// (TimeIntegrator.cpp:569-572, 622-625).
+__global__ void k_resync(DevGrid g, CF5 V, F5 U, const unsigned char* __restrict__ ID, const DevCounters* cnt) {
+  if (cnt->stage_clipped == 0) return;
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    double v[5], u[5];
+    load5(V, c, v);
+    eos_p2c(c_mats.m[ID[c]], v, u);
+    store5(U, c, u);
+  }
+}
+
+__global__ void k_reset_stage(DevCounters* cnt) { cnt->stage_clipped = 0; }
+
+// SpaceOperator::ApplyBoundaryConditions on a state field (face ghosts)
+__global__ void k_apply_bc(DevGrid g, F5 V) {
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    if (i != 0 && i != g.NX - 1 && j != 0 && j != g.NY - 1 && k != 0 && k != g.NZ - 1) continue;
+    double v[5];
+    CF5 cv{{V.c[0], V.c[1], V.c[2], V.c[3], V.c[4]}};
+    load5(cv, gidx(g, i, j, k), v);
+    write_ghosts(g, V, i, j, k, v);
+  }
+}
+
+// ClipDensityAndPressure on interior cells (SpaceOperator.cpp:646-693)
+__global__ void k_clip(DevGrid g, F5 V, const unsigned char* __restrict__ ID, DevCounters* cnt) {
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    double v[5];
+    for (int p = 0; p < 5; p++) v[p] = V.c[p][c];
+    const DevEos& E = c_mats.m[ID[c]];
+    if (eos_clip(E, v)) {
+      atomicAdd(&cnt->clipped, 1ull);
+      atomicAdd(&cnt->stage_clipped, 1ull);
+      for (int p = 0; p < 5; p++) V.c[p][c] = v[p];
+    }
+    if (eos_check5(E, v)) atomicOr(&cnt->error, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Time step (SpaceOperator.cpp:1223-1330 and the loop of Main.cpp:653-712):
+// dt = cfl * min over cells and active axes of d_a/(|u_a|+c).  Block minima
+// go to `part`; the last block to finish reduces them and, when `advance`,
+// clamps dt at MaxTime and moves t, like the driver loop.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nmin(double a, double b) { return (b < a) ? b : a; }  // std::min(a,b)
+
+__global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const unsigned char* __restrict__ ID,
+                                            double* part, DevCounters* cnt, int advance) {
+  __shared__ double red[256];
+  __shared__ bool last;
+  double m = DBL_MAX;
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    int id = ID[c];
+    if (id == s.inactive) continue;
+    const DevEos& E = c_mats.m[id];
+    double v[5];
+    load5(V, c, v);
+    double e = eos_e(E, v[0], v[4]);
+    double c2 = eos_c2(E, v[0], e);
+    if (c2 < 0) atomicOr(&cnt->error, 1);
+    double cc = sqrt(c2);
+    double lf = fabs(v[1]) + cc, lg = fabs(v[2]) + cc, lh = fabs(v[3]) + cc;
+    if (g.NX > 1) m = nmin(m, g.dx[i + 1] / lf);
+    if (g.NY > 1) m = nmin(m, g.dy[j + 1] / lg);
+    if (g.NZ > 1) m = nmin(m, g.dz[k + 1] / lh);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = nmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0];
+    __threadfence();
+    unsigned int tk = atomicAdd(&cnt->dt_ticket, 1u);
+    last = (tk == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double mm = DBL_MAX;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) mm = nmin(mm, ((volatile double*)part)[b]);
+  red[threadIdx.x] = mm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = nmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    cnt->dt_ticket = 0;
+    double cfl = s.cfl;
+    double dt = cfl * red[0];
+    if (advance) {
+      double t = cnt->t, tmax = cnt->tmax;
+      if (t + dt > tmax) {
+        cfl *= (tmax - t) / dt;
+        dt = tmax - t;
+      }
+      cnt->dts = dt;
+      cnt->t = t + dt;
+      cnt->time_step += 1;
+      cnt->steps += 1;
+    }
+    cnt->dt = dt;
+    cnt->cfl = cfl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// AoS (reference layout) <-> SoA transposes for upload/download
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ src, F5 dst, int dof, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    for (int p = 0; p < dof; p++) dst.c[p][c] = src[c * dof + p];
+}
+__global__ void k_soa_to_aos(CF5 src, double* __restrict__ dst, int dof, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    for (int p = 0; p < dof; p++) dst[c * dof + p] = src.c[p][c];
+}
+__global__ void k_i32_to_u8(const int* __restrict__ src, unsigned char* __restrict__ dst, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    dst[c] = (unsigned char)src[c];
+}
+__global__ void k_u8_to_i32(const unsigned char* __restrict__ src, int* __restrict__ dst, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    dst[c] = src[c];
+}
+
+// Batched numerical flux (FluxFcnBase::ComputeNumericalFluxAtCellInterface)
+__global__ void k_flux_batch(Scheme s, int n, int d, const double* Vm, const double* Vp, const int* id, double* F,
+                             DevCounters* cnt) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int err = 0;
+  flux_num(c_mats.m[id[q]], s.flux, s.del, d, Vm + 5 * q, Vp + 5 * q, F + 5 * q, err);
+  if (err) atomicOr(&cnt->error, 1);
+}
+
+}  // namespace m2c

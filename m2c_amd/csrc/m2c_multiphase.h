@@ -1,0 +1,904 @@
+// Multi-material part of the explicit step on gfx950:
+//  - classification of material-interface faces into a compact list, once
+//    per time step (IDs do not change between RK stages);
+//  - the exact two-material Riemann solve over that list, each stage, giving
+//    the two interface fluxes and, in stage 1, the star states kept for the
+//    end-of-step update (SpaceOperator.cpp:1613-1690);
+//  - level-set FD3 advection stages (LevelSetOperator.cpp:796-870,
+//    GradientCalculatorFD3.cpp:54-171) and boundary fill (:408-508);
+//  - the end-of-step material update: IDs from the level-set sign change with
+//    the neighbour vote for swept cells (MultiPhaseOperator.cpp:153-357),
+//    ghost IDs (:919-942), and the Riemann-solution state update of cells that
+//    changed material (:980-1150) with its failsafe (:1278-1525).
+#pragma once
+#include <string>
+#include <vector>
+#include "m2c_kernels.h"
+#include "m2c_riemann.h"
+
+namespace m2c {
+
+struct MPBuffers {
+  int* fidx[3] = {};
+  IfaceList L{};
+  double *phi = nullptr, *phi1 = nullptr, *phin = nullptr, *rls = nullptr, *phitmp = nullptr;
+  double* fd3 = nullptr;  // FD3 coefficient tables (see mp_create)
+  long off_cxL = 0, off_cxR = 0, off_cyL = 0, off_cyR3 = 0, off_cyR1 = 0, off_czL = 0, off_czR = 0;
+  int lsid = 1, nls = 0;
+  int lsbc[6] = {1, 1, 1, 1, 1, 1};
+  RiemannCfg rc{};
+  double* slab = nullptr;
+  int lanes = 0, pathcap = 0;
+  int* remaining = nullptr;
+  int* changed = nullptr;
+  int* unresolved = nullptr;
+  unsigned char* tag = nullptr;
+  unsigned char* done = nullptr;
+  unsigned char* IDn = nullptr;
+  int list_cap = 0;
+  // learned p* guesses per interface slot (see m2c_mlp.h); null = acoustic
+  float* pguess = nullptr;
+  int guess = 0;
+};
+
+// FD3 coefficients: d/dx at 0 of the Lagrange interpolant through d[0..3]
+// (the 4x4 Taylor system GradientCalculatorFD3.cpp:457-469 solves with Eigen),
+// in extended precision, rounded once.
+static inline void fd3_coeffs(double d0, double d1, double d2, double d3, double* c) {
+  long double d[4] = {d0, d1, d2, d3};
+  for (int m = 0; m < 4; m++) {
+    long double s = 0;
+    for (int l = 0; l < 4; l++) {
+      if (l == m) continue;
+      long double prod = 1.0L / (d[m] - d[l]);
+      for (int n = 0; n < 4; n++) {
+        if (n == m || n == l) continue;
+        prod *= (0.0L - d[n]) / (d[m] - d[n]);
+      }
+      s += prod;
+    }
+    c[m] = (double)s;
+  }
+}
+
+inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vector<double>* ax, hipStream_t st) {
+  const long n = g.ncells;
+  const long nin = (long)g.NX * g.NY * g.NZ;
+  for (int d = 0; d < 3; d++) {
+    if (hipMalloc(&B.fidx[d], n * 4) != hipSuccess) return M2CX_E_OOM;
+    if (hipMemsetAsync(B.fidx[d], 0, n * 4, st) != hipSuccess) return M2CX_E_HIP;
+  }
+  long cap = std::max<long>(1 << 16, std::min<long>(nin / 2, 1L << 22));
+  B.L.cap = (int)cap;
+  if (hipMalloc(&B.L.cell, cap * 4) || hipMalloc(&B.L.dir, cap * 4) || hipMalloc(&B.L.f1, cap * 40) ||
+      hipMalloc(&B.L.f2, cap * 40) || hipMalloc(&B.L.vsm, cap * 40) || hipMalloc(&B.L.vsp, cap * 40) ||
+      hipMalloc(&B.L.err, cap * 4) || hipMalloc(&B.L.stats, cap * 8))
+    return M2CX_E_OOM;
+  for (double** p : {&B.phi, &B.phi1, &B.phin, &B.rls, &B.phitmp}) {
+    if (hipMalloc(p, n * 8) != hipSuccess) return M2CX_E_OOM;
+    if (hipMemsetAsync(*p, 0, n * 8, st) != hipSuccess) return M2CX_E_HIP;
+  }
+  // FD3 tables (GradientCalculatorFD3.cpp:300-453), per axis because the mesh
+  // is a tensor product; cyR1 holds the first-order right-biased y stencil the
+  // reference uses where its `i<NY-1` test (:400) fails.
+  const std::vector<double>&x = ax[0], &y = ax[1], &z = ax[2];
+  const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+  std::vector<double> t;
+  auto put = [&](long& off, int N, auto fn) {
+    off = (long)t.size();
+    t.resize(t.size() + (size_t)N * 4);
+    for (int i = 0; i < N; i++) fn(i, &t[off + 4L * i]);
+  };
+  auto first = [](double d, double* c) { c[0] = 0.0; c[1] = -1.0 / d; c[2] = 1.0 / d; c[3] = 0.0; };
+  auto C = [](const std::vector<double>& v, int i) { return v[i + 1]; };
+  put(B.off_cxL, NX, [&](int i, double* c) {
+    if (i > 0) fd3_coeffs(C(x, i - 2) - C(x, i), C(x, i - 1) - C(x, i), 0.0, C(x, i + 1) - C(x, i), c);
+    else first(C(x, i) - C(x, i - 1), c);
+  });
+  put(B.off_cxR, NX, [&](int i, double* c) {
+    if (i < NX - 1) fd3_coeffs(C(x, i - 1) - C(x, i), 0.0, C(x, i + 1) - C(x, i), C(x, i + 2) - C(x, i), c);
+    else first(C(x, i + 1) - C(x, i), c);
+  });
+  put(B.off_cyL, NY, [&](int j, double* c) {
+    if (j > 0) fd3_coeffs(C(y, j - 2) - C(y, j), C(y, j - 1) - C(y, j), 0.0, C(y, j + 1) - C(y, j), c);
+    else first(C(y, j) - C(y, j - 1), c);
+  });
+  put(B.off_cyR3, NY, [&](int j, double* c) {
+    if (j + 2 <= NY) fd3_coeffs(C(y, j - 1) - C(y, j), 0.0, C(y, j + 1) - C(y, j), C(y, j + 2) - C(y, j), c);
+    else c[0] = c[1] = c[2] = c[3] = 0.0;  // never read (LevelSetOperator.cpp:840-845)
+  });
+  put(B.off_cyR1, NY, [&](int j, double* c) { first(C(y, j + 1) - C(y, j), c); });
+  put(B.off_czL, NZ, [&](int k, double* c) {
+    if (k > 0) fd3_coeffs(C(z, k - 2) - C(z, k), C(z, k - 1) - C(z, k), 0.0, C(z, k + 1) - C(z, k), c);
+    else first(C(z, k) - C(z, k - 1), c);
+  });
+  put(B.off_czR, NZ, [&](int k, double* c) {
+    if (k < NZ - 1) fd3_coeffs(C(z, k - 1) - C(z, k), 0.0, C(z, k + 1) - C(z, k), C(z, k + 2) - C(z, k), c);
+    else first(C(z, k + 1) - C(z, k), c);
+  });
+  if (hipMalloc(&B.fd3, t.size() * 8) != hipSuccess) return M2CX_E_OOM;
+  if (hipMemcpy(B.fd3, t.data(), t.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return M2CX_E_HIP;
+  B.nls = sch.n_levelsets;
+  B.lsid = sch.ls_materialid;
+  for (int q = 0; q < 6; q++) B.lsbc[q] = sch.ls_bc[q];
+  B.rc = RiemannCfg{sch.riem_maxits_main, sch.riem_maxits_bracket, sch.riem_maxits_shock,
+                    sch.riem_steps_rarefaction, sch.riem_tol_main, sch.riem_tol_shock, sch.riem_tol_rarefaction,
+                    sch.riem_min_pressure, sch.riem_failure_threshold, sch.riem_pressure_at_failure};
+  B.guess = sch.riem_guess;
+  // per-lane integration-path scratch: lanes x 2 paths x cap points x 3
+  B.lanes = 256 * 128;
+  B.pathcap = 512;
+  if (hipMalloc(&B.slab, (long)B.lanes * 2 * B.pathcap * 3 * 8) != hipSuccess) return M2CX_E_OOM;
+  B.list_cap = (int)std::max<long>(1 << 16, std::min<long>(nin, 1L << 22));
+  if (hipMalloc(&B.remaining, B.list_cap * 4) || hipMalloc(&B.changed, B.list_cap * 4) ||
+      hipMalloc(&B.unresolved, B.list_cap * 4) || hipMalloc(&B.tag, n) || hipMalloc(&B.done, n) ||
+      hipMalloc(&B.IDn, n))
+    return M2CX_E_OOM;
+  if (hipMemsetAsync(B.tag, 0, n, st) || hipMemsetAsync(B.done, 0, n, st) || hipMemsetAsync(B.IDn, 0, n, st))
+    return M2CX_E_HIP;
+  return 0;
+}
+
+inline void mp_destroy(MPBuffers& B) {
+  for (int d = 0; d < 3; d++) (void)hipFree(B.fidx[d]);
+  for (void* p : {(void*)B.L.cell, (void*)B.L.dir, (void*)B.L.f1, (void*)B.L.f2, (void*)B.L.vsm, (void*)B.L.vsp,
+                  (void*)B.L.err, (void*)B.L.stats, (void*)B.phi, (void*)B.phi1, (void*)B.phin, (void*)B.rls,
+                  (void*)B.phitmp, (void*)B.fd3, (void*)B.slab, (void*)B.remaining, (void*)B.changed,
+                  (void*)B.unresolved, (void*)B.tag, (void*)B.done, (void*)B.IDn})
+    (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// interface classification
+// ---------------------------------------------------------------------------
+__global__ void k_if_reset(DevCounters* cnt) { cnt->nif = 0; }
+
+__global__ void k_classify(DevGrid g, Scheme s, const unsigned char* __restrict__ ID, int* fx, int* fy, int* fz,
+                           IfaceList L, DevCounters* cnt) {
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    int my = ID[c];
+    if (my == s.inactive) continue;
+    int a3[3] = {i, j, k};
+    long st3[3] = {1, g.SX, g.S};
+    int* fi3[3] = {fx, fy, fz};
+    for (int d = 0; d < 3; d++) {
+      if (a3[d] == 0) continue;  // ghost neighbour: its ID is the image's
+      int nb = ID[c - st3[d]];
+      if (nb == my || nb == s.inactive) continue;
+      int slot = atomicAdd(&cnt->nif, 1);
+      if (slot < L.cap) {
+        L.cell[slot] = (int)c;
+        L.dir[slot] = d;
+        fi3[d][c] = slot;
+      } else {
+        atomicOr(&cnt->error, 4);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact Riemann over the interface list
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void if_states(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
+                                          const unsigned char* ID, long c, double* Vm, double* Vp) {
+  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+  if (s.mp_recon == M2CX_MPREC_CONSTANT) {
+    load5(V, c - st, Vm);
+    load5(V, c, Vp);
+    return;
+  }
+  long r = c;
+  int i = (int)(r % g.SX) - 1;
+  r /= g.SX;
+  int j = (int)(r % g.SY) - 1;
+  int k = (int)(r / g.SY) - 1;
+  int im = (D == 0) ? i - 1 : i, jm = (D == 1) ? j - 1 : j, km = (D == 2) ? k - 1 : k;
+  bool f, u;
+  face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
+  face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+}
+
+__global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
+                                                 const unsigned char* __restrict__ ID, IfaceList L, int stage1,
+                                                 double* slab, int lanes, int pathcap, const float* guess,
+                                                 DevCounters* cnt) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= lanes) return;
+  const int nif = min(cnt->nif, L.cap);
+  const DevMats& M = c_mats;
+  DevRiemann R(M, rc, slab + lane, lanes, pathcap);
+  unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
+  int fatal = 0, ovf = 0;
+  for (int q = lane; q < nif; q += lanes) {
+    const long c = L.cell[q];
+    const int d = L.dir[q];
+    const long st = (d == 0) ? 1 : (d == 1 ? g.SX : g.S);
+    const int idl = ID[c - st], idr = ID[c];
+    double Vm[5], Vp[5];
+    if (d == 0) if_states<0>(g, s, M, V, ID, c, Vm, Vp);
+    else if (d == 1) if_states<1>(g, s, M, V, ID, c, Vm, Vp);
+    else if_states<2>(g, s, M, V, ID, c, Vm, Vp);
+    double dir[3] = {0.0, 0.0, 0.0};
+    dir[d] = 1.0;
+    // learned p* guess for this face (m2c_mlp.h), rejected when not a
+    // finite positive pressure: the acoustic guess is then used
+    double p0g = -1.0;
+    if (guess) {
+      float gp = guess[q];
+      p0g = (isfinite(gp) && gp > 0.0f) ? (double)gp : -1.0;
+      if (p0g <= 0) gfb++;
+    }
+    double Vs[5], Vsm[5], Vsp[5];
+    int mid = idl;
+    R.st = RStats{0, 0, false, false, false};
+    R.path1.overflow = R.path3.overflow = false;
+    int err = R.solve(dir, Vm, idl, Vp, idr, Vs, mid, Vsm, Vsp, p0g);
+    calls++;
+    triv += R.st.trivial;
+    errs += (err != 0);
+    its += R.st.it_main;
+    fev += R.st.fevals;
+    fb += R.st.fallback;
+    fatal |= R.st.fatal;
+    ovf |= R.path1.overflow || R.path3.overflow;
+    eos_clip(M.m[idl], Vsm);
+    eos_clip(M.m[idr], Vsp);
+    if (stage1) {
+      for (int p = 0; p < 5; p++) {
+        L.vsm[(long)p * L.cap + q] = Vsm[p];
+        L.vsp[(long)p * L.cap + q] = Vsp[p];
+      }
+      L.err[q] = err;
+    }
+    L.stats[q] = R.st.it_main;
+    L.stats[L.cap + q] = R.st.fevals;
+    double f1[5], f2[5];
+    int ferr = 0;
+    if (s.mp_flux == M2CX_MPFLUX_EXACT) {
+      flux_phys(M.m[mid], d, Vs, f1);
+      for (int p = 0; p < 5; p++) f2[p] = f1[p];
+    } else {
+      flux_num(M.m[idl], s.flux, s.del, d, Vm, Vsm, f1, ferr);
+      flux_num(M.m[idr], s.flux, s.del, d, Vsp, Vp, f2, ferr);
+    }
+    for (int p = 0; p < 5; p++) {
+      L.f1[(long)p * L.cap + q] = f1[p];
+      L.f2[(long)p * L.cap + q] = f2[p];
+    }
+    if (ferr) fatal = 1;
+  }
+  if (calls) {
+    atomicAdd(&cnt->riemann_calls, calls);
+    atomicAdd(&cnt->riemann_trivial, triv);
+    atomicAdd(&cnt->riemann_errors, errs);
+    atomicAdd(&cnt->riemann_iters, its);
+    atomicAdd(&cnt->riemann_fevals, fev);
+    atomicAdd(&cnt->riemann_fallbacks, fb);
+    atomicAdd(&cnt->guess_fallbacks, gfb);
+  }
+  if (fatal) atomicOr(&cnt->error, 2);
+  if (ovf) atomicOr(&cnt->error, 4);
+}
+
+// point-kernel entry: n problems given explicitly
+__global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, const double* dir, const double* Vm,
+                                                       const int* idm, const double* Vp, const int* idp,
+                                                       double* Vs, int* id, double* Vsm, double* Vsp, int* err,
+                                                       int* iters, int* fevals, double* slab, int lanes,
+                                                       int pathcap, DevCounters* cnt) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= lanes) return;
+  DevRiemann R(c_mats, rc, slab + lane, lanes, pathcap);
+  for (int q = lane; q < n; q += lanes) {
+    R.st = RStats{0, 0, false, false, false};
+    R.path1.overflow = R.path3.overflow = false;
+    int mid = idm[q];
+    err[q] = R.solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q, Vsp + 5 * q);
+    id[q] = mid;
+    iters[q] = R.st.it_main;
+    fevals[q] = R.st.fevals;
+    if (R.st.fatal) atomicOr(&cnt->error, 2);
+    if (R.path1.overflow || R.path3.overflow) atomicOr(&cnt->error, 4);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// level set: FD3 residual (LevelSetOperator.cpp:796-870) fused with the RK
+// combination of the stage (TimeIntegrator.cpp:376-430, 581-690)
+// ---------------------------------------------------------------------------
+struct LSArgs {
+  DevGrid g;
+  CF5 V;
+  const double* phi_st;   // stencil field of this stage
+  const double* phi_a;    // phi(n)
+  const double* phi_b;    // phi1 (own-cell term)
+  double* out;
+  double* rls;
+  const double* fd3;
+  long ocxL, ocxR, ocyL, ocyR3, ocyR1, oczL, oczR;
+  int which;
+  const DevCounters* cnt;
+};
+
+__global__ void k_levelset(LSArgs a) {
+  const DevGrid& g = a.g;
+  long n = (long)g.NX * g.NY * g.NZ;
+  const double dt = a.cnt->dt;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    const double* P = a.phi_st;
+    const long sx = 1, sy = g.SX, sz = g.S;
+    auto fd = [&](long off, int idx, long st, long c0) {
+      const double* cc = a.fd3 + off + 4L * idx;
+      return cc[0] * P[c0] + cc[1] * P[c0 + st] + cc[2] * P[c0 + 2 * st] + cc[3] * P[c0 + 3 * st];
+    };
+    const double xi = g.x[i + 1], yj = g.y[j + 1], zk = g.z[k + 1];
+    double av = (i - 2 >= -1) ? fd(a.ocxL, i, sx, c - 2 * sx) : (P[c] - P[c - sx]) / (xi - g.x[i]);
+    double bv = (i + 2 <= g.NX) ? fd(a.ocxR, i, sx, c - sx) : (P[c + sx] - P[c]) / (g.x[i + 2] - xi);
+    double cv = (j - 2 >= -1) ? fd(a.ocyL, j, sy, c - 2 * sy) : (P[c] - P[c - sy]) / (yj - g.y[j]);
+    double dv = (j + 2 <= g.NY) ? fd((i < g.NY - 1) ? a.ocyR3 : a.ocyR1, j, sy, c - sy)
+                                : (P[c + sy] - P[c]) / (g.y[j + 2] - yj);
+    double ev = (k - 2 >= -1) ? fd(a.oczL, k, sz, c - 2 * sz) : (P[c] - P[c - sz]) / (zk - g.z[k]);
+    double fv = (k + 2 <= g.NZ) ? fd(a.oczR, k, sz, c - sz) : (P[c + sz] - P[c]) / (g.z[k + 2] - zk);
+    const double u = a.V.c[1][c], v = a.V.c[2][c], w = a.V.c[3][c];
+    double r = (u >= 0 ? av * u : bv * u) + (v >= 0 ? cv * v : dv * v) + (w >= 0 ? ev * w : fv * w);
+    r = r * -1.0;
+    double o;
+    switch (a.which) {
+      case 0: o = 1.0 * a.phi_a[c] + dt * r; break;                                    // FE
+      case 1: o = 1.0 * a.phi_a[c] + dt * r; break;                                    // RK stage 1
+      case 2: o = 0.5 * a.phi_a[c] + 0.5 * a.phi_b[c]; o = 1.0 * o + 0.5 * dt * r; break;  // RK2 final
+      case 3: o = 0.25 * a.phi_b[c] + 0.75 * a.phi_a[c]; o = 1.0 * o + 0.25 * dt * r; break;  // RK3 stage 2
+      case 4: o = 1.0 / 3.0 * a.phi_a[c] + 2.0 / 3.0 * a.phi_b[c]; o = 1.0 * o + 2.0 / 3.0 * dt * r; break;
+      default: o = 0.0;
+    }
+    if (a.which < 0) a.rls[c] = r;
+    else a.out[c] = o;
+  }
+}
+
+// LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:408-508)
+__global__ void k_ls_bc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, int b4, int b5) {
+  const int bcs[6] = {b0, b1, b2, b3, b4, b5};
+  long nfaces = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nfaces; q += (long)gridDim.x * blockDim.x) {
+    long r = q;
+    int side, i, j, k;
+    long nyz = (long)g.NY * g.NZ, nxz = (long)g.NX * g.NZ, nxy = (long)g.NX * g.NY;
+    if (r < 2 * nyz) { side = (int)(r / nyz); r %= nyz; j = r % g.NY; k = r / g.NY; i = side == 0 ? -1 : g.NX; }
+    else if ((r -= 2 * nyz) < 2 * nxz) { side = 2 + (int)(r / nxz); r %= nxz; i = r % g.NX; k = r / g.NX; j = side == 2 ? -1 : g.NY; }
+    else { r -= 2 * nxz; side = 4 + (int)(r / nxy); r %= nxy; i = r % g.NX; j = r / g.NX; k = side == 4 ? -1 : g.NZ; }
+    int ii = i, jj = j, kk = k;
+    if (side == 0) ii = 0; else if (side == 1) ii = g.NX - 1;
+    else if (side == 2) jj = 0; else if (side == 3) jj = g.NY - 1;
+    else if (side == 4) kk = 0; else kk = g.NZ - 1;
+    const int bc = bcs[side];
+    long c = gidx(g, i, j, k);
+    double im = phi[gidx(g, ii, jj, kk)];
+    double ph = phi[c];
+    if (bc == M2CX_LSBC_ZERO_NEUMANN) {
+      ph = im;
+    } else if (bc == M2CX_LSBC_LINEAR_EXTRAPOLATION || bc == M2CX_LSBC_NON_NEGATIVE) {
+      const int ax = side / 2;
+      const int N = ax == 0 ? g.NX : (ax == 1 ? g.NY : g.NZ);
+      const int a = ax == 0 ? i : (ax == 1 ? j : k);
+      const int dirn = (side % 2 == 0) ? 1 : -1;
+      const double* X = ax == 0 ? g.x : (ax == 1 ? g.y : g.z);
+      const long st = ax == 0 ? 1 : (ax == 1 ? g.SX : g.S);
+      bool ok = (side % 2 == 0) ? (a + 2 < N) : (a - 2 >= 0);
+      if (ok) {
+        double rr = X[a + 1], r1 = X[a + dirn + 1], r2 = X[a + 2 * dirn + 1];
+        double f1 = phi[c + dirn * st], f2 = phi[c + 2 * dirn * st];
+        ph = f1 + (f2 - f1) / (r2 - r1) * (rr - r1);
+      } else {
+        ph = im;
+      }
+      if (bc == M2CX_LSBC_NON_NEGATIVE) {
+        const int ai = ax == 0 ? ii : (ax == 1 ? jj : kk);
+        double d2wall = 0.5 * fabs(X[ai + 1] - X[a + 1]);
+        double m = -d2wall;
+        ph = (m < ph) ? ph : m;  // std::max(-d2wall, phi)
+      }
+    }
+    phi[c] = ph;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// end of step
+// ---------------------------------------------------------------------------
+__global__ void k_end_reset(DevCounters* cnt) {
+  cnt->n_remaining = 0;
+  cnt->n_changed = 0;
+  cnt->n_unresolved = 0;
+}
+
+// UpdateMaterialIDByLevelSet, sign-change part (MultiPhaseOperator.cpp:177-221)
+__global__ void k_id_sweep(DevGrid g, Scheme s, const double* __restrict__ phi0, const double* __restrict__ phi,
+                           unsigned char* ID, unsigned char* tag, int* remaining, int cap, int lsid,
+                           DevCounters* cnt) {
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    if (ID[c] == s.inactive) continue;
+    double a = phi[c], b = phi0[c];
+    if (a * b > 0.0) continue;
+    if (a < 0) {
+      ID[c] = (unsigned char)lsid;
+    } else {
+      tag[c] = 1;
+      int slot = atomicAdd(&cnt->n_remaining, 1);
+      if (slot < cap) remaining[slot] = (int)c;
+      else atomicOr(&cnt->error, 4);
+    }
+  }
+}
+
+// Neighbour vote for swept cells outside every level-set subdomain
+// (MultiPhaseOperator.cpp:236-338), Jacobi sweeps in one work-group.  With a
+// single material not tracked by a level set (all configurations here) every
+// candidate set is that material or empty, so the result is the same as the
+// reference's in-order sweep.
+__global__ void __launch_bounds__(1024) k_id_vote(DevGrid g, Scheme s, unsigned char* ID, unsigned char* tag,
+                                                  const int* remaining, int cap, int lsid, DevCounters* cnt) {
+  __shared__ int progress, left;
+  const int nrem = min(cnt->n_remaining, cap);
+  for (int iter = 0; iter < 100; iter++) {
+    if (threadIdx.x == 0) { progress = 0; left = 0; }
+    __syncthreads();
+    // phase 1: find candidates (read only)
+    for (int q = threadIdx.x; q < nrem; q += blockDim.x) {
+      long c = remaining[q];
+      if (!tag[c]) continue;
+      atomicAdd(&left, 1);
+    }
+    __syncthreads();
+    if (left == 0) break;
+    // candidate of each remaining node into the high bits of its tag
+    for (int q = threadIdx.x; q < nrem; q += blockDim.x) {
+      long c = remaining[q];
+      if (!(tag[c] & 1)) continue;
+      long r = c;
+      int i = (int)(r % g.SX) - 1; r /= g.SX;
+      int j = (int)(r % g.SY) - 1;
+      int k = (int)(r / g.SY) - 1;
+      int best = 256;
+      for (int dk = -1; dk <= 1; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+          for (int di = -1; di <= 1; di++) {
+            int ni = i + di, nj = j + dj, nk = k + dk;
+            if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
+            long nc = gidx(g, ni, nj, nk);
+            if (tag[nc] & 1) continue;
+            int nid = ID[nc];
+            if (nid == s.inactive || nid == lsid) continue;
+            if (nid < best) best = nid;
+          }
+      if (best < 256) tag[c] = (unsigned char)(1 | 2);  // decided this sweep
+      if (best < 256) {
+        // remember the candidate: the node's own ID is not read by others
+        // while it is tagged, so it can hold it now
+        ID[c] = (unsigned char)best;
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nrem; q += blockDim.x) {
+      long c = remaining[q];
+      if (tag[c] == 3) {
+        tag[c] = 0;
+        atomicAdd(&progress, 1);
+      }
+    }
+    __syncthreads();
+    if (progress == 0) {
+      for (int q = threadIdx.x; q < nrem; q += blockDim.x) {
+        long c = remaining[q];
+        if (tag[c] & 1) ID[c] = 0;  // orphans (MultiPhaseOperator.cpp:323-333)
+      }
+      break;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nrem; q += blockDim.x) tag[remaining[q]] = 0;
+}
+
+// UpdateMaterialIDAtGhostNodes (MultiPhaseOperator.cpp:919-942)
+__global__ void k_ghost_ids(DevGrid g, unsigned char* ID) {
+  long nyz = (long)g.NY * g.NZ, nxz = (long)g.NX * g.NZ, nxy = (long)g.NX * g.NY;
+  long nfaces = 2 * (nyz + nxz + nxy);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nfaces; q += (long)gridDim.x * blockDim.x) {
+    long r = q;
+    int i, j, k, ii, jj, kk;
+    if (r < 2 * nyz) { int sd = (int)(r / nyz); r %= nyz; j = r % g.NY; k = r / g.NY; i = sd ? g.NX : -1; ii = sd ? g.NX - 1 : 0; jj = j; kk = k; }
+    else if ((r -= 2 * nyz) < 2 * nxz) { int sd = (int)(r / nxz); r %= nxz; i = r % g.NX; k = r / g.NX; j = sd ? g.NY : -1; jj = sd ? g.NY - 1 : 0; ii = i; kk = k; }
+    else { r -= 2 * nxz; int sd = (int)(r / nxy); r %= nxy; i = r % g.NX; j = r / g.NX; k = sd ? g.NZ : -1; kk = sd ? g.NZ - 1 : 0; ii = i; jj = j; }
+    ID[gidx(g, i, j, k)] = ID[gidx(g, ii, jj, kk)];
+  }
+}
+
+__global__ void k_changed(DevGrid g, Scheme s, const unsigned char* __restrict__ ID,
+                          const unsigned char* __restrict__ IDn, int* changed, int cap, DevCounters* cnt) {
+  long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    long c = gidx(g, i, j, k);
+    if (ID[c] == IDn[c] || ID[c] == s.inactive) continue;
+    int slot = atomicAdd(&cnt->n_changed, 1);
+    if (slot < cap) changed[slot] = (int)c;
+    else atomicOr(&cnt->error, 4);
+  }
+}
+
+struct UpdArgs {
+  DevGrid g;
+  Scheme s;
+  F5 V;
+  const unsigned char* ID;
+  const unsigned char* IDn;
+  const int* fidx[3];
+  IfaceList L;
+  const int* changed;
+  int* unresolved;
+  unsigned char* done;
+  int cap;
+  DevCounters* cnt;
+};
+
+// MultiPhaseOperator::LocalUpdateByRiemannSolutions (:1039-1150)
+__device__ bool local_update(const UpdArgs& a, long c, int id, bool upwind) {
+  const DevGrid& g = a.g;
+  const long st[3] = {1, g.SX, g.S};
+  int counter = 0;
+  double sum_weight = 0.0;
+  double v[5];
+  for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+  long r = c;
+  const int i = (int)(r % g.SX) - 1; r /= g.SX;
+  const int j = (int)(r % g.SY) - 1;
+  const int k = (int)(r / g.SY) - 1;
+  const int ijk[3] = {i, j, k};
+  const int N[3] = {g.NX, g.NY, g.NZ};
+  const bool llf = a.s.mp_flux == M2CX_MPFLUX_LLF;
+  for (int q = 0; q < 6; q++) {
+    const int d = q / 2;
+    const bool plus = q & 1;          // left/bottom/back: minus side of c; right/top/front: plus side
+    const long nb = plus ? c + st[d] : c - st[d];
+    const long fc = plus ? nb : c;    // plus-side cell of the face (owner of the slot)
+    const int fa = plus ? ijk[d] + 1 : ijk[d];
+    if (llf || fa < 1 || fa > N[d] - 1) continue;   // interior faces only hold Riemann solutions
+    const int idm = a.IDn[fc - st[d]], idp = a.IDn[fc];
+    if (idm == idp || idm == a.s.inactive || idp == a.s.inactive) continue;
+    const int slot = a.fidx[d][fc];
+    if (a.L.err[slot] != 0) continue;
+    const int sid = plus ? idp : idm;                // ID stored with the star state
+    if (sid != id) continue;
+    const double vn = a.V.c[d + 1][nb];
+    if (upwind && !(plus ? (vn < 0) : (vn > 0))) continue;
+    const double* sv = plus ? a.L.vsp : a.L.vsm;
+    double weight = 1.0;
+    if (upwind) {
+      const double u1 = a.V.c[1][nb], u2 = a.V.c[2][nb], u3 = a.V.c[3][nb];
+      const double nrm = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+      weight = (plus ? -vn : vn) / nrm;
+    }
+    sum_weight += weight;
+    if (counter == 0)
+      for (int p = 0; p < 5; p++) v[p] = weight * sv[(long)p * a.L.cap + slot];
+    else
+      for (int p = 0; p < 5; p++) v[p] += weight * sv[(long)p * a.L.cap + slot];
+    counter++;
+  }
+  if (sum_weight > 0.0) {
+    const double cst = 1.0 / sum_weight;
+    for (int p = 0; p < 5; p++) v[p] *= cst;
+  }
+  if (counter > 0)
+    for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+  return counter > 0;
+}
+
+// UpdateStateVariablesByRiemannSolutions (:980-1034) in dependency rounds:
+// the reference sweeps cells in (k,j,i) order updating V in place, so a cell
+// sees new values only at its i-1, j-1, k-1 neighbours.  A cell is processed
+// once those of them that also changed are done; every other neighbour it
+// reads is still unmodified, exactly as in the sweep.
+__global__ void __launch_bounds__(1024) k_state_update(UpdArgs a) {
+  __shared__ int left;
+  const DevGrid& g = a.g;
+  const int nch = min(a.cnt->n_changed, a.cap);
+  const long st[3] = {1, g.SX, g.S};
+  for (int round = 0; round < 1 << 20; round++) {
+    if (threadIdx.x == 0) left = 0;
+    __syncthreads();
+    // phase 1: who is ready (reads done flags only)
+    constexpr int PER = 16;
+    int ready[PER];
+    int nready = 0;
+    for (int q = threadIdx.x; q < nch; q += blockDim.x) {
+      long c = a.changed[q];
+      if (a.done[c]) continue;
+      atomicAdd(&left, 1);
+      bool ok = true;
+      for (int d = 0; d < 3; d++) {
+        long nb = c - st[d];
+        if (a.ID[nb] != a.IDn[nb] && a.ID[nb] != a.s.inactive && !a.done[nb]) {
+          // only interior neighbours are swept by the reference's loop
+          long r = nb;
+          int ii = (int)(r % g.SX) - 1; r /= g.SX;
+          int jj = (int)(r % g.SY) - 1;
+          int kk = (int)(r / g.SY) - 1;
+          if (ii >= 0 && jj >= 0 && kk >= 0) ok = false;
+        }
+      }
+      if (ok && nready < PER) ready[nready++] = q;
+    }
+    __syncthreads();
+    if (left == 0) break;
+    // phase 2: process the ready ones
+    for (int r = 0; r < nready; r++) {
+      long c = a.changed[ready[r]];
+      int id = a.ID[c];
+      bool res = local_update(a, c, id, true);
+      if (!res) res = local_update(a, c, id, false);
+      if (!res) {
+        int slot = atomicAdd(&a.cnt->n_unresolved, 1);
+        if (slot < a.cap) a.unresolved[slot] = (int)c;
+      }
+      a.done[c] = 1;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// FixUnresolvedNodes (:1278-1525) with apply_failsafe_density = On and
+// phase-change direction UPWIND; one lane, nodes in (k,j,i) order, with the
+// reference's `reset` flag shared by all nodes.
+__global__ void k_fix_unresolved(UpdArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nu = min(a.cnt->n_unresolved, a.cap);
+  if (nu == 0) return;
+  const DevGrid& g = a.g;
+  int* U = a.unresolved;
+  for (int x = 1; x < nu; x++) {  // insertion sort by linear index = (k,j,i) order
+    int v = U[x], y = x - 1;
+    while (y >= 0 && U[y] > v) { U[y + 1] = U[y]; y--; }
+    U[y + 1] = v;
+  }
+  auto unres = [&](long c) {
+    int lo = 0, hi = nu - 1;
+    while (lo <= hi) { int m = (lo + hi) / 2; if (U[m] == c) return true; if (U[m] < c) lo = m + 1; else hi = m - 1; }
+    return false;
+  };
+  bool reset = false;
+  for (int u = 0; u < nu; u++) {
+    long c = U[u];
+    long r = c;
+    const int i = (int)(r % g.SX) - 1; r /= g.SX;
+    const int j = (int)(r % g.SY) - 1;
+    const int k = (int)(r / g.SY) - 1;
+    const int myid = a.ID[c];
+    const double x0[3] = {g.x[i + 1], g.y[j + 1], g.z[k + 1]};
+    double sum_weight = 0.0, sum_weight2 = 0.0, vtmp[5] = {0, 0, 0, 0, 0};
+    double v[5];
+    for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+    for (int nk = k - 1; nk <= k + 1; nk++)
+      for (int nj = j - 1; nj <= j + 1; nj++)
+        for (int ni = i - 1; ni <= i + 1; ni++) {
+          if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
+          long nc = gidx(g, ni, nj, nk);
+          if (a.ID[nc] != myid) continue;
+          if (nc == c) continue;
+          if (unres(nc)) continue;
+          double vn[5];
+          for (int p = 0; p < 5; p++) vn[p] = a.V.c[p][nc];
+          double v1[3] = {vn[1], vn[2], vn[3]};
+          double v1n = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+          if (v1n != 0) { double cs = 1.0 / v1n; v1[0] *= cs; v1[1] *= cs; v1[2] *= cs; }
+          double xx[3] = {x0[0] - g.x[ni + 1], x0[1] - g.y[nj + 1], x0[2] - g.z[nk + 1]};
+          double xn = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+          double cs = 1.0 / xn;
+          xx[0] *= cs; xx[1] *= cs; xx[2] *= cs;
+          double dot = xx[0] * v1[0] + xx[1] * v1[1] + xx[2] * v1[2];
+          double weight = (0.0 < dot) ? dot : 0.0;  // std::max(0.0, dot)
+          if (weight > 0) {
+            sum_weight += weight;
+            if (reset) for (int p = 0; p < 5; p++) v[p] += weight * vn[p];
+            else { for (int p = 0; p < 5; p++) v[p] = weight * vn[p]; reset = true; }
+          }
+          for (int p = 0; p < 5; p++) vtmp[p] += 1.0 / xn * vn[p];
+          sum_weight2 += 1.0 / xn;
+        }
+    bool fixed = false;
+    if (sum_weight > 0) {
+      double cs = 1.0 / sum_weight;
+      for (int p = 0; p < 5; p++) v[p] *= cs;
+      fixed = true;
+    } else if (sum_weight2 > 0) {
+      for (int p = 0; p < 5; p++) v[p] = vtmp[p] / sum_weight2;
+      fixed = true;
+    }
+    if (!fixed) {
+      double density = 0.0;
+      for (int layer = 1; layer <= 10; layer++) {
+        for (int nk = k - layer; nk <= k + layer; nk++)
+          for (int nj = j - layer; nj <= j + layer; nj++)
+            for (int ni = i - layer; ni <= i + layer; ni++) {
+              if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
+              long nc = gidx(g, ni, nj, nk);
+              if (a.ID[nc] != myid) continue;
+              if (nc == c) continue;
+              if (unres(nc)) continue;
+              double dx0 = g.x[ni + 1] - x0[0], dx1 = g.y[nj + 1] - x0[1], dx2 = g.z[nk + 1] - x0[2];
+              double dist = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+              sum_weight += 1.0 / dist;
+              density += (1.0 / dist) * a.V.c[0][nc];
+            }
+        if (sum_weight > 0) { v[0] = density / sum_weight; break; }
+      }
+      if (sum_weight == 0) v[0] = c_mats.m[myid].failsafe;
+    }
+    for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+  }
+}
+
+// ClipDensityAndPressure + ApplyBoundaryConditions after the update
+// (TimeIntegrator.cpp:802-803), on the cells that changed (all others are
+// unchanged since their last clip), and clearing of the done flags.
+__global__ void k_post_update(UpdArgs a) {
+  const int nch = min(a.cnt->n_changed, a.cap);
+  const DevGrid& g = a.g;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nch; q += gridDim.x * blockDim.x) {
+    long c = a.changed[q];
+    a.done[c] = 0;
+    double v[5];
+    for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+    const DevEos& E = c_mats.m[a.ID[c]];
+    if (eos_clip(E, v)) {
+      atomicAdd(&a.cnt->clipped, 1ull);
+      for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+    }
+    if (eos_check5(E, v)) atomicOr(&a.cnt->error, 1);
+    long r = c;
+    const int i = (int)(r % g.SX) - 1; r /= g.SX;
+    const int j = (int)(r % g.SY) - 1;
+    const int k = (int)(r / g.SY) - 1;
+    write_ghosts(g, a.V, i, j, k, v);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.cnt->id_changed, (unsigned long long)nch);
+}
+
+// ---------------------------------------------------------------------------
+// host-side orchestration
+// ---------------------------------------------------------------------------
+static inline int blocks_for(long n, int bs = 256) { return (int)std::min<long>((n + bs - 1) / bs, 256L * 64L); }
+
+inline void mp_classify(MPBuffers& B, const DevGrid& g, const Scheme& s, const unsigned char* ID, DevCounters* cnt,
+                        hipStream_t st) {
+  k_if_reset<<<1, 1, 0, st>>>(cnt);
+  k_classify<<<blocks_for((long)g.NX * g.NY * g.NZ), 256, 0, st>>>(g, s, ID, B.fidx[0], B.fidx[1], B.fidx[2], B.L,
+                                                                     cnt);
+}
+
+inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, const unsigned char* ID, int stage1,
+                       DevCounters* cnt, hipStream_t st) {
+  if (s.mp_flux == M2CX_MPFLUX_LLF) return;
+  k_riemann<<<(B.lanes + 127) / 128, 128, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
+                                                   B.guess ? B.pguess : nullptr, cnt);
+}
+
+inline void mp_begin_step(MPBuffers& B, const DevGrid& g, const Scheme& s, const unsigned char* ID, DevCounters* cnt,
+                          hipStream_t st) {
+  mp_classify(B, g, s, ID, cnt, st);
+  (void)hipMemcpyAsync(B.IDn, ID, g.ncells, hipMemcpyDeviceToDevice, st);
+  if (B.nls) (void)hipMemcpyAsync(B.phin, B.phi, g.ncells * 8, hipMemcpyDeviceToDevice, st);
+}
+
+// which: -1 residual only, 0 FE, 1 RK stage 1, 2 RK2 final, 3 RK3 stage 2, 4 RK3 final
+inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCounters* cnt, hipStream_t st) {
+  if (!B.nls) return;
+  LSArgs a{};
+  a.g = g;
+  a.V = V;
+  a.fd3 = B.fd3;
+  a.ocxL = B.off_cxL; a.ocxR = B.off_cxR; a.ocyL = B.off_cyL; a.ocyR3 = B.off_cyR3; a.ocyR1 = B.off_cyR1;
+  a.oczL = B.off_czL; a.oczR = B.off_czR;
+  a.which = which;
+  a.cnt = cnt;
+  a.rls = B.rls;
+  double* written = nullptr;
+  switch (which) {
+    case -1: a.phi_st = B.phi; break;
+    case 0: a.phi_st = B.phi; a.phi_a = B.phi; a.out = B.phitmp; written = B.phitmp; break;
+    case 1: a.phi_st = B.phi; a.phi_a = B.phi; a.out = B.phi1; written = B.phi1; break;
+    case 2: a.phi_st = B.phi1; a.phi_a = B.phi; a.phi_b = B.phi1; a.out = B.phi; written = B.phi; break;
+    case 3: a.phi_st = B.phi1; a.phi_a = B.phi; a.phi_b = B.phi1; a.out = B.phitmp; written = B.phitmp; break;
+    case 4: a.phi_st = B.phi1; a.phi_a = B.phi; a.phi_b = B.phi1; a.out = B.phi; written = B.phi; break;
+  }
+  k_levelset<<<blocks_for((long)g.NX * g.NY * g.NZ), 256, 0, st>>>(a);
+  if (which == 0) std::swap(B.phi, B.phitmp);
+  if (which == 3) std::swap(B.phi1, B.phitmp);
+  if (written) {
+    long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+    double* tgt = (which == 0) ? B.phi : (which == 3 ? B.phi1 : written);
+    k_ls_bc<<<blocks_for(nf), 256, 0, st>>>(g, tgt, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3], B.lsbc[4],
+                                           B.lsbc[5]);
+  }
+}
+
+inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
+                        hipStream_t st) {
+  const long nin = (long)g.NX * g.NY * g.NZ;
+  k_end_reset<<<1, 1, 0, st>>>(cnt);
+  k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, s, B.phin, B.phi, ID, B.tag, B.remaining, B.list_cap, B.lsid,
+                                              cnt);
+  k_id_vote<<<1, 1024, 0, st>>>(g, s, ID, B.tag, B.remaining, B.list_cap, B.lsid, cnt);
+  long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+  k_ghost_ids<<<blocks_for(nf), 256, 0, st>>>(g, ID);
+  k_changed<<<blocks_for(nin), 256, 0, st>>>(g, s, ID, B.IDn, B.changed, B.list_cap, cnt);
+  UpdArgs a{};
+  a.g = g;
+  a.s = s;
+  a.V = V;
+  a.ID = ID;
+  a.IDn = B.IDn;
+  for (int d = 0; d < 3; d++) a.fidx[d] = B.fidx[d];
+  a.L = B.L;
+  a.changed = B.changed;
+  a.unresolved = B.unresolved;
+  a.done = B.done;
+  a.cap = B.list_cap;
+  a.cnt = cnt;
+  k_state_update<<<1, 1024, 0, st>>>(a);
+  k_fix_unresolved<<<1, 64, 0, st>>>(a);
+  k_post_update<<<256, 256, 0, st>>>(a);
+}
+
+inline int mp_riemann_batch(MPBuffers& B, const Scheme& s, int n, const double* dir, const double* Vm,
+                            const int* idm, const double* Vp, const int* idp, double* Vs, int* id, double* Vsm,
+                            double* Vsp, int* err, int* iters, int* fevals, DevCounters* cnt, hipStream_t st,
+                            std::string* msg) {
+  (void)s;
+  double *dd, *dVm, *dVp, *dVs, *dVsm, *dVsp;
+  int *didm, *didp, *did, *derr, *dit, *dfe;
+  const long N = n;
+  if (hipMalloc(&dd, N * 24) || hipMalloc(&dVm, N * 40) || hipMalloc(&dVp, N * 40) || hipMalloc(&dVs, N * 40) ||
+      hipMalloc(&dVsm, N * 40) || hipMalloc(&dVsp, N * 40) || hipMalloc(&didm, N * 4) || hipMalloc(&didp, N * 4) ||
+      hipMalloc(&did, N * 4) || hipMalloc(&derr, N * 4) || hipMalloc(&dit, N * 4) || hipMalloc(&dfe, N * 4)) {
+    if (msg) *msg = "out of device memory";
+    return M2CX_E_OOM;
+  }
+  (void)hipMemcpy(dd, dir, N * 24, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dVm, Vm, N * 40, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dVp, Vp, N * 40, hipMemcpyHostToDevice);
+  (void)hipMemcpy(didm, idm, N * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(didp, idp, N * 4, hipMemcpyHostToDevice);
+  k_riemann_batch<<<(B.lanes + 127) / 128, 128, 0, st>>>(B.rc, n, dd, dVm, didm, dVp, didp, dVs, did, dVsm, dVsp,
+                                                         derr, dit, dfe, B.slab, B.lanes, B.pathcap, cnt);
+  hipError_t e = hipStreamSynchronize(st);
+  (void)hipMemcpy(Vs, dVs, N * 40, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Vsm, dVsm, N * 40, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Vsp, dVsp, N * 40, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(id, did, N * 4, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(err, derr, N * 4, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(iters, dit, N * 4, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(fevals, dfe, N * 4, hipMemcpyDeviceToHost);
+  for (void* p : {(void*)dd, (void*)dVm, (void*)dVp, (void*)dVs, (void*)dVsm, (void*)dVsp, (void*)didm, (void*)didp,
+                  (void*)did, (void*)derr, (void*)dit, (void*)dfe})
+    (void)hipFree(p);
+  if (e != hipSuccess) {
+    if (msg) *msg = hipGetErrorString(e);
+    return M2CX_E_HIP;
+  }
+  return 0;
+}
+
+}  // namespace m2c

@@ -1,0 +1,132 @@
+"""GPU parity of the multi-material step (exact interface Riemann solver,
+level set, end-of-step material update) against the CPU oracle, bit-exact,
+plus the reference's Test1/Test6 line fixtures replayed on the GPU."""
+import numpy as np
+import pytest
+
+import deck as dk
+import oracle_lib as ol
+from m2c_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _rt():
+    from m2c_amd import runtime
+    return runtime
+
+
+def _random_riemann(n, seed, mats_kind):
+    rng = np.random.default_rng(seed)
+    if mats_kind == "sg":
+        mats = abi.materials(abi.material("SG", gamma=6.12, pc=3.43e8), abi.material("SG", gamma=1.4))
+        rho = [1000.0, 1.2]
+        p = 1e5
+    elif mats_kind == "sg14":
+        mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.667))
+        rho = [1.0, 0.3]
+        p = 1.0
+    elif mats_kind == "nasg":
+        mats = abi.materials(abi.material("NASG", gamma=1.19, pc=6.2178e8, b=6.7212e-4, q=-1.177788e6),
+                             abi.material("SG", gamma=1.4))
+        rho = [1000.0, 1.2]
+        p = 1e5
+    else:  # mg
+        mats = abi.materials(abi.material("MG", rho0=8.96e-3, c0=3.97e6, s=1.479, Gamma0=2.12),
+                             abi.material("MG", rho0=2.204e-3, c0=2.22e6, s=1.61, Gamma0=0.65))
+        rho = [8.96e-3, 2.204e-3]
+        p = 1e5
+    Vm = np.zeros((n, 5))
+    Vp = np.zeros((n, 5))
+    for V, r0 in ((Vm, rho[0]), (Vp, rho[1])):
+        V[:, 0] = r0 * np.exp(rng.uniform(-0.5, 0.5, n))
+        V[:, 4] = p * np.exp(rng.uniform(-1.0, 3.0, n))
+        cs = np.sqrt(V[:, 4] / V[:, 0])
+        V[:, 1:4] = rng.uniform(-0.5, 0.5, (n, 3)) * cs[:, None]
+    if mats_kind == "mg":
+        Vm[:, 1] += 1e6
+    dirs = np.zeros((n, 3))
+    dirs[np.arange(n), rng.integers(0, 3, n)] = 1.0
+    idm = np.zeros(n, dtype=np.int32)
+    idp = np.ones(n, dtype=np.int32)
+    # a few trivial problems (equal normal velocity and pressure)
+    Vp[:8, 4] = Vm[:8, 4]
+    Vp[:8, 1:4] = Vm[:8, 1:4]
+    return mats, dirs, Vm, idm, Vp, idp
+
+
+@pytest.mark.parametrize("kind", ["sg", "sg14", "nasg", "mg"])
+def test_riemann_batch_bitwise(kind):
+    mats, dirs, Vm, idm, Vp, idp = _random_riemann(2000, 17, kind)
+    s = abi.default_scheme(riem_tol_main=1e-10 if kind == "sg" else 1e-5)
+    gpu = _rt().Solver(abi.grid((4, 4, 4)), mats, 2, s)
+    g = gpu.riemann_batch(dirs, Vm, idm, Vp, idp)
+    o = ol.riemann_batch(mats, 2, s, dirs, Vm, idm, Vp, idp)
+    for key in ("Vs", "Vsm", "Vsp", "id", "err", "iters", "fevals"):
+        a, b = g[key], o[key]
+        bad = np.nonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))[0]
+        assert bad.size == 0, (kind, key, bad[:5], a[bad[:3]], b[bad[:3]])
+
+
+@pytest.mark.parametrize("name", ["Test1", "Test6"])
+def test_line_deck_on_gpu(fixtures, name):
+    d = fixtures[name]["deck"]
+    g, mats, nmat, s, V, ID, Phi = dk.setup(d)
+    o = ol.Oracle(g, mats, nmat, s)
+    o.set_state(V, ID, Phi)
+    o.apply_bc()
+    o.update_ghost_ids()
+    o.ls_apply_bc()
+    V0, ID0, Phi0 = o.get_state()
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(V0, ID0, Phi0)
+    nsteps = len(fixtures[name]["log_steps"])
+    for step in range(nsteps):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        Vg, IDg, Pg = gpu.get_state()
+        Vo, IDo, Po = o.get_state()
+        ok = (np.array_equal(Vg[1:-1, 1:-1, 1:-1], Vo[1:-1, 1:-1, 1:-1]) and np.array_equal(IDg[1:-1, 1:-1, 1:-1], IDo[1:-1, 1:-1, 1:-1])
+              and np.array_equal(Pg[1:-1, 1:-1, 1:-1], Po[1:-1, 1:-1, 1:-1]))
+        assert ok, (name, step, np.max(np.abs(Vg - Vo)), np.max(np.abs(Pg - Po)), np.sum(IDg != IDo))
+    assert gpu.time()[0] == o.time()[0]
+    rows = np.array(fixtures[name]["line_rows"])
+    assert np.max(np.abs(Vg[1, 1, 1:-1, 0] - rows[:, 1])) <= 3e-8 * np.max(rows[:, 1])
+    assert np.array_equal(IDg[1, 1, 1:-1], rows[:, 7].astype(np.int32))
+
+
+@pytest.mark.parametrize("ti", [abi.TI_RK2, abi.TI_RK3])
+def test_shock_bubble_steps_bitwise(ti):
+    g, mats, nmat, s, V, ID, Phi = configs.c2_shock_bubble(24)
+    s.integrator = ti
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(V, ID, Phi)
+    o = ol.Oracle(g, mats, nmat, s)
+    o.set_state(V, ID, Phi)
+    for step in range(6):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        Vg, IDg, Pg = gpu.get_state()
+        Vo, IDo, Po = o.get_state()
+        assert np.array_equal(IDg[1:-1, 1:-1, 1:-1], IDo[1:-1, 1:-1, 1:-1]), (step, np.sum(IDg != IDo))
+        assert np.array_equal(Pg[1:-1, 1:-1, 1:-1], Po[1:-1, 1:-1, 1:-1]), (step, np.max(np.abs(Pg - Po)))
+        dv = np.abs(Vg[1:-1, 1:-1, 1:-1] - Vo[1:-1, 1:-1, 1:-1])
+        assert np.array_equal(Vg[1:-1, 1:-1, 1:-1], Vo[1:-1, 1:-1, 1:-1]), (step, dv.max(), np.argwhere(dv > 0)[:3])
+    cg, co = gpu.counters(), o.counters()
+    assert cg["riemann_calls"] == co["riemann_calls"]
+    assert cg["riemann_main_iters"] == co["riemann_iters"]
+    assert cg["riemann_fevals"] == co["riemann_fevals"]
+    assert cg["id_changed_cells"] == co["id_changed"]
+
+
+def test_levelset_residual_bitwise():
+    g, mats, nmat, s, V, ID, Phi = configs.c2_shock_bubble(20)
+    rng = np.random.default_rng(5)
+    V[..., 1:4] = rng.uniform(-300, 300, V.shape[:3] + (3,))
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(V, ID, Phi)
+    o = ol.Oracle(g, mats, nmat, s)
+    o.set_state(V, ID, Phi)
+    rg = gpu.levelset_residual()
+    ro = o.ls_residual()
+    assert np.array_equal(rg[1:-1, 1:-1, 1:-1], ro[1:-1, 1:-1, 1:-1])
