@@ -250,7 +250,10 @@ int m2cx_numerical_flux_batch(m2cx_solver* h, int n, int dir, const double* Vm,
 /* ExactRiemannSolverBase::ComputeRiemannSolution (ExactRiemannSolverBase.h:42-47,
  * ExactRiemannSolverBase.cpp:59-347) for n problems: dir n x 3, Vm/Vp n x 5,
  * idm/idp n; outputs Vs n x 5, id n, Vsm/Vsp n x 5, err n, and per problem
- * the secant main-loop iteration count and f(p) evaluation count. */
+ * the secant main-loop iteration count and f(p) evaluation count.  A problem
+ * on which the reference exits (invalid input state, NaN in the rarefaction
+ * integration: ExactRiemannSolverBase.cpp:99,110,1326-1335) gets err = -1,
+ * id = -1 and NaN states; the other problems are unaffected. */
 int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir,
                        const double* Vm, const int32_t* idm,
                        const double* Vp, const int32_t* idp,

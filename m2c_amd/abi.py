@@ -18,6 +18,11 @@ MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
+OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY = \
+    0, -1, -2, -3, -4, -5, -6, -7, -8
+
+# Python name -> C name where they differ beyond the M2CX_ prefix
+C_NAMES = {"EOS_SG": "M2CX_EOS_SG", "LIM_GMM": "M2CX_LIM_GENERALIZED_MINMOD", "TI_FE": "M2CX_TI_FORWARD_EULER"}
 
 DBL_MAX = 1.7976931348623157e308
 

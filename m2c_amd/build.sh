@@ -1,7 +1,8 @@
 #!/bin/bash
 # Builds libm2c_amd.so (gfx950) in-tree.  No FMA contraction and no fast-math:
 # results must match the reference's non-contracted x86-64 arithmetic bit for
-# bit.  Packed-FP32 instructions are disabled (see the build notes).
+# bit.  Packed-FP32 instructions are disabled: some of their forms have given
+# wrong results on this hardware.
 set -e
 cd "$(dirname "$0")"
 HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}

@@ -111,7 +111,7 @@ def c2_shock_bubble(n=256, L=1.0, radius=0.15, mach=1.5, shock_x=0.2, tol=1e-10)
     Y = x[None, :, None]
     Z = x[:, None, None]
     dist = np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2) - R
-    Phi = np.ascontiguousarray(np.broadcast_to(dist, V.shape[:3]))
+    Phi = np.array(np.broadcast_to(dist, V.shape[:3]), dtype=np.float64, order="C")
     inside = dist < 0
     V[inside] = [1.2, 0.0, 0.0, 0.0, p_w]
     ID[inside] = 1

@@ -302,7 +302,11 @@ __global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, con
     id[q] = mid;
     iters[q] = R.st.it_main;
     fevals[q] = R.st.fevals;
-    if (R.st.fatal) atomicOr(&cnt->error, 2);
+    if (R.st.fatal) {  // the reference exits here; the batch marks the problem instead
+      err[q] = -1;
+      id[q] = -1;
+      for (int p = 0; p < 5; p++) Vs[5 * q + p] = Vsm[5 * q + p] = Vsp[5 * q + p] = __builtin_nan("");
+    }
     if (R.path1.overflow || R.path3.overflow) atomicOr(&cnt->error, 4);
   }
 }

@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): extern "C" entry points used by tests/
 // (through ctypes) and by bench.py's cpu_baseline leg.  Never linked into the
 // product library.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include "solver.h"
@@ -152,6 +153,11 @@ int orc_riemann_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sc
     err[q] = rs.Solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q,
                       Vsp + 5 * q);
     id[q] = mid;
+    if (rs.st.fatal) {  // the reference exits here; the batch marks the problem instead
+      err[q] = -1;
+      id[q] = -1;
+      for (int p = 0; p < 5; p++) Vs[5 * q + p] = Vsm[5 * q + p] = Vsp[5 * q + p] = std::nan("");
+    }
     if (iters) iters[q] = rs.st.it_main;
     if (fevals) fevals[q] = rs.st.fevals;
     if (flags) flags[q] = (rs.st.trivial ? 1 : 0) | (rs.st.fallback ? 2 : 0) | (rs.st.fatal ? 4 : 0);
