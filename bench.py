@@ -98,6 +98,35 @@ def cpu_baseline(name, budget_s=20.0):
             "sample": sample % steps, "cpu": platform.processor() or platform.machine()}
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_stage launch measured by rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE) for this workload, scaled by the same counters on a copy of
+    known size; kept as a `bytes_per_launch: N` line in
+    profiles/k_stage_<workload>_traffic.txt.  None when no such measurement
+    exists."""
+    path = os.path.join(ROOT, "profiles", "k_stage_%s_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(nbytes):
+    """A plain device copy of `nbytes` (read nbytes, write nbytes), the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +134,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=os.environ.get("M2C_BENCH_CONFIG", "c2"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build does not run the step across GPUs "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -158,7 +192,7 @@ def main():
                     "errors": cnt["riemann_errors"], "riemann_ms_per_step": riem_ms / steps},
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused residual + RK stage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(args.config),
                      "bytes_per_cell_per_launch": per, "avg_launch_ms": avg_stage_ms},
     }
     if not args.no_cpu_baseline:

@@ -127,6 +127,22 @@ int orc_counters(orc_solver* h, long* out) {
   return 0;
 }
 
+// Record of interface Riemann problems: enable (clears), then read up to
+// `cap` rows of 17 doubles; returns the number of rows recorded.
+int orc_riemann_log_enable(orc_solver* h, int on) {
+  h->s->riem_log_on = on != 0;
+  h->s->riem_log.clear();
+  return 0;
+}
+
+long orc_riemann_log(orc_solver* h, double* out, long cap) {
+  const std::vector<double>& L = h->s->riem_log;
+  long n = (long)L.size() / 17;
+  for (long r = 0; r < n && r < cap; r++)
+    for (int c = 0; c < 17; c++) out[r * 17 + c] = L[r * 17 + c];
+  return n;
+}
+
 // Point kernels -----------------------------------------------------------
 
 int orc_flux_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, int dir, const double* Vm,

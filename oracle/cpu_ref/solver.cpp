@@ -357,6 +357,17 @@ int Solver::Riemann(int d, const double* Vm, int idm, const double* Vp, int idp,
   cnt.riemann_iters += riem.st.it_main;
   cnt.riemann_fevals += riem.st.fevals;
   cnt.riemann_fallbacks += riem.st.fallback;
+  if (riem_log_on) {  // test-side record of every interface problem
+    riem_log.push_back(d);
+    for (int p = 0; p < 5; p++) riem_log.push_back(Vm[p]);
+    riem_log.push_back(idm);
+    for (int p = 0; p < 5; p++) riem_log.push_back(Vp[p]);
+    riem_log.push_back(idp);
+    riem_log.push_back(err);
+    riem_log.push_back(riem.st.it_main);
+    riem_log.push_back(riem.st.fevals);
+    riem_log.push_back(riem.st.fallback);
+  }
   if (riem.st.fatal) Fail("Exact Riemann solver reached a fatal state");
   return err;
 }

@@ -53,6 +53,10 @@ class Solver {
   int nmat, inactive_id;
   Flux flux;
   ExactRiemann riem;
+  // record of interface Riemann problems (17 doubles each: dir, Vm, idm, Vp,
+  // idp, err, iterations, f evaluations, fallback), for tests
+  bool riem_log_on = false;
+  std::vector<double> riem_log;
   // fields (AoS like SpaceVariable3D, ghost width 1)
   std::vector<double> V, V1, V2, Un, U1, R, Vl, Vr, Vb, Vt, Vk, Vf, vol;
   std::vector<double> A3, B3;  // Reconstructor CoeffA/CoeffB (3 per cell)

@@ -43,6 +43,9 @@ def lib():
         L.orc_get_time.argtypes = [P, D, D, I]
         L.orc_set_time.argtypes = [P, C.c_double, C.c_int]
         L.orc_counters.argtypes = [P, C.POINTER(C.c_long)]
+        L.orc_riemann_log_enable.argtypes = [P, C.c_int]
+        L.orc_riemann_log.restype = C.c_long
+        L.orc_riemann_log.argtypes = [P, D, C.c_long]
         L.orc_flux_batch.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, C.c_int,
                                      D, D, I, D]
         L.orc_riemann_batch.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, D, D, I,
@@ -149,6 +152,18 @@ class Oracle:
         keys = ["riemann_calls", "riemann_trivial", "riemann_errors", "riemann_iters", "riemann_fevals",
                 "riemann_fallbacks", "clipped", "id_changed", "steps"]
         return dict(zip(keys, list(out)))
+
+
+    def riemann_log_enable(self, on=True):
+        self.L.orc_riemann_log_enable(self.h, int(on))
+
+    def riemann_log(self):
+        """Rows (dir, Vm[5], idm, Vp[5], idp, err, iterations, f-evals,
+        fallback) of every interface Riemann problem since enabling."""
+        n = self.L.orc_riemann_log(self.h, None, 0)
+        out = np.zeros((n, 17))
+        self.L.orc_riemann_log(self.h, dp(out), n)
+        return out
 
 
 def flux_batch(mats, nmat, scheme, d, Vm, Vp, ids):
