@@ -16,8 +16,9 @@
  *    cell, i fastest: index ((k+1)*(ny+2) + (j+1))*(nx+2) + (i+1), times dof,
  *    plus p (SpaceVariable.h:86-96, SpaceVariable.cpp:55-162).  Material IDs
  *    are int32 in the same brick (the reference stores them as doubles).
- *  - On the device everything is structure-of-arrays FP64 with an x-padded
- *    pitch; the AoS<->SoA transpose happens only in upload/download.
+ *  - On the device every field is structure-of-arrays FP64 (one array per
+ *    component, same brick); the AoS<->SoA transpose happens only in
+ *    upload/download.
  *  - One host thread per handle; all work is ordered on the handle's stream.
  */
 #ifndef M2C_AMD_H
@@ -253,12 +254,67 @@ int m2cx_numerical_flux_batch(m2cx_solver* h, int n, int dir, const double* Vm,
  * the secant main-loop iteration count and f(p) evaluation count.  A problem
  * on which the reference exits (invalid input state, NaN in the rarefaction
  * integration: ExactRiemannSolverBase.cpp:99,110,1326-1335) gets err = -1,
- * id = -1 and NaN states; the other problems are unaffected. */
+ * id = -1 and NaN states; a problem whose rarefaction integration path
+ * outgrows the device path buffer (512 points) gets err = -2 and NaN states;
+ * the other problems are unaffected. */
 int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir,
                        const double* Vm, const int32_t* idm,
                        const double* Vp, const int32_t* idp,
                        double* Vs, int32_t* id, double* Vsm, double* Vsp,
                        int32_t* err, int32_t* iters, int32_t* fevals);
+
+/* ------------------------------------------------------------------ */
+/* Learned p* guess                                                    */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference.  When scheme.riem_guess is
+ * M2CX_GUESS_MLP and parameters are loaded, an MLP (8 -> 64 -> 64 -> 64 -> 1,
+ * SiLU, BF16 MFMA with FP32 accumulation) predicts p* of every interface
+ * problem; the prediction replaces the acoustic-theory starting pressure of
+ * FindInitialFeasiblePointsByAcousticTheory (ExactRiemannSolverBase.cpp:
+ * 1107-1158) and the FP64 iteration after it is unchanged, so results agree
+ * with the plain solver to the Riemann tolerance.  A prediction that is not a
+ * finite positive pressure falls back to the acoustic guess (counted in
+ * riemann_guess_fallbacks).  Features x[8] and target T of a problem:
+ * P = max(pL+pcL, pR+pcR), R = max(rhoL, rhoR), C = sqrt(P/R),
+ * x = [log(rhoL/R), log(rhoR/R), unL/C, unR/C, log((pL+pcL)/P),
+ * log((pR+pcR)/P), gammaL, gammaR], T = log((p* + min(pcL, pcR))/P).
+ * Parameters: FP32 vector of M2CX_MLP_NPARAM values, W1[8][64] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4 (W[in][out]). */
+#define M2CX_MLP_NPARAM 8961
+int m2cx_mlp_set_params(m2cx_solver* h, const float* theta);
+int m2cx_mlp_get_params(m2cx_solver* h, float* theta);
+/* features X (n x 8), and targets T (n) when pstar (n) is given; ok[i] = 0
+ * marks a problem without valid scales (no guess, excluded from training) */
+int m2cx_mlp_features(m2cx_solver* h, int n, const double* dir, const double* Vm, const int32_t* idm,
+                      const double* Vp, const int32_t* idp, const double* pstar, float* X, float* T,
+                      int32_t* ok);
+/* network output y (n) for feature rows X (n x 8) */
+int m2cx_mlp_predict(m2cx_solver* h, int n, const float* X, float* y);
+/* mean-squared-error loss over (X, T) and its gradient (M2CX_MLP_NPARAM
+ * values), computed by the training kernels; n <= 65536 */
+int m2cx_mlp_loss_grad(m2cx_solver* h, int n, const float* X, const float* T, float* grad, float* loss);
+/* Adam (beta 0.9/0.999, eps 1e-8) on the GPU over shuffled mini-batches;
+ * loss_per_epoch (epochs values) receives the mean training loss */
+int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int epochs, int batch, float lr,
+                   uint32_t seed, float* loss_per_epoch);
+
+/* ------------------------------------------------------------------ */
+/* Ensembles                                                           */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (which runs one simulation per MPI job).
+ * Advances many independent solvers (uneven sizes, same device, all RK2 or
+ * all RK3) in lockstep: per RK stage the interface Riemann problems of all
+ * members are solved in ONE launch, everything else runs per member on one
+ * stream, and one ensemble time step (each member with its own CFL dt) is
+ * captured once as a HIP graph and replayed.  Each member's results are
+ * bit-identical to running it alone with m2cx_run_steps.  Members keep
+ * working as solvers (upload/download, counters); the ensemble does not own
+ * them and must be destroyed before them. */
+typedef struct m2cx_ensemble m2cx_ensemble;
+int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out);
+int m2cx_ensemble_run_steps(m2cx_ensemble* e, int nsteps);
+const char* m2cx_ensemble_last_error(const m2cx_ensemble* e);
+int m2cx_ensemble_destroy(m2cx_ensemble* e);
 
 /* ------------------------------------------------------------------ */
 /* Measurement                                                         */

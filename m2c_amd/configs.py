@@ -117,3 +117,17 @@ def c2_shock_bubble(n=256, L=1.0, radius=0.15, mach=1.5, shock_x=0.2, tol=1e-10)
     ID[inside] = 1
     fill_ghosts(V, ID, Phi, g)
     return g, mats, 2, s, V, ID, Phi
+
+
+def c4_instances(count=64, seed=4, edges=(64, 96, 128, 160, 192)):
+    """Uneven ensemble of shock-bubble runs: per instance the grid edge is
+    drawn from `edges`, the bubble radius uniformly from [0.05, 0.25] L and the
+    shock Mach number from [1.2, 3.0] (seeded)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(edges))
+        r = float(rng.uniform(0.05, 0.25))
+        m = float(rng.uniform(1.2, 3.0))
+        out.append(((n, r, m), c2_shock_bubble(n, radius=r, mach=m)))
+    return out

@@ -11,6 +11,9 @@
 #include <vector>
 #include "m2c_kernels.h"
 #include "m2c_multiphase.h"
+#include "m2c_mlp.h"
+#include <algorithm>
+#include <random>
 
 using namespace m2c;
 
@@ -37,9 +40,8 @@ struct m2cx_solver {
   double* R[5] = {};
   unsigned char* ID = nullptr;
   unsigned char* IDn = nullptr;
-  double* axis[6] = {};   // x, y, z, dx, dy, dz
-  double* coefAB[6] = {}; // Ax, Ay, Az, Bx, By, Bz
-  int* coefK[3] = {};
+  double* geo = nullptr;  // x, y, z, dx, dy, dz (ghosted), CoeffA, CoeffB per axis
+  int* coefK = nullptr;   // CoeffK per axis
   double* fd3 = nullptr;  // level-set FD3 coefficient tables
   double* staging = nullptr;
   int* staging_i = nullptr;
@@ -49,6 +51,15 @@ struct m2cx_solver {
   // multi-material
   MPBuffers mp{};
   bool multimat = false;
+  hipStream_t stream_saved = nullptr;  // own stream while an ensemble borrows it
+  // learned p* guess
+  struct {
+    bool ready = false;
+    float *theta = nullptr, *m = nullptr, *v = nullptr;
+    __bf16 *wt1 = nullptr, *wt2 = nullptr, *wt3 = nullptr, *wt4 = nullptr, *w2 = nullptr, *w3 = nullptr;
+    float* guess = nullptr;  // per interface slot
+    long adam_t = 0;
+  } mlp;
   // profiling
   bool profile = false;
   std::map<std::string, Timer> timers;
@@ -66,8 +77,9 @@ static int fail(m2cx_solver* h, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail((h), M2CX_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-static F5 f5(double* const* a) { return F5{{a[0], a[1], a[2], a[3], a[4]}}; }
-static CF5 cf5(double* const* a) { return CF5{{a[0], a[1], a[2], a[3], a[4]}}; }
+// 5-component fields are one allocation each, components n apart; a[p] = a[0] + p*n
+static F5 f5(const m2cx_solver* h, double* const* a) { return F5{a[0], h->ncells}; }
+static CF5 cf5(const m2cx_solver* h, double* const* a) { return CF5{a[0], h->ncells}; }
 
 static void tic(m2cx_solver* h, const char* name, hipEvent_t* a) {
   if (!h->profile) return;
@@ -122,6 +134,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
                 m2cx_solver** out) {
   if (!grid || !mats || !sch || !out || nmat < 1 || nmat > M2C_MAX_MAT) return M2CX_E_ARG;
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return M2CX_E_ARG;
+  if ((long)grid->nx * grid->ny * grid->nz >= (1L << 31)) return M2CX_E_ARG;  // 32-bit cell numbering
   if (sch->var_type != M2CX_VAR_PRIMITIVE || sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
   auto* h = new m2cx_solver;
   *out = h;
@@ -153,33 +166,31 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   axis(NX, grid->x0, grid->xmax, ax[0], ax[3]);
   axis(NY, grid->y0, grid->ymax, ax[1], ax[4]);
   axis(NZ, grid->z0, grid->zmax, ax[2], ax[5]);
-  for (int q = 0; q < 6; q++) {
-    HIPCHK(h, hipMalloc(&h->axis[q], ax[q].size() * 8));
-    HIPCHK(h, hipMemcpy(h->axis[q], ax[q].data(), ax[q].size() * 8, hipMemcpyHostToDevice));
-  }
-  g.x = h->axis[0]; g.y = h->axis[1]; g.z = h->axis[2];
-  g.dx = h->axis[3]; g.dy = h->axis[4]; g.dz = h->axis[5];
-  // Reconstructor CoeffA/B/K per axis (Reconstructor.cpp:53-99)
+  // one geometry table: x, y, z, dx, dy, dz, then Reconstructor CoeffA and
+  // CoeffB per axis (Reconstructor.cpp:53-99); CoeffK in an int table
+  std::vector<double> geo;
+  for (int q = 0; q < 6; q++) geo.insert(geo.end(), ax[q].begin(), ax[q].end());
   int Ns[3] = {NX, NY, NZ};
+  std::vector<double> Acoef, Bcoef;
+  std::vector<int> Kcoef;
   for (int a = 0; a < 3; a++) {
     const std::vector<double>& d = ax[3 + a];
-    std::vector<double> A(Ns[a]), B(Ns[a]);
-    std::vector<int> K(Ns[a]);
     for (int i = 0; i < Ns[a]; i++) {
-      A[i] = (d[i] + d[i + 1]) / (d[i + 1] + d[i + 2]);
-      B[i] = 2.0 * d[i + 1] / (d[i + 1] + d[i + 2]);
-      K[i] = slope_k(sch->limiter, A[i], B[i]);
+      double A = (d[i] + d[i + 1]) / (d[i + 1] + d[i + 2]);
+      double B = 2.0 * d[i + 1] / (d[i + 1] + d[i + 2]);
+      Acoef.push_back(A);
+      Bcoef.push_back(B);
+      Kcoef.push_back(slope_k(sch->limiter, A, B));
     }
-    HIPCHK(h, hipMalloc(&h->coefAB[a], Ns[a] * 8));
-    HIPCHK(h, hipMalloc(&h->coefAB[3 + a], Ns[a] * 8));
-    HIPCHK(h, hipMalloc(&h->coefK[a], Ns[a] * 4));
-    HIPCHK(h, hipMemcpy(h->coefAB[a], A.data(), Ns[a] * 8, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->coefAB[3 + a], B.data(), Ns[a] * 8, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->coefK[a], K.data(), Ns[a] * 4, hipMemcpyHostToDevice));
   }
-  g.Ax = h->coefAB[0]; g.Ay = h->coefAB[1]; g.Az = h->coefAB[2];
-  g.Bx = h->coefAB[3]; g.By = h->coefAB[4]; g.Bz = h->coefAB[5];
-  g.Kx = h->coefK[0]; g.Ky = h->coefK[1]; g.Kz = h->coefK[2];
+  geo.insert(geo.end(), Acoef.begin(), Acoef.end());
+  geo.insert(geo.end(), Bcoef.begin(), Bcoef.end());
+  HIPCHK(h, hipMalloc(&h->geo, geo.size() * 8));
+  HIPCHK(h, hipMemcpy(h->geo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMalloc(&h->coefK, Kcoef.size() * 4));
+  HIPCHK(h, hipMemcpy(h->coefK, Kcoef.data(), Kcoef.size() * 4, hipMemcpyHostToDevice));
+  g.geo = h->geo;
+  g.kt = h->coefK;
   // scheme + materials
   Scheme& s = h->s;
   s.flux = sch->flux; s.rec_type = sch->rec_type; s.limiter = sch->limiter;
@@ -196,16 +207,16 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_mats), &M, sizeof(DevMats)));
   // fields
   const long n = h->ncells;
-  for (int f = 0; f < 3; f++)
-    for (int p = 0; p < 5; p++) HIPCHK(h, hipMalloc(&h->V[f][p], n * 8));
-  for (int p = 0; p < 5; p++) {
-    HIPCHK(h, hipMalloc(&h->Un[p], n * 8));
-    HIPCHK(h, hipMalloc(&h->U1[p], n * 8));
-    HIPCHK(h, hipMalloc(&h->R[p], n * 8));
-    HIPCHK(h, hipMemsetAsync(h->U1[p], 0, n * 8, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->R[p], 0, n * 8, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->Un[p], 0, n * 8, h->stream));
-    for (int f = 0; f < 3; f++) HIPCHK(h, hipMemsetAsync(h->V[f][p], 0, n * 8, h->stream));
+  auto field = [&](double** comp) -> int {
+    double* base = nullptr;
+    if (hipMalloc(&base, 5 * n * 8) != hipSuccess) return M2CX_E_OOM;
+    if (hipMemsetAsync(base, 0, 5 * n * 8, h->stream) != hipSuccess) return M2CX_E_HIP;
+    for (int p = 0; p < 5; p++) comp[p] = base + p * n;
+    return 0;
+  };
+  for (double** comp : {h->V[0], h->V[1], h->V[2], h->Un, h->U1, h->R}) {
+    int rc = field(comp);
+    if (rc) return fail(h, rc, "device field allocation");
   }
   HIPCHK(h, hipMalloc(&h->ID, n));
   HIPCHK(h, hipMalloc(&h->IDn, n));
@@ -234,15 +245,17 @@ int m2cx_destroy(m2cx_solver* h) {
   hipSetDevice(h->device);
   hipStreamSynchronize(h->stream);
   flush_timers(h);
-  for (int f = 0; f < 3; f++)
-    for (int p = 0; p < 5; p++) hipFree(h->V[f][p]);
-  for (int p = 0; p < 5; p++) { hipFree(h->Un[p]); hipFree(h->U1[p]); hipFree(h->R[p]); }
+  // each field is one allocation starting at component 0 (pointer swaps
+  // exchange whole fields)
+  for (int f = 0; f < 3; f++) hipFree(h->V[f][0]);
+  hipFree(h->Un[0]); hipFree(h->U1[0]); hipFree(h->R[0]);
   hipFree(h->ID); hipFree(h->IDn);
-  for (auto* p : h->axis) hipFree(p);
-  for (auto* p : h->coefAB) hipFree(p);
-  for (auto* p : h->coefK) hipFree(p);
+  hipFree(h->geo); hipFree(h->coefK);
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
+  for (void* p : {(void*)h->mlp.theta, (void*)h->mlp.m, (void*)h->mlp.v, (void*)h->mlp.wt1, (void*)h->mlp.wt2,
+                  (void*)h->mlp.wt3, (void*)h->mlp.wt4, (void*)h->mlp.w2, (void*)h->mlp.w3, (void*)h->mlp.guess})
+    hipFree(p);
   hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -268,7 +281,7 @@ int m2cx_upload(m2cx_solver* h, int field, const void* host) {
     double** f = field_ptr(h, field);
     if (!f) return fail(h, M2CX_E_ARG, "bad field");
     HIPCHK(h, hipMemcpyAsync(h->staging, host, n * 5 * 8, hipMemcpyHostToDevice, h->stream));
-    k_aos_to_soa<<<grid_blocks(n), 256, 0, h->stream>>>(h->staging, f5(f), 5, n);
+    k_aos_to_soa<<<grid_blocks(n), 256, 0, h->stream>>>(h->staging, f5(h, f), 5, n);
   }
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -288,7 +301,7 @@ int m2cx_download(m2cx_solver* h, int field, void* host) {
   } else {
     double** f = field_ptr(h, field);
     if (!f) return fail(h, M2CX_E_ARG, "bad field");
-    k_soa_to_aos<<<grid_blocks(n), 256, 0, h->stream>>>(cf5(f), h->staging, 5, n);
+    k_soa_to_aos<<<grid_blocks(n), 256, 0, h->stream>>>(cf5(h, f), h->staging, 5, n);
     HIPCHK(h, hipMemcpyAsync(host, h->staging, n * 5 * 8, hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(h, hipGetLastError());
@@ -304,6 +317,13 @@ static int check_error(m2cx_solver* h) {
   if (c.error & 2) return fail(h, M2CX_E_RIEMANN_FATAL, "exact Riemann solver reached a fatal state");
   if (c.error & 4) return fail(h, M2CX_E_CAPACITY, "device work-list capacity exceeded");
   if (c.error & 8) return fail(h, M2CX_E_UNRESOLVED, "unresolved phase-change cells");
+  return 0;
+}
+
+// the per-solver integration-path slab of the exact solver, on first use
+static int ensure_slab(m2cx_solver* h) {
+  if (!h->multimat) return 0;
+  if (mp_slab(h->mp)) return fail(h, M2CX_E_OOM, "out of device memory (Riemann integration paths)");
   return 0;
 }
 
@@ -330,11 +350,11 @@ static StageArgs stage_args(m2cx_solver* h, int vin, int vout, int mode) {
   StageArgs a{};
   a.g = h->g;
   a.s = h->s;
-  a.Vin = cf5(h->V[vin]);
-  a.Vout = (vout >= 0) ? f5(h->V[vout]) : F5{};
-  a.Un = f5(h->Un);
-  a.U1 = f5(h->U1);
-  a.R = f5(h->R);
+  a.Vin = cf5(h, h->V[vin]);
+  a.Vout = (vout >= 0) ? f5(h, h->V[vout]) : F5{};
+  a.Un = f5(h, h->Un);
+  a.U1 = f5(h, h->U1);
+  a.R = f5(h, h->R);
   a.ID = h->ID;
   if (h->multimat) {
     for (int d = 0; d < 3; d++) a.fidx[d] = h->mp.fidx[d];
@@ -347,12 +367,11 @@ static StageArgs stage_args(m2cx_solver* h, int vin, int vout, int mode) {
 
 static void launch_stage(m2cx_solver* h, int vin, int vout, int mode) {
   StageArgs a = stage_args(h, vin, vout, mode);
-  dim3 grid((h->g.NX + 1 + TX - 1) / TX, (h->g.NY + 1 + TY - 1) / TY, (h->g.NZ + 1 + TZ - 1) / TZ);
-  // the tile grid must cover index NX (the last face); a tile starting past
-  // the domain end has no work and returns after the face phase
-  grid.x = (h->g.NX + TX - 1) / TX;
-  grid.y = (h->g.NY + TY - 1) / TY;
-  grid.z = (h->g.NZ + TZ - 1) / TZ;
+  // one work-group per tile, padded to a multiple of the XCD count (see
+  // k_stage for the tile order)
+  const long ntiles = (long)((h->g.NX + TX - 1) / TX) * ((h->g.NY + TY - 1) / TY) * ((h->g.NZ + TZ - 1) / TZ);
+  const long per = (ntiles + NXCD - 1) / NXCD;
+  dim3 grid((unsigned)(per * NXCD));
   hipEvent_t e0{};
   tic(h, "stage", &e0);
   switch (mode) {
@@ -367,11 +386,32 @@ static void launch_stage(m2cx_solver* h, int vin, int vout, int mode) {
 }
 
 // Interface Riemann problems of this stage (multi-material only).
+static MLPDev mlp_dev(const m2cx_solver* h) {
+  return MLPDev{h->mlp.wt1, h->mlp.wt2, h->mlp.wt3, h->mlp.wt4, h->mlp.w2, h->mlp.w3, h->mlp.theta};
+}
+
+// learned guesses for the faces of this stage (when enabled)
+static void guess_stage(m2cx_solver* h, int vin) {
+  if (!h->multimat) return;
+  const bool learned = h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready && h->mlp.guess;
+  h->mp.guess = learned ? 1 : 0;
+  h->mp.pguess = h->mlp.guess;
+  if (learned) {
+    hipEvent_t eg{};
+    tic(h, "mlp", &eg);
+    const int blocks = (int)std::min<long>((h->mp.L.cap + 15) / 16, 4096L);
+    k_mlp_guess<<<blocks, 64, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[vin]), h->ID, h->mp.L, mlp_dev(h),
+                                              h->mlp.guess, h->cnt);
+    toc(h, "mlp", eg);
+  }
+}
+
 static void interface_stage(m2cx_solver* h, int vin, int stage1) {
   if (!h->multimat) return;
+  guess_stage(h, vin);
   hipEvent_t e0{};
   tic(h, "riemann", &e0);
-  mp_riemann(h->mp, h->g, h->s, cf5(h->V[vin]), h->ID, stage1, h->cnt, h->stream);
+  mp_riemann(h->mp, h->g, h->s, cf5(h, h->V[vin]), h->ID, stage1, h->cnt, h->stream);
   toc(h, "riemann", e0);
 }
 
@@ -379,13 +419,14 @@ static void levelset_stage(m2cx_solver* h, int vin, int which) {
   if (!h->multimat || h->sch.n_levelsets == 0) return;
   hipEvent_t e0{};
   tic(h, "levelset", &e0);
-  mp_levelset(h->mp, h->g, cf5(h->V[vin]), which, h->cnt, h->stream);
+  mp_levelset(h->mp, h->g, cf5(h, h->V[vin]), which, h->cnt, h->stream);
   toc(h, "levelset", e0);
 }
 
 int m2cx_compute_residual(m2cx_solver* h, int v_field, int capture_riemann) {
   if (!h || v_field < 0 || v_field > 2) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = ensure_slab(h)) return rc;
   if (h->multimat) mp_classify(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
   interface_stage(h, v_field, capture_riemann);
   launch_stage(h, v_field, -1, MODE_RESIDUAL);
@@ -400,7 +441,7 @@ int m2cx_compute_residual(m2cx_solver* h, int v_field, int capture_riemann) {
 int m2cx_compute_time_step(m2cx_solver* h, double* dt, double* cfl) {
   if (!h) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
-  k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h->V[0]), h->ID, h->dt_part, h->cnt, 0);
+  k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 0);
   HIPCHK(h, hipGetLastError());
   int rc = check_error(h);
   if (rc) return rc;
@@ -414,7 +455,7 @@ int m2cx_compute_time_step(m2cx_solver* h, double* dt, double* cfl) {
 int m2cx_apply_boundary_conditions(m2cx_solver* h, int v_field) {
   double** f = field_ptr(h, v_field);
   if (!f) return M2CX_E_ARG;
-  k_apply_bc<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, f5(f));
+  k_apply_bc<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, f5(h, f));
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -424,7 +465,7 @@ int m2cx_clip(m2cx_solver* h, int v_field) {
   double** f = field_ptr(h, v_field);
   if (!f) return M2CX_E_ARG;
   k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
-  k_clip<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, f5(f), h->ID, h->cnt);
+  k_clip<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, f5(h, f), h->ID, h->cnt);
   int rc = check_error(h);
   if (rc) return rc;
   DevCounters c;
@@ -432,10 +473,47 @@ int m2cx_clip(m2cx_solver* h, int v_field) {
   return (int)c.stage_clipped;
 }
 
+// The RK stages of TimeIntegratorRK2/RK3::AdvanceOneTimeStep
+// (TimeIntegrator.cpp:319-455, 527-715): stage input field, whether the
+// Riemann star states are captured, stage kernel mode and output field, and
+// the level-set combination of the stage.
+struct StageSpec {
+  int vin, stage1, mode, vout, resync, ls_which;
+};
+static int stage_specs(int ti, StageSpec* out) {
+  if (ti == M2CX_TI_RK2) {
+    out[0] = StageSpec{0, 1, MODE_STAGE1, 1, 1, 1};
+    out[1] = StageSpec{1, 0, MODE_RK2_FINAL, 0, 0, 2};
+    return 2;
+  }
+  out[0] = StageSpec{0, 1, MODE_STAGE1, 1, 1, 1};
+  out[1] = StageSpec{1, 0, MODE_RK3_STAGE2, 2, 1, 3};
+  out[2] = StageSpec{2, 0, MODE_RK3_FINAL, 0, 0, 4};
+  return 3;
+}
+
+// the part of a stage after its interface Riemann problems
+static void stage_after_riemann(m2cx_solver* h, const StageSpec& sp) {
+  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  if (sp.resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
+  launch_stage(h, sp.vin, sp.vout, sp.mode);
+  if (sp.resync)
+    k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, cf5(h, h->V[sp.vout]), f5(h, h->U1), h->ID, h->cnt);
+  levelset_stage(h, sp.vin, sp.ls_which);
+}
+
+static void end_of_step(m2cx_solver* h) {
+  if (h->multimat && h->sch.n_levelsets > 0) {
+    hipEvent_t e0{};
+    tic(h, "endstep", &e0);
+    mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
+    toc(h, "endstep", e0);
+  }
+}
+
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
 static void step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
-  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
   if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
   if (ti == M2CX_TI_FORWARD_EULER) {
     interface_stage(h, 0, 1);
@@ -445,37 +523,20 @@ static void step_kernels(m2cx_solver* h) {
     for (int p = 0; p < 5; p++) std::swap(h->V[0][p], h->V[1][p]);
     levelset_stage(h, 0, 0);
   } else {
-    interface_stage(h, 0, 1);
-    k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
-    launch_stage(h, 0, 1, MODE_STAGE1);
-    k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, cf5(h->V[1]), f5(h->U1), h->ID, h->cnt);
-    levelset_stage(h, 0, 1);
-    if (ti == M2CX_TI_RK2) {
-      interface_stage(h, 1, 0);
-      launch_stage(h, 1, 0, MODE_RK2_FINAL);
-      levelset_stage(h, 1, 2);
-    } else {
-      interface_stage(h, 1, 0);
-      k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
-      launch_stage(h, 1, 2, MODE_RK3_STAGE2);
-      k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, cf5(h->V[2]), f5(h->U1), h->ID, h->cnt);
-      levelset_stage(h, 1, 3);
-      interface_stage(h, 2, 0);
-      launch_stage(h, 2, 0, MODE_RK3_FINAL);
-      levelset_stage(h, 2, 4);
+    StageSpec sp[3];
+    const int ns = stage_specs(ti, sp);
+    for (int q = 0; q < ns; q++) {
+      interface_stage(h, sp[q].vin, sp[q].stage1);
+      stage_after_riemann(h, sp[q]);
     }
   }
-  if (h->multimat && h->sch.n_levelsets > 0) {
-    hipEvent_t e0{};
-    tic(h, "endstep", &e0);
-    mp_end_step(h->mp, h->g, h->s, f5(h->V[0]), h->ID, h->cnt, h->stream);
-    toc(h, "endstep", e0);
-  }
+  end_of_step(h);
 }
 
 int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_step) {
   if (!h) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = ensure_slab(h)) return rc;
   DevCounters c;
   HIPCHK(h, hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
   c.dt = dt;
@@ -490,6 +551,7 @@ int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_
 int m2cx_run_steps(m2cx_solver* h, int nsteps) {
   if (!h) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = ensure_slab(h)) return rc;
   const bool clamp = h->sch.max_time > 0;
   for (int q = 0; q < nsteps; q++) {
     if (clamp) {
@@ -500,7 +562,7 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
     }
     hipEvent_t e0{};
     tic(h, "dt", &e0);
-    k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h->V[0]), h->ID, h->dt_part, h->cnt, 1);
+    k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 1);
     toc(h, "dt", e0);
     step_kernels(h);
     HIPCHK(h, hipGetLastError());
@@ -557,13 +619,257 @@ int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir, const double* V
                        const double* Vp, const int32_t* idp, double* Vs, int32_t* id, double* Vsm, double* Vsp,
                        int32_t* err, int32_t* iters, int32_t* fevals) {
   if (!h || n <= 0) return M2CX_E_ARG;
-  return mp_riemann_batch(h->mp, h->s, n, dir, Vm, idm, Vp, idp, Vs, id, Vsm, Vsp, err, iters, fevals, h->cnt,
-                          h->stream, &h->err);
+  if (!h->multimat) return fail(h, M2CX_E_ARG, "Riemann batch needs a multi-material solver");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = ensure_slab(h)) return rc;
+  const long N = n;
+  double *dd, *dVm, *dVp, *dVs, *dVsm, *dVsp;
+  int *didm, *didp, *did, *derr, *dit, *dfe;
+  HIPCHK(h, hipMalloc(&dd, N * 24));
+  HIPCHK(h, hipMalloc(&dVm, N * 40));
+  HIPCHK(h, hipMalloc(&dVp, N * 40));
+  HIPCHK(h, hipMalloc(&dVs, N * 40));
+  HIPCHK(h, hipMalloc(&dVsm, N * 40));
+  HIPCHK(h, hipMalloc(&dVsp, N * 40));
+  HIPCHK(h, hipMalloc(&didm, N * 4));
+  HIPCHK(h, hipMalloc(&didp, N * 4));
+  HIPCHK(h, hipMalloc(&did, N * 4));
+  HIPCHK(h, hipMalloc(&derr, N * 4));
+  HIPCHK(h, hipMalloc(&dit, N * 4));
+  HIPCHK(h, hipMalloc(&dfe, N * 4));
+  HIPCHK(h, hipMemcpy(dd, dir, N * 24, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dVm, Vm, N * 40, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dVp, Vp, N * 40, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(didm, idm, N * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(didp, idp, N * 4, hipMemcpyHostToDevice));
+  float* dguess = nullptr;
+  if (h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready) {
+    // learned starting pressures for these problems
+    float *X, *y;
+    double *P, *pc;
+    int* ok;
+    HIPCHK(h, hipMalloc(&dguess, N * 4));
+    HIPCHK(h, hipMalloc(&X, N * 32));
+    HIPCHK(h, hipMalloc(&y, N * 4));
+    HIPCHK(h, hipMalloc(&P, N * 8));
+    HIPCHK(h, hipMalloc(&pc, N * 8));
+    HIPCHK(h, hipMalloc(&ok, N * 4));
+    k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(n, dd, dVm, didm, dVp, didp, nullptr, X, P, pc, nullptr,
+                                                           ok);
+    k_mlp_predict<<<(n + 15) / 16, 64, 0, h->stream>>>(n, X, mlp_dev(h), y);
+    k_mlp_compose<<<(n + 255) / 256, 256, 0, h->stream>>>(n, y, P, pc, ok, dguess);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    hipFree(X); hipFree(y); hipFree(P); hipFree(pc); hipFree(ok);
+  }
+  MPBuffers& B = h->mp;
+  k_riemann_batch<<<(B.lanes + 127) / 128, 128, 0, h->stream>>>(B.rc, n, dd, dVm, didm, dVp, didp, dVs, did, dVsm,
+                                                                 dVsp, derr, dit, dfe, B.slab, B.lanes, B.pathcap,
+                                                                 dguess, h->cnt);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(Vs, dVs, N * 40, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(Vsm, dVsm, N * 40, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(Vsp, dVsp, N * 40, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(id, did, N * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(err, derr, N * 4, hipMemcpyDeviceToHost));
+  if (iters) HIPCHK(h, hipMemcpy(iters, dit, N * 4, hipMemcpyDeviceToHost));
+  if (fevals) HIPCHK(h, hipMemcpy(fevals, dfe, N * 4, hipMemcpyDeviceToHost));
+  for (void* p : {(void*)dd, (void*)dVm, (void*)dVp, (void*)dVs, (void*)dVsm, (void*)dVsp, (void*)didm, (void*)didp,
+                  (void*)did, (void*)derr, (void*)dit, (void*)dfe, (void*)dguess})
+    hipFree(p);
+  return 0;
+}
+
+// ---------------------------------------------------------------- learned guess
+static int mlp_alloc(m2cx_solver* h) {
+  if (h->mlp.theta) return 0;
+  auto& m = h->mlp;
+  HIPCHK(h, hipMalloc(&m.theta, MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&m.m, MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&m.v, MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&m.wt1, 64 * 32 * 2));
+  HIPCHK(h, hipMalloc(&m.wt2, 64 * 64 * 2));
+  HIPCHK(h, hipMalloc(&m.wt3, 64 * 64 * 2));
+  HIPCHK(h, hipMalloc(&m.wt4, 16 * 64 * 2));
+  HIPCHK(h, hipMalloc(&m.w2, 64 * 64 * 2));
+  HIPCHK(h, hipMalloc(&m.w3, 64 * 64 * 2));
+  if (h->multimat) HIPCHK(h, hipMalloc(&m.guess, (long)h->mp.L.cap * 4));
+  return 0;
+}
+
+int m2cx_mlp_set_params(m2cx_solver* h, const float* theta) {
+  if (!h || !theta) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  int rc = mlp_alloc(h);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpy(h->mlp.theta, theta, MLP_NPAR * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(h->mlp.m, 0, MLP_NPAR * 4));
+  HIPCHK(h, hipMemset(h->mlp.v, 0, MLP_NPAR * 4));
+  h->mlp.adam_t = 0;
+  k_mlp_refresh<<<16, 256, 0, h->stream>>>(h->mlp.theta, mlp_dev(h));
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->mlp.ready = true;
+  return 0;
+}
+
+int m2cx_mlp_get_params(m2cx_solver* h, float* theta) {
+  if (!h || !theta || !h->mlp.ready) return M2CX_E_ARG;
+  HIPCHK(h, hipMemcpy(theta, h->mlp.theta, MLP_NPAR * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int m2cx_mlp_features(m2cx_solver* h, int n, const double* dir, const double* Vm, const int32_t* idm,
+                      const double* Vp, const int32_t* idp, const double* pstar, float* X, float* T,
+                      int32_t* ok) {
+  if (!h || n <= 0 || !X || !ok || (pstar && !T)) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  const long N = n;
+  double *dd, *dVm, *dVp, *dps = nullptr, *P, *pc;
+  int *didm, *didp, *dok;
+  float *dX, *dT = nullptr;
+  HIPCHK(h, hipMalloc(&dd, N * 24));
+  HIPCHK(h, hipMalloc(&dVm, N * 40));
+  HIPCHK(h, hipMalloc(&dVp, N * 40));
+  HIPCHK(h, hipMalloc(&didm, N * 4));
+  HIPCHK(h, hipMalloc(&didp, N * 4));
+  HIPCHK(h, hipMalloc(&dok, N * 4));
+  HIPCHK(h, hipMalloc(&dX, N * 32));
+  HIPCHK(h, hipMalloc(&P, N * 8));
+  HIPCHK(h, hipMalloc(&pc, N * 8));
+  if (pstar) {
+    HIPCHK(h, hipMalloc(&dps, N * 8));
+    HIPCHK(h, hipMalloc(&dT, N * 4));
+    HIPCHK(h, hipMemcpy(dps, pstar, N * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipMemcpy(dd, dir, N * 24, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dVm, Vm, N * 40, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dVp, Vp, N * 40, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(didm, idm, N * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(didp, idp, N * 4, hipMemcpyHostToDevice));
+  k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(n, dd, dVm, didm, dVp, didp, dps, dX, P, pc, dT, dok);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(X, dX, N * 32, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(ok, dok, N * 4, hipMemcpyDeviceToHost));
+  if (pstar) HIPCHK(h, hipMemcpy(T, dT, N * 4, hipMemcpyDeviceToHost));
+  for (void* p : {(void*)dd, (void*)dVm, (void*)dVp, (void*)dps, (void*)P, (void*)pc, (void*)didm, (void*)didp,
+                  (void*)dok, (void*)dX, (void*)dT})
+    hipFree(p);
+  return 0;
+}
+
+int m2cx_mlp_predict(m2cx_solver* h, int n, const float* X, float* y) {
+  if (!h || n <= 0 || !X || !y || !h->mlp.ready) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  float *dX, *dy;
+  HIPCHK(h, hipMalloc(&dX, (long)n * 32));
+  HIPCHK(h, hipMalloc(&dy, (long)n * 4));
+  HIPCHK(h, hipMemcpy(dX, X, (long)n * 32, hipMemcpyHostToDevice));
+  k_mlp_predict<<<(n + 15) / 16, 64, 0, h->stream>>>(n, dX, mlp_dev(h), dy);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(y, dy, (long)n * 4, hipMemcpyDeviceToHost));
+  hipFree(dX);
+  hipFree(dy);
+  return 0;
+}
+
+// One pass over perm[start .. start + count): partial gradients, then their
+// sum into `gout` (when given) and an Adam step (when lr > 0).
+static int mlp_batch(m2cx_solver* h, const float* dX, const float* dT, const int* dperm, int n, int start, int count,
+                     float lr, float* gpart, float* lpart, double* loss_acc, float* gout) {
+  const int waves = (count + 31) / 32;
+  MLPTrainArgs a{mlp_dev(h), dX, dT, dperm, n, start, count, 1.0f / count, gpart, lpart};
+  k_mlp_train_step<<<waves, 64, 0, h->stream>>>(a);
+  float c1 = 1.0f, c2 = 1.0f;
+  const float b1 = 0.9f, b2 = 0.999f;
+  if (lr > 0) {
+    h->mlp.adam_t++;
+    c1 = 1.0f - std::pow(b1, (float)h->mlp.adam_t);
+    c2 = 1.0f - std::pow(b2, (float)h->mlp.adam_t);
+  }
+  k_mlp_adam<<<(MLP_NPAR + 255) / 256, 256, 0, h->stream>>>(h->mlp.theta, h->mlp.m, h->mlp.v, gpart, waves, lr, b1,
+                                                             b2, 1e-8f, c1, c2, gout, lpart, loss_acc);
+  if (lr > 0) k_mlp_refresh<<<16, 256, 0, h->stream>>>(h->mlp.theta, mlp_dev(h));
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int m2cx_mlp_loss_grad(m2cx_solver* h, int n, const float* X, const float* T, float* grad, float* loss) {
+  if (!h || n <= 0 || n > 65536 || !X || !T || !h->mlp.ready) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  const long N = n;
+  const int waves = (n + 31) / 32;
+  float *dX, *dT, *gpart, *lpart, *dg;
+  int* dperm;
+  double* lacc;
+  HIPCHK(h, hipMalloc(&dX, N * 32));
+  HIPCHK(h, hipMalloc(&dT, N * 4));
+  HIPCHK(h, hipMalloc(&dperm, N * 4));
+  HIPCHK(h, hipMalloc(&gpart, (long)waves * MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&lpart, waves * 4));
+  HIPCHK(h, hipMalloc(&dg, MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&lacc, 8));
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  HIPCHK(h, hipMemcpy(dX, X, N * 32, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dT, T, N * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dperm, perm.data(), N * 4, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(lacc, 0, 8));
+  int rc = mlp_batch(h, dX, dT, dperm, n, 0, n, 0.0f, gpart, lpart, lacc, dg);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double l = 0;
+  HIPCHK(h, hipMemcpy(&l, lacc, 8, hipMemcpyDeviceToHost));
+  if (grad) HIPCHK(h, hipMemcpy(grad, dg, MLP_NPAR * 4, hipMemcpyDeviceToHost));
+  if (loss) *loss = (float)(l / n);
+  for (void* p : {(void*)dX, (void*)dT, (void*)dperm, (void*)gpart, (void*)lpart, (void*)dg, (void*)lacc}) hipFree(p);
+  return 0;
+}
+
+int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int epochs, int batch, float lr,
+                   uint32_t seed, float* loss_per_epoch) {
+  if (!h || n <= 0 || !X || !T || epochs <= 0 || batch <= 0 || batch > 65536 || !(lr > 0) || !h->mlp.ready)
+    return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  const long N = n;
+  const int waves = (batch + 31) / 32;
+  float *dX, *dT, *gpart, *lpart;
+  int* dperm;
+  double* lacc;
+  HIPCHK(h, hipMalloc(&dX, N * 32));
+  HIPCHK(h, hipMalloc(&dT, N * 4));
+  HIPCHK(h, hipMalloc(&dperm, N * 4));
+  HIPCHK(h, hipMalloc(&gpart, (long)waves * MLP_NPAR * 4));
+  HIPCHK(h, hipMalloc(&lpart, waves * 4));
+  HIPCHK(h, hipMalloc(&lacc, 8));
+  HIPCHK(h, hipMemcpy(dX, X, N * 32, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(dT, T, N * 4, hipMemcpyHostToDevice));
+  std::mt19937 rng(seed);
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  for (int ep = 0; ep < epochs; ep++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIPCHK(h, hipMemcpyAsync(dperm, perm.data(), N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(lacc, 0, 8, h->stream));
+    for (int start = 0; start < n; start += batch) {
+      const int count = std::min(batch, n - start);
+      int rc = mlp_batch(h, dX, dT, dperm, n, start, count, lr, gpart, lpart, lacc, nullptr);
+      if (rc) return rc;
+    }
+    double l = 0;
+    HIPCHK(h, hipMemcpyAsync(&l, lacc, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (loss_per_epoch) loss_per_epoch[ep] = (float)(l / n);
+  }
+  for (void* p : {(void*)dX, (void*)dT, (void*)dperm, (void*)gpart, (void*)lpart, (void*)lacc}) hipFree(p);
+  return 0;
 }
 
 int m2cx_levelset_residual(m2cx_solver* h, int v_field) {
   if (!h || !h->mp.phi) return M2CX_E_ARG;
-  mp_levelset(h->mp, h->g, cf5(h->V[v_field]), -1, h->cnt, h->stream);
+  mp_levelset(h->mp, h->g, cf5(h, h->V[v_field]), -1, h->cnt, h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -572,6 +878,138 @@ int m2cx_levelset_residual(m2cx_solver* h, int v_field) {
 int m2cx_download_ls_residual(m2cx_solver* h, double* host) {
   if (!h || !h->mp.rls) return M2CX_E_ARG;
   HIPCHK(h, hipMemcpy(host, h->mp.rls, h->ncells * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---------------------------------------------------------------- ensembles
+// Many independent solvers advanced in lockstep on one stream: per member,
+// the dt reduction, the stage kernels, the level set and the end-of-step
+// update; per RK stage, ONE launch solves the interface Riemann problems of
+// all members (k_riemann_multi).  One time step of the whole ensemble is
+// captured once as a HIP graph and replayed.
+struct m2cx_ensemble {
+  std::vector<m2cx_solver*> m;
+  int device = 0;
+  int ti = 0, nstages = 0;
+  hipStream_t stream = nullptr;
+  RiemMember* dmem[3] = {};  // member tables, one per RK stage
+  int nriem = 0;              // members with interface faces
+  int* doff = nullptr;
+  double* slab = nullptr;
+  int lanes = 256 * 256, pathcap = 512;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+static int efail(m2cx_ensemble* e, int code, const std::string& msg) {
+  e->err = msg;
+  return code;
+}
+
+#define EHIPCHK(e, x)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return efail((e), M2CX_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// one ensemble time step, enqueued on e->stream (no host synchronisation)
+static void ensemble_step(m2cx_ensemble* e) {
+  for (m2cx_solver* h : e->m) {
+    k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 1);
+    if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
+  }
+  StageSpec sp[3];
+  stage_specs(e->ti, sp);
+  for (int q = 0; q < e->nstages; q++) {
+    for (m2cx_solver* h : e->m) guess_stage(h, sp[q].vin);
+    if (e->nriem) {
+      k_riem_offsets<<<1, 1, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff);
+      k_riemann_multi<<<(e->lanes + 127) / 128, 128, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1,
+                                                                     e->slab, e->lanes, e->pathcap);
+      k_riem_count<<<e->nriem, 256, 0, e->stream>>>(e->dmem[q]);
+    }
+    for (m2cx_solver* h : e->m) stage_after_riemann(h, sp[q]);
+  }
+  for (m2cx_solver* h : e->m) end_of_step(h);
+}
+
+int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out) {
+  if (!members || n < 1 || !out) return M2CX_E_ARG;
+  auto* e = new m2cx_ensemble;
+  *out = e;
+  e->m.assign(members, members + n);
+  e->device = members[0]->device;
+  e->ti = members[0]->sch.integrator;
+  for (m2cx_solver* h : e->m) {
+    if (h->device != e->device) return efail(e, M2CX_E_ARG, "ensemble members must share one device");
+    if (h->sch.integrator != e->ti) return efail(e, M2CX_E_ARG, "ensemble members must share the integrator");
+  }
+  // forward Euler exchanges the state buffers every step, which a replayed
+  // graph cannot follow
+  if (e->ti == M2CX_TI_FORWARD_EULER) return efail(e, M2CX_E_UNSUPPORTED, "ensembles run RK2 or RK3");
+  StageSpec sp[3];
+  e->nstages = stage_specs(e->ti, sp);
+  EHIPCHK(e, hipSetDevice(e->device));
+  EHIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  std::vector<RiemMember> tab[3];
+  for (m2cx_solver* h : e->m) {
+    h->profile = false;
+    h->stream_saved = h->stream;
+    if (!h->multimat) continue;
+    const bool learned = h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready && h->mlp.guess;
+    for (int q = 0; q < e->nstages; q++)
+      tab[q].push_back(RiemMember{h->g, h->s, h->mp.rc, cf5(h, h->V[sp[q].vin]), h->ID, h->mp.L,
+                                  learned ? h->mlp.guess : nullptr, h->cnt});
+  }
+  e->nriem = (int)tab[0].size();
+  if (e->nriem) {
+    for (int q = 0; q < e->nstages; q++) {
+      EHIPCHK(e, hipMalloc(&e->dmem[q], tab[q].size() * sizeof(RiemMember)));
+      EHIPCHK(e, hipMemcpy(e->dmem[q], tab[q].data(), tab[q].size() * sizeof(RiemMember), hipMemcpyHostToDevice));
+    }
+    EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 1) * sizeof(int)));
+    EHIPCHK(e, hipMalloc(&e->slab, (long)e->lanes * 2 * e->pathcap * 3 * 8));
+  }
+  // every member works on the ensemble stream while it is captured and run
+  for (m2cx_solver* h : e->m) h->stream = e->stream;
+  EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  ensemble_step(e);
+  hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+  for (m2cx_solver* h : e->m) h->stream = h->stream_saved;
+  if (ce != hipSuccess) return efail(e, M2CX_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+  EHIPCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int m2cx_ensemble_run_steps(m2cx_ensemble* e, int nsteps) {
+  if (!e || !e->exec || nsteps < 0) return M2CX_E_ARG;
+  EHIPCHK(e, hipSetDevice(e->device));
+  // members' own streams may still hold work (uploads)
+  for (m2cx_solver* h : e->m) EHIPCHK(e, hipStreamSynchronize(h->stream));
+  for (int q = 0; q < nsteps; q++) EHIPCHK(e, hipGraphLaunch(e->exec, e->stream));
+  EHIPCHK(e, hipStreamSynchronize(e->stream));
+  int riem_fail = 0;
+  for (size_t i = 0; i < e->m.size(); i++) {
+    int rc = check_error(e->m[i]);
+    if (rc) return efail(e, rc, "member " + std::to_string(i) + ": " + e->m[i]->err);
+  }
+  return riem_fail;
+}
+
+const char* m2cx_ensemble_last_error(const m2cx_ensemble* e) { return e ? e->err.c_str() : "null handle"; }
+
+int m2cx_ensemble_destroy(m2cx_ensemble* e) {
+  if (!e) return 0;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  for (auto* p : e->dmem) hipFree(p);
+  hipFree(e->doff);
+  hipFree(e->slab);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
   return 0;
 }
 

@@ -21,15 +21,39 @@ constexpr int TX = 16, TY = 4, TZ = 4, NT = TX * TY * TZ;  // stage tile: 256 th
 struct DevGrid {
   int NX, NY, NZ, SX, SY, SZ;
   long S;        // plane stride SX*SY
-  long ncells;   // SX*SY*SZ
-  // per-axis coordinates and spacings with ghosts (index i+1)
-  const double *x, *y, *z, *dx, *dy, *dz;
-  // Reconstructor CoeffA / CoeffB / CoeffK per axis for interior cells (index i)
-  const double *Ax, *Ay, *Az, *Bx, *By, *Bz;
-  const int *Kx, *Ky, *Kz;
+  long ncells;   // SX*SY*SZ: also the component stride of every 5-field
+  // One geometry table (see g_axis / g_coefA / g_coefB): coordinates x, y, z
+  // and spacings dx, dy, dz with ghosts (index i+1), then the Reconstructor
+  // CoeffA and CoeffB of interior cells per axis (index i).
+  const double* geo;
+  const int* kt;  // CoeffK per axis (index i)
   int bc[6];
   double inlet[5], inlet2[5];
 };
+
+// coordinates (q = 0..2) or spacings (q = 3..5) along axis q % 3, ghosted
+__device__ __host__ __forceinline__ const double* g_axis(const DevGrid& g, int q) {
+  const int a = q % 3;
+  long off = (q >= 3) ? (long)g.SX + g.SY + g.SZ : 0;
+  if (a >= 1) off += g.SX;
+  if (a >= 2) off += g.SY;
+  return g.geo + off;
+}
+__device__ __forceinline__ const double* g_coefA(const DevGrid& g, int D) {
+  long off = 2L * (g.SX + g.SY + g.SZ);
+  if (D >= 1) off += g.NX;
+  if (D >= 2) off += g.NY;
+  return g.geo + off;
+}
+__device__ __forceinline__ const double* g_coefB(const DevGrid& g, int D) {
+  return g_coefA(g, D) + (g.NX + g.NY + g.NZ);
+}
+__device__ __forceinline__ const int* g_coefK(const DevGrid& g, int D) {
+  long off = 0;
+  if (D >= 1) off += g.NX;
+  if (D >= 2) off += g.NY;
+  return g.kt + off;
+}
 
 struct Scheme {
   int flux, rec_type, limiter, slope_zero, integrator;
@@ -40,21 +64,36 @@ struct Scheme {
 __device__ __forceinline__ long gidx(const DevGrid& g, int i, int j, int k) {
   return ((long)(k + 1) * g.SY + (j + 1)) * g.SX + (i + 1);
 }
+// (i, j, k) of interior cell number q (i fastest), by 32-bit division: the
+// interior cell count is checked below 2^31 when a solver is created
+__device__ __forceinline__ void cell_ijk(const DevGrid& g, long q, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)q, nx = (unsigned)g.NX, ny = (unsigned)g.NY;
+  const unsigned r = u / nx;
+  i = (int)(u - r * nx);
+  k = (int)(r / ny);
+  j = (int)(r - (unsigned)k * ny);
+}
+__device__ __forceinline__ long axis_stride(const DevGrid& g, int D) { return D == 0 ? 1 : (D == 1 ? g.SX : g.S); }
 
+// A 5-component field: components stored one after another, `cs` apart.
 struct F5 {
-  double* c[5];
+  double* b;
+  long cs;
+  __device__ __forceinline__ double& operator()(int p, long c) const { return b[p * cs + c]; }
 };
 struct CF5 {
-  const double* c[5];
+  const double* b;
+  long cs;
+  __device__ __forceinline__ double operator()(int p, long c) const { return b[p * cs + c]; }
 };
 
 __device__ __forceinline__ void load5(const CF5& f, long c, double* v) {
 #pragma unroll
-  for (int p = 0; p < 5; p++) v[p] = f.c[p][c];
+  for (int p = 0; p < 5; p++) v[p] = f(p, c);
 }
 __device__ __forceinline__ void store5(const F5& f, long c, const double* v) {
 #pragma unroll
-  for (int p = 0; p < 5; p++) f.c[p][c] = v[p];
+  for (int p = 0; p < 5; p++) f(p, c) = v[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -69,12 +108,11 @@ __device__ __forceinline__ void store5(const F5& f, long c, const double* v) {
 // clipping would modify it.  `fail`/`fixable` report the two conditions so
 // the owning cell can detect the abort case.
 // ---------------------------------------------------------------------------
-template <int D>
 __device__ __forceinline__ void face_state(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
-                                           const unsigned char* __restrict__ ID, int i, int j, int k, int side,
-                                           double* out, bool& fail, bool& unfixable) {
+                                           const unsigned char* __restrict__ ID, int D, int i, int j, int k,
+                                           int side, double* out, bool& fail, bool& unfixable) {
   const long c = gidx(g, i, j, k);
-  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+  const long st = axis_stride(g, D);
   double v[5];
   load5(V, c, v);
   fail = false;
@@ -96,9 +134,9 @@ __device__ __forceinline__ void face_state(const DevGrid& g, const Scheme& s, co
   load5(V, c - st, vm);
   load5(V, c + st, vp);
   const int a = (D == 0) ? i : (D == 1 ? j : k);
-  const double A = (D == 0) ? g.Ax[a] : (D == 1 ? g.Ay[a] : g.Az[a]);
-  const double B = (D == 0) ? g.Bx[a] : (D == 1 ? g.By[a] : g.Bz[a]);
-  const int K = (D == 0) ? g.Kx[a] : (D == 1 ? g.Ky[a] : g.Kz[a]);
+  const double A = g_coefA(g, D)[a];
+  const double B = g_coefB(g, D)[a];
+  const int K = g_coefK(g, D)[a];
   const bool zero = s.slope_zero && (myid != idm || myid != idp);
 #pragma unroll
   for (int p = 0; p < 5; p++) {
@@ -144,10 +182,9 @@ __device__ __forceinline__ void ghost_face(int bc, int D, const double* vghost, 
 // Both states of the face between cell (i,j,k) - e_D and (i,j,k), i.e. the
 // minus face of (i,j,k).  `bits_m`/`bits_p` return fail/unfixable bits for the
 // two owning cells (bit 0 fail, bit 1 unfixable).
-template <int D>
 __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
-                                          const unsigned char* __restrict__ ID, int i, int j, int k, double* Vm,
-                                          double* Vp, int& bits_m, int& bits_p) {
+                                          const unsigned char* __restrict__ ID, int D, int i, int j, int k,
+                                          double* Vm, double* Vp, int& bits_m, int& bits_p) {
   const int im = (D == 0) ? i - 1 : i, jm = (D == 1) ? j - 1 : j, km = (D == 2) ? k - 1 : k;
   const int N = (D == 0) ? g.NX : (D == 1 ? g.NY : g.NZ);
   const int a = (D == 0) ? i : (D == 1 ? j : k);
@@ -156,7 +193,7 @@ __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, con
   if (a == 0) {
     // ghost on the minus side
     double fp[5], vg[5];
-    face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+    face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
     bits_p = (f ? 1 : 0) | (u ? 2 : 0);
     load5(V, gidx(g, im, jm, km), vg);
     for (int p = 0; p < 5; p++) fp[p] = Vp[p];
@@ -164,15 +201,15 @@ __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, con
   } else if (a == N) {
     // ghost on the plus side
     double fm[5], vg[5];
-    face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
+    face_state(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
     bits_m = (f ? 1 : 0) | (u ? 2 : 0);
     load5(V, gidx(g, i, j, k), vg);
     for (int p = 0; p < 5; p++) fm[p] = Vm[p];
     ghost_face(g.bc[2 * D + 1], D, vg, fm, Vp);
   } else {
-    face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
+    face_state(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
     bits_m = (f ? 1 : 0) | (u ? 2 : 0);
-    face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+    face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
     bits_p = (f ? 1 : 0) | (u ? 2 : 0);
   }
 }

@@ -33,7 +33,7 @@ struct IfaceList {
   double* vsm;    // [5][cap]: captured star state, minus side
   double* vsp;    // [5][cap]
   int* err;       // Riemann error of the capture
-  int* stats;     // [2][cap]: main-loop iterations, f(p) evaluations (last solve)
+  int* stats;     // [3][cap]: main-loop iterations, f(p) evaluations, flags (last solve)
   int cap;
 };
 
@@ -79,17 +79,16 @@ __device__ __forceinline__ void write_ghosts(const DevGrid& g, const F5& V, int 
 }
 
 // Flux through the minus face (axis D) of cell (i,j,k); returns f1 (applied to
-// the minus-side cell) and sets cell_bits_m/p for the reconstruction check.
-template <int D>
-__device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, int i, int j, int k, double* f1,
-                                          int& bm, int& bp, int& err) {
+// the minus-side cell) and the reconstruction-check bits of the two cells.
+__device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, int D, int i, int j, int k,
+                                          double* f1, int& bm, int& bp, int& err) {
   const DevGrid& g = a.g;
   const Scheme& s = a.s;
   const long c = gidx(g, i, j, k);
-  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+  const long st = axis_stride(g, D);
   const int myid = a.ID[c], nid = a.ID[c - st];
   double Vm[5], Vp[5];
-  face_pair<D>(g, s, M, a.Vin, a.ID, i, j, k, Vm, Vp, bm, bp);
+  face_pair(g, s, M, a.Vin, a.ID, D, i, j, k, Vm, Vp, bm, bp);
   if (nid == myid) {
     if (myid != s.inactive) flux_num(M.m[myid], s.flux, s.del, D, Vm, Vp, f1, err);
     else for (int p = 0; p < 5; p++) f1[p] = 0.0;
@@ -101,85 +100,87 @@ __device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, 
     else for (int p = 0; p < 5; p++) { vm[p] = Vm[p]; vp[p] = Vp[p]; }
     flux_llf(M.m[nid], M.m[myid], D, vm, vp, f1, err);
   } else {
-    const int slot = a.fidx[D][c];
+    const int* fx = (D == 0) ? a.fidx[0] : (D == 1 ? a.fidx[1] : a.fidx[2]);
+    const int slot = fx[c];
     for (int p = 0; p < 5; p++) f1[p] = a.L.f1[(long)p * a.L.cap + slot];
   }
+}
+
+// LDS slot of the face flux of axis D whose plus cell has tile-local
+// position (li, lj, lk); x faces: (TX+1) x TY x TZ, y: TX x (TY+1) x TZ,
+// z: TX x TY x (TZ+1), stored one after another.
+constexpr int NXCD = 8;  // XCDs of an MI355X
+constexpr int NFX = (TX + 1) * TY * TZ, NFY = TX * (TY + 1) * TZ, NFZ = TX * TY * (TZ + 1);
+constexpr int NFACE = NFX + NFY + NFZ;
+__device__ __forceinline__ int face_slot(int D, int li, int lj, int lk) {
+  if (D == 0) return (lk * TY + lj) * (TX + 1) + li;
+  if (D == 1) return NFX + (lk * (TY + 1) + lj) * TX + li;
+  return NFX + NFY + (lk * TY + lj) * TX + li;
 }
 
 __constant__ DevMats c_mats;
 
 template <int MODE>
 __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
-  __shared__ double sFx[5][TZ * TY * (TX + 1)];
-  __shared__ double sFy[5][TZ * (TY + 1) * TX];
-  __shared__ double sFz[5][(TZ + 1) * TY * TX];
+  __shared__ double sF[5][NFACE];
   __shared__ int sMask[NT];
   const DevGrid& g = a.g;
   const Scheme& s = a.s;
   const DevMats& M = c_mats;
   const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
   const int t = threadIdx.x;
-  const int ib = blockIdx.x * TX, jb = blockIdx.y * TY, kb = blockIdx.z * TZ;
+  // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
+  // each with its own L2; work-group w runs tile (w % 8) * per + w / 8, so
+  // every XCD sweeps one contiguous run of tiles (x fastest) and the halos
+  // its neighbouring tiles share stay in its L2.
+  const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY, ntz = (g.NZ + TZ - 1) / TZ;
+  const int ntiles = ntx * nty * ntz;
+  const int per = (ntiles + NXCD - 1) / NXCD;
+  const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+  if (tile >= ntiles) return;
+  const int ib = (tile % ntx) * TX, jb = ((tile / ntx) % nty) * TY, kb = (tile / (ntx * nty)) * TZ;
   const int i = ib + tx, j = jb + ty, k = kb + tz;
   const bool inside = (i < g.NX) && (j < g.NY) && (k < g.NZ);
   sMask[t] = 0;
   __syncthreads();
   int err = 0;
-  double f[5];
-  int bm, bp;
-  // own minus faces of this thread's cell position (also for the first
-  // position past the domain end, whose minus face closes the last cell)
-  if (i <= g.NX && j < g.NY && k < g.NZ) {
-    face_flux<0>(a, M, i, j, k, f, bm, bp, err);
-    const int fi = (tz * TY + ty) * (TX + 1) + tx;
-    for (int p = 0; p < 5; p++) sFx[p][fi] = f[p];
-    if (tx > 0 && bm) atomicOr(&sMask[t - 1], bm << 2);        // vr of cell tx-1: face slot 1
-    if (bp) atomicOr(&sMask[t], bp << 0);                       // vl of this cell: face slot 0
-  }
-  if (i < g.NX && j <= g.NY && k < g.NZ) {
-    face_flux<1>(a, M, i, j, k, f, bm, bp, err);
-    const int fi = (tz * (TY + 1) + ty) * TX + tx;
-    for (int p = 0; p < 5; p++) sFy[p][fi] = f[p];
-    if (ty > 0 && bm) atomicOr(&sMask[t - TX], bm << 6);
-    if (bp) atomicOr(&sMask[t], bp << 4);
-  }
-  if (i < g.NX && j < g.NY && k <= g.NZ) {
-    face_flux<2>(a, M, i, j, k, f, bm, bp, err);
-    const int fi = (tz * TY + ty) * TX + tx;
-    for (int p = 0; p < 5; p++) sFz[p][fi] = f[p];
-    if (tz > 0 && bm) atomicOr(&sMask[t - TX * TY], bm << 10);
-    if (bp) atomicOr(&sMask[t], bp << 8);
-  }
-  // the plus faces of the tile's last layer: one direction per wave
-  if (t < 64) {  // y: (tx, tz) -> face at j = jb + TY
-    const int ex = t % TX, ez = t / TX;
-    const int ei = ib + ex, ej = jb + TY, ek = kb + ez;
-    if (ei < g.NX && ej <= g.NY && ek < g.NZ) {
-      face_flux<1>(a, M, ei, ej, ek, f, bm, bp, err);
-      const int fi = (ez * (TY + 1) + TY) * TX + ex;
-      for (int p = 0; p < 5; p++) sFy[p][fi] = f[p];
-      if (bm) atomicOr(&sMask[ex + TX * ((TY - 1) + TY * ez)], bm << 6);
+  // Face fluxes of the tile, one loop body for all of them (a single copy of
+  // the reconstruction + flux code keeps the instruction footprint small):
+  // jobs 0..2 are the minus faces x, y, z of this thread's position (also at
+  // the first position past the domain end, whose minus face closes the last
+  // cell); job 3 (threads < 144) is one plus face of the tile's last layer:
+  // y faces for t < 64, z faces for t < 128, x faces for t < 144.
+  const int njobs = (t < 128 + TY * TZ) ? 4 : 3;
+#pragma unroll 1
+  for (int job = 0; job < njobs; job++) {
+    int D, li, lj, lk;  // axis and tile-local position of the face's plus cell
+    if (job < 3) {
+      D = job; li = tx; lj = ty; lk = tz;
+    } else if (t < 64) {
+      D = 1; li = t % TX; lj = TY; lk = t / TX;
+    } else if (t < 128) {
+      D = 2; li = (t - 64) % TX; lj = (t - 64) / TX; lk = TZ;
+    } else {
+      D = 0; li = TX; lj = (t - 128) % TY; lk = (t - 128) / TY;
     }
-  } else if (t < 128) {  // z: (tx, ty) -> face at k = kb + TZ
-    const int e = t - 64;
-    const int ex = e % TX, ey = e / TX;
-    const int ei = ib + ex, ej = jb + ey, ek = kb + TZ;
-    if (ei < g.NX && ej < g.NY && ek <= g.NZ) {
-      face_flux<2>(a, M, ei, ej, ek, f, bm, bp, err);
-      const int fi = (TZ * TY + ey) * TX + ex;
-      for (int p = 0; p < 5; p++) sFz[p][fi] = f[p];
-      if (bm) atomicOr(&sMask[ex + TX * (ey + TY * (TZ - 1))], bm << 10);
-    }
-  } else if (t < 128 + TY * TZ) {  // x: (ty, tz) -> face at i = ib + TX
-    const int e = t - 128;
-    const int ey = e % TY, ez = e / TY;
-    const int ei = ib + TX, ej = jb + ey, ek = kb + ez;
-    if (ei <= g.NX && ej < g.NY && ek < g.NZ) {
-      face_flux<0>(a, M, ei, ej, ek, f, bm, bp, err);
-      const int fi = (ez * TY + ey) * (TX + 1) + TX;
-      for (int p = 0; p < 5; p++) sFx[p][fi] = f[p];
-      if (bm) atomicOr(&sMask[(TX - 1) + TX * (ey + TY * ez)], bm << 2);
-    }
+    const int fi = ib + li, fj = jb + lj, fk = kb + lk;
+    const bool ok = (D == 0) ? (fi <= g.NX && fj < g.NY && fk < g.NZ)
+                  : (D == 1) ? (fi < g.NX && fj <= g.NY && fk < g.NZ)
+                             : (fi < g.NX && fj < g.NY && fk <= g.NZ);
+    if (!ok) continue;
+    double f[5];
+    int bm, bp;
+    face_flux(a, M, D, fi, fj, fk, f, bm, bp, err);
+    const int slot = face_slot(D, li, lj, lk);
+#pragma unroll
+    for (int p = 0; p < 5; p++) sF[p][slot] = f[p];
+    // check bits: the plus cell's minus face (slot 2D), the minus cell's plus
+    // face (slot 2D+1), for the cells inside the tile
+    const int pl = li + TX * (lj + TY * lk);
+    if (bp && li < TX && lj < TY && lk < TZ) atomicOr(&sMask[pl], bp << (4 * D));
+    const int mi = li - (D == 0), mj = lj - (D == 1), mk = lk - (D == 2);
+    if (bm && mi >= 0 && mj >= 0 && mk >= 0 && mi < TX && mj < TY && mk < TZ)
+      atomicOr(&sMask[mi + TX * (mj + TY * mk)], bm << (4 * D + 2));
   }
   __syncthreads();
   if (err) atomicOr(&a.cnt->error, 1);
@@ -201,12 +202,13 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
 
   const long c = gidx(g, i, j, k);
   const int myid = a.ID[c];
-  const double ax = g.dy[j + 1] * g.dz[k + 1];
-  const double ay = g.dx[i + 1] * g.dz[k + 1];
-  const double az = g.dx[i + 1] * g.dy[j + 1];
-  const int fxm = (tz * TY + ty) * (TX + 1) + tx, fxp = fxm + 1;
-  const int fym = (tz * (TY + 1) + ty) * TX + tx, fyp = fym + TX;
-  const int fzm = (tz * TY + ty) * TX + tx, fzp = fzm + TY * TX;
+  const double dxi = g_axis(g, 3)[i + 1], dyj = g_axis(g, 4)[j + 1], dzk = g_axis(g, 5)[k + 1];
+  const double ax = dyj * dzk;
+  const double ay = dxi * dzk;
+  const double az = dxi * dyj;
+  const int fxm = face_slot(0, tx, ty, tz), fxp = face_slot(0, tx + 1, ty, tz);
+  const int fym = face_slot(1, tx, ty, tz), fyp = face_slot(1, tx, ty + 1, tz);
+  const int fzm = face_slot(2, tx, ty, tz), fzp = face_slot(2, tx, ty, tz + 1);
   // minus faces of an interface take localflux2 from the Riemann list
   const bool mp = s.mp_flux != M2CX_MPFLUX_LLF;
   const int idx_m = a.ID[c - 1], idy_m = a.ID[c - g.SX], idz_m = a.ID[c - g.S];
@@ -215,18 +217,18 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
   const bool ifz = mp && idz_m != myid && idz_m != s.inactive && myid != s.inactive;
   const int slx = ifx ? a.fidx[0][c] : 0, sly = ify ? a.fidx[1][c] : 0, slz = ifz ? a.fidx[2][c] : 0;
   double r[5];
-  const double cst = 1.0 / (-(g.dx[i + 1] * g.dy[j + 1] * g.dz[k + 1]));
+  const double cst = 1.0 / (-(dxi * dyj * dzk));
 #pragma unroll
   for (int p = 0; p < 5; p++) {
-    const double Fm = ifx ? a.L.f2[(long)p * a.L.cap + slx] : sFx[p][fxm];
-    const double Gm = ify ? a.L.f2[(long)p * a.L.cap + sly] : sFy[p][fym];
-    const double Hm = ifz ? a.L.f2[(long)p * a.L.cap + slz] : sFz[p][fzm];
+    const double Fm = ifx ? a.L.f2[(long)p * a.L.cap + slx] : sF[p][fxm];
+    const double Gm = ify ? a.L.f2[(long)p * a.L.cap + sly] : sF[p][fym];
+    const double Hm = ifz ? a.L.f2[(long)p * a.L.cap + slz] : sF[p][fzm];
     double acc = 0.0 - ax * Fm;
     acc = acc - ay * Gm;
     acc = acc - az * Hm;
-    acc = acc + ax * sFx[p][fxp];
-    acc = acc + ay * sFy[p][fyp];
-    acc = acc + az * sFz[p][fzp];
+    acc = acc + ax * sF[p][fxp];
+    acc = acc + ay * sF[p][fyp];
+    acc = acc + az * sF[p][fzp];
     r[p] = acc * cst;
   }
 
@@ -247,7 +249,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
   } else {
     double u1[5], un[5];
 #pragma unroll
-    for (int p = 0; p < 5; p++) { u1[p] = a.U1.c[p][c]; un[p] = a.Un.c[p][c]; }
+    for (int p = 0; p < 5; p++) { u1[p] = a.U1(p, c); un[p] = a.Un(p, c); }
     if (MODE == MODE_RK2_FINAL) {
       const double h = 0.5 * dt;
 #pragma unroll
@@ -280,7 +282,8 @@ __global__ void k_resync(DevGrid g, CF5 V, F5 U, const unsigned char* __restrict
   if (cnt->stage_clipped == 0) return;
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     double v[5], u[5];
     load5(V, c, v);
@@ -295,10 +298,11 @@ __global__ void k_reset_stage(DevCounters* cnt) { cnt->stage_clipped = 0; }
 __global__ void k_apply_bc(DevGrid g, F5 V) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     if (i != 0 && i != g.NX - 1 && j != 0 && j != g.NY - 1 && k != 0 && k != g.NZ - 1) continue;
     double v[5];
-    CF5 cv{{V.c[0], V.c[1], V.c[2], V.c[3], V.c[4]}};
+    CF5 cv{V.b, V.cs};
     load5(cv, gidx(g, i, j, k), v);
     write_ghosts(g, V, i, j, k, v);
   }
@@ -308,15 +312,16 @@ __global__ void k_apply_bc(DevGrid g, F5 V) {
 __global__ void k_clip(DevGrid g, F5 V, const unsigned char* __restrict__ ID, DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     double v[5];
-    for (int p = 0; p < 5; p++) v[p] = V.c[p][c];
+    for (int p = 0; p < 5; p++) v[p] = V(p, c);
     const DevEos& E = c_mats.m[ID[c]];
     if (eos_clip(E, v)) {
       atomicAdd(&cnt->clipped, 1ull);
       atomicAdd(&cnt->stage_clipped, 1ull);
-      for (int p = 0; p < 5; p++) V.c[p][c] = v[p];
+      for (int p = 0; p < 5; p++) V(p, c) = v[p];
     }
     if (eos_check5(E, v)) atomicOr(&cnt->error, 1);
   }
@@ -337,7 +342,8 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
   double m = DBL_MAX;
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     int id = ID[c];
     if (id == s.inactive) continue;
@@ -349,9 +355,9 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
     if (c2 < 0) atomicOr(&cnt->error, 1);
     double cc = sqrt(c2);
     double lf = fabs(v[1]) + cc, lg = fabs(v[2]) + cc, lh = fabs(v[3]) + cc;
-    if (g.NX > 1) m = nmin(m, g.dx[i + 1] / lf);
-    if (g.NY > 1) m = nmin(m, g.dy[j + 1] / lg);
-    if (g.NZ > 1) m = nmin(m, g.dz[k + 1] / lh);
+    if (g.NX > 1) m = nmin(m, g_axis(g, 3)[i + 1] / lf);
+    if (g.NY > 1) m = nmin(m, g_axis(g, 4)[j + 1] / lg);
+    if (g.NZ > 1) m = nmin(m, g_axis(g, 5)[k + 1] / lh);
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -401,11 +407,11 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
 // ---------------------------------------------------------------------------
 __global__ void k_aos_to_soa(const double* __restrict__ src, F5 dst, int dof, long n) {
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
-    for (int p = 0; p < dof; p++) dst.c[p][c] = src[c * dof + p];
+    for (int p = 0; p < dof; p++) dst(p, c) = src[c * dof + p];
 }
 __global__ void k_soa_to_aos(CF5 src, double* __restrict__ dst, int dof, long n) {
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
-    for (int p = 0; p < dof; p++) dst[c * dof + p] = src.c[p][c];
+    for (int p = 0; p < dof; p++) dst[c * dof + p] = src(p, c);
 }
 __global__ void k_i32_to_u8(const int* __restrict__ src, unsigned char* __restrict__ dst, long n) {
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)

@@ -68,11 +68,15 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
     if (hipMalloc(&B.fidx[d], n * 4) != hipSuccess) return M2CX_E_OOM;
     if (hipMemsetAsync(B.fidx[d], 0, n * 4, st) != hipSuccess) return M2CX_E_HIP;
   }
-  long cap = std::max<long>(1 << 16, std::min<long>(nin / 2, 1L << 22));
+  // interface faces: a level-set surface crosses at most a few faces per
+  // cell of a cross-section; 16 x (sum of the three cross-sections) covers
+  // any bubble population here, and overflow is reported as an error
+  const long sections = (long)g.NX * g.NY + (long)g.NY * g.NZ + (long)g.NX * g.NZ;
+  long cap = std::max<long>(1 << 16, std::min<long>(std::min<long>(nin, 16 * sections), 1L << 22));
   B.L.cap = (int)cap;
   if (hipMalloc(&B.L.cell, cap * 4) || hipMalloc(&B.L.dir, cap * 4) || hipMalloc(&B.L.f1, cap * 40) ||
       hipMalloc(&B.L.f2, cap * 40) || hipMalloc(&B.L.vsm, cap * 40) || hipMalloc(&B.L.vsp, cap * 40) ||
-      hipMalloc(&B.L.err, cap * 4) || hipMalloc(&B.L.stats, cap * 8))
+      hipMalloc(&B.L.err, cap * 4) || hipMalloc(&B.L.stats, cap * 12))
     return M2CX_E_OOM;
   for (double** p : {&B.phi, &B.phi1, &B.phin, &B.rls, &B.phitmp}) {
     if (hipMalloc(p, n * 8) != hipSuccess) return M2CX_E_OOM;
@@ -128,7 +132,8 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
   // per-lane integration-path scratch: lanes x 2 paths x cap points x 3
   B.lanes = 256 * 128;
   B.pathcap = 512;
-  if (hipMalloc(&B.slab, (long)B.lanes * 2 * B.pathcap * 3 * 8) != hipSuccess) return M2CX_E_OOM;
+  // the integration-path slab is allocated on first use (mp_slab): solvers
+  // that run in an ensemble share the ensemble's slab instead
   B.list_cap = (int)std::max<long>(1 << 16, std::min<long>(nin, 1L << 22));
   if (hipMalloc(&B.remaining, B.list_cap * 4) || hipMalloc(&B.changed, B.list_cap * 4) ||
       hipMalloc(&B.unresolved, B.list_cap * 4) || hipMalloc(&B.tag, n) || hipMalloc(&B.done, n) ||
@@ -157,7 +162,8 @@ __global__ void k_classify(DevGrid g, Scheme s, const unsigned char* __restrict_
                            IfaceList L, DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     int my = ID[c];
     if (my == s.inactive) continue;
@@ -183,10 +189,9 @@ __global__ void k_classify(DevGrid g, Scheme s, const unsigned char* __restrict_
 // ---------------------------------------------------------------------------
 // exact Riemann over the interface list
 // ---------------------------------------------------------------------------
-template <int D>
 __device__ __forceinline__ void if_states(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
-                                          const unsigned char* ID, long c, double* Vm, double* Vp) {
-  const long st = (D == 0) ? 1 : (D == 1 ? g.SX : g.S);
+                                          const unsigned char* ID, int D, long c, double* Vm, double* Vp) {
+  const long st = axis_stride(g, D);
   if (s.mp_recon == M2CX_MPREC_CONSTANT) {
     load5(V, c - st, Vm);
     load5(V, c, Vp);
@@ -199,8 +204,69 @@ __device__ __forceinline__ void if_states(const DevGrid& g, const Scheme& s, con
   int k = (int)(r / g.SY) - 1;
   int im = (D == 0) ? i - 1 : i, jm = (D == 1) ? j - 1 : j, km = (D == 2) ? k - 1 : k;
   bool f, u;
-  face_state<D>(g, s, M, V, ID, im, jm, km, 1, Vm, f, u);
-  face_state<D>(g, s, M, V, ID, i, j, k, 0, Vp, f, u);
+  face_state(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
+  face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
+}
+
+// One interface face: the face states, the exact solve and the two fluxes
+// into the list (SpaceOperator.cpp:1613-1690), shared by the single-solver
+// and the ensemble kernels.  Returns flag bits: 1 trivial, 2 error, 4
+// fallback solver used, 8 learned guess rejected.
+__device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, const CF5& V,
+                                            const unsigned char* __restrict__ ID, const IfaceList& L, int q,
+                                            int stage1, const float* guess, DevRiemann& R, int& fatal, int& ovf) {
+  const DevMats& M = c_mats;
+  const long c = L.cell[q];
+  const int d = L.dir[q];
+  const long st = axis_stride(g, d);
+  const int idl = ID[c - st], idr = ID[c];
+  double Vm[5], Vp[5];
+  if_states(g, s, M, V, ID, d, c, Vm, Vp);
+  const double dir[3] = {(d == 0) ? 1.0 : 0.0, (d == 1) ? 1.0 : 0.0, (d == 2) ? 1.0 : 0.0};
+  // learned p* guess for this face (m2c_mlp.h), rejected when not a finite
+  // positive pressure: the acoustic guess is then used
+  double p0g = -1.0;
+  int flags = 0;
+  if (guess) {
+    const float gp = guess[q];
+    p0g = (isfinite(gp) && gp > 0.0f) ? (double)gp : -1.0;
+    if (p0g <= 0) flags |= 8;
+  }
+  double Vs[5], Vsm[5], Vsp[5];
+  int mid = idl;
+  R.st = RStats{0, 0, false, false, false};
+  R.path1.overflow = R.path3.overflow = false;
+  const int err = R.solve(dir, Vm, idl, Vp, idr, Vs, mid, Vsm, Vsp, p0g);
+  flags |= (R.st.trivial ? 1 : 0) | (err != 0 ? 2 : 0) | (R.st.fallback ? 4 : 0);
+  fatal |= R.st.fatal;
+  ovf |= R.path1.overflow || R.path3.overflow;
+  eos_clip(M.m[idl], Vsm);
+  eos_clip(M.m[idr], Vsp);
+  if (stage1) {
+    for (int p = 0; p < 5; p++) {
+      L.vsm[(long)p * L.cap + q] = Vsm[p];
+      L.vsp[(long)p * L.cap + q] = Vsp[p];
+    }
+    L.err[q] = err;
+  }
+  L.stats[q] = R.st.it_main;
+  L.stats[L.cap + q] = R.st.fevals;
+  L.stats[2L * L.cap + q] = flags;
+  double f1[5], f2[5];
+  int ferr = 0;
+  if (s.mp_flux == M2CX_MPFLUX_EXACT) {
+    flux_phys(M.m[mid], d, Vs, f1);
+    for (int p = 0; p < 5; p++) f2[p] = f1[p];
+  } else {
+    flux_num(M.m[idl], s.flux, s.del, d, Vm, Vsm, f1, ferr);
+    flux_num(M.m[idr], s.flux, s.del, d, Vsp, Vp, f2, ferr);
+  }
+  for (int p = 0; p < 5; p++) {
+    L.f1[(long)p * L.cap + q] = f1[p];
+    L.f2[(long)p * L.cap + q] = f2[p];
+  }
+  if (ferr) fatal = 1;
+  return flags;
 }
 
 __global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
@@ -210,67 +276,18 @@ __global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= lanes) return;
   const int nif = min(cnt->nif, L.cap);
-  const DevMats& M = c_mats;
-  DevRiemann R(M, rc, slab + lane, lanes, pathcap);
+  DevRiemann R(c_mats, rc, slab + lane, lanes, pathcap);
   unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
   int fatal = 0, ovf = 0;
   for (int q = lane; q < nif; q += lanes) {
-    const long c = L.cell[q];
-    const int d = L.dir[q];
-    const long st = (d == 0) ? 1 : (d == 1 ? g.SX : g.S);
-    const int idl = ID[c - st], idr = ID[c];
-    double Vm[5], Vp[5];
-    if (d == 0) if_states<0>(g, s, M, V, ID, c, Vm, Vp);
-    else if (d == 1) if_states<1>(g, s, M, V, ID, c, Vm, Vp);
-    else if_states<2>(g, s, M, V, ID, c, Vm, Vp);
-    double dir[3] = {0.0, 0.0, 0.0};
-    dir[d] = 1.0;
-    // learned p* guess for this face (m2c_mlp.h), rejected when not a
-    // finite positive pressure: the acoustic guess is then used
-    double p0g = -1.0;
-    if (guess) {
-      float gp = guess[q];
-      p0g = (isfinite(gp) && gp > 0.0f) ? (double)gp : -1.0;
-      if (p0g <= 0) gfb++;
-    }
-    double Vs[5], Vsm[5], Vsp[5];
-    int mid = idl;
-    R.st = RStats{0, 0, false, false, false};
-    R.path1.overflow = R.path3.overflow = false;
-    int err = R.solve(dir, Vm, idl, Vp, idr, Vs, mid, Vsm, Vsp, p0g);
+    const int f = riemann_face(g, s, V, ID, L, q, stage1, guess, R, fatal, ovf);
     calls++;
-    triv += R.st.trivial;
-    errs += (err != 0);
+    triv += f & 1;
+    errs += (f >> 1) & 1;
+    fb += (f >> 2) & 1;
+    gfb += (f >> 3) & 1;
     its += R.st.it_main;
     fev += R.st.fevals;
-    fb += R.st.fallback;
-    fatal |= R.st.fatal;
-    ovf |= R.path1.overflow || R.path3.overflow;
-    eos_clip(M.m[idl], Vsm);
-    eos_clip(M.m[idr], Vsp);
-    if (stage1) {
-      for (int p = 0; p < 5; p++) {
-        L.vsm[(long)p * L.cap + q] = Vsm[p];
-        L.vsp[(long)p * L.cap + q] = Vsp[p];
-      }
-      L.err[q] = err;
-    }
-    L.stats[q] = R.st.it_main;
-    L.stats[L.cap + q] = R.st.fevals;
-    double f1[5], f2[5];
-    int ferr = 0;
-    if (s.mp_flux == M2CX_MPFLUX_EXACT) {
-      flux_phys(M.m[mid], d, Vs, f1);
-      for (int p = 0; p < 5; p++) f2[p] = f1[p];
-    } else {
-      flux_num(M.m[idl], s.flux, s.del, d, Vm, Vsm, f1, ferr);
-      flux_num(M.m[idr], s.flux, s.del, d, Vsp, Vp, f2, ferr);
-    }
-    for (int p = 0; p < 5; p++) {
-      L.f1[(long)p * L.cap + q] = f1[p];
-      L.f2[(long)p * L.cap + q] = f2[p];
-    }
-    if (ferr) fatal = 1;
   }
   if (calls) {
     atomicAdd(&cnt->riemann_calls, calls);
@@ -285,12 +302,94 @@ __global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg
   if (ovf) atomicOr(&cnt->error, 4);
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble form: the interface faces of many independent solvers in one
+// launch.  Each member describes its solver; `off` holds the prefix sums of
+// the members' face counts (k_riem_offsets); lanes walk the concatenated
+// list, so the latency-bound exact solves of all members overlap.
+// ---------------------------------------------------------------------------
+struct RiemMember {
+  DevGrid g;
+  Scheme s;
+  RiemannCfg rc;
+  CF5 V;
+  const unsigned char* ID;
+  IfaceList L;
+  const float* guess;
+  DevCounters* cnt;
+};
+
+__global__ void k_riem_offsets(const RiemMember* mem, int nm, int* off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int acc = 0;
+  for (int m = 0; m < nm; m++) {
+    off[m] = acc;
+    acc += min(mem[m].cnt->nif, mem[m].L.cap);
+  }
+  off[nm] = acc;
+}
+
+__global__ void __launch_bounds__(128) k_riemann_multi(const RiemMember* __restrict__ mem, int nm,
+                                                       const int* __restrict__ off, int stage1, double* slab,
+                                                       int lanes, int pathcap) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= lanes) return;
+  const int total = off[nm];
+  for (int q = lane; q < total; q += lanes) {
+    int lo = 0, hi = nm - 1;  // member m with off[m] <= q < off[m+1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const RiemMember& B = mem[lo];
+    DevRiemann R(c_mats, B.rc, slab + lane, lanes, pathcap);
+    int fatal = 0, ovf = 0;
+    riemann_face(B.g, B.s, B.V, B.ID, B.L, q - off[lo], stage1, B.guess, R, fatal, ovf);
+    if (fatal) atomicOr(&B.cnt->error, 2);
+    if (ovf) atomicOr(&B.cnt->error, 4);
+  }
+}
+
+// per-member counters of an ensemble Riemann launch (one block per member)
+__global__ void __launch_bounds__(256) k_riem_count(const RiemMember* mem) {
+  const RiemMember& B = mem[blockIdx.x];
+  const int nif = min(B.cnt->nif, B.L.cap);
+  __shared__ unsigned long long acc[6][256];
+  unsigned long long a[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = threadIdx.x; q < nif; q += blockDim.x) {
+    const int f = B.L.stats[2L * B.L.cap + q];
+    a[0] += f & 1;
+    a[1] += (f >> 1) & 1;
+    a[2] += (f >> 2) & 1;
+    a[3] += (f >> 3) & 1;
+    a[4] += B.L.stats[q];
+    a[5] += B.L.stats[B.L.cap + q];
+  }
+  for (int k = 0; k < 6; k++) acc[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 6; k++) acc[k][threadIdx.x] += acc[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && nif > 0) {
+    atomicAdd(&B.cnt->riemann_calls, (unsigned long long)nif);
+    atomicAdd(&B.cnt->riemann_trivial, acc[0][0]);
+    atomicAdd(&B.cnt->riemann_errors, acc[1][0]);
+    atomicAdd(&B.cnt->riemann_fallbacks, acc[2][0]);
+    atomicAdd(&B.cnt->guess_fallbacks, acc[3][0]);
+    atomicAdd(&B.cnt->riemann_iters, acc[4][0]);
+    atomicAdd(&B.cnt->riemann_fevals, acc[5][0]);
+  }
+}
+
 // point-kernel entry: n problems given explicitly
 __global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, const double* dir, const double* Vm,
                                                        const int* idm, const double* Vp, const int* idp,
                                                        double* Vs, int* id, double* Vsm, double* Vsp, int* err,
                                                        int* iters, int* fevals, double* slab, int lanes,
-                                                       int pathcap, DevCounters* cnt) {
+                                                       int pathcap, const float* guess, DevCounters* cnt) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= lanes) return;
   DevRiemann R(c_mats, rc, slab + lane, lanes, pathcap);
@@ -298,7 +397,13 @@ __global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, con
     R.st = RStats{0, 0, false, false, false};
     R.path1.overflow = R.path3.overflow = false;
     int mid = idm[q];
-    err[q] = R.solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q, Vsp + 5 * q);
+    double p0g = -1.0;
+    if (guess) {
+      const float gp = guess[q];
+      p0g = (isfinite(gp) && gp > 0.0f) ? (double)gp : -1.0;
+    }
+    err[q] = R.solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q, Vsp + 5 * q,
+                     p0g);
     id[q] = mid;
     iters[q] = R.st.it_main;
     fevals[q] = R.st.fevals;
@@ -307,7 +412,11 @@ __global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, con
       id[q] = -1;
       for (int p = 0; p < 5; p++) Vs[5 * q + p] = Vsm[5 * q + p] = Vsp[5 * q + p] = __builtin_nan("");
     }
-    if (R.path1.overflow || R.path3.overflow) atomicOr(&cnt->error, 4);
+    if (R.path1.overflow || R.path3.overflow) {  // integration path longer than the slab holds
+      err[q] = -2;
+      id[q] = -1;
+      for (int p = 0; p < 5; p++) Vs[5 * q + p] = Vsm[5 * q + p] = Vsp[5 * q + p] = __builtin_nan("");
+    }
   }
 }
 
@@ -334,7 +443,8 @@ __global__ void k_levelset(LSArgs a) {
   long n = (long)g.NX * g.NY * g.NZ;
   const double dt = a.cnt->dt;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     const double* P = a.phi_st;
     const long sx = 1, sy = g.SX, sz = g.S;
@@ -342,15 +452,18 @@ __global__ void k_levelset(LSArgs a) {
       const double* cc = a.fd3 + off + 4L * idx;
       return cc[0] * P[c0] + cc[1] * P[c0 + st] + cc[2] * P[c0 + 2 * st] + cc[3] * P[c0 + 3 * st];
     };
-    const double xi = g.x[i + 1], yj = g.y[j + 1], zk = g.z[k + 1];
-    double av = (i - 2 >= -1) ? fd(a.ocxL, i, sx, c - 2 * sx) : (P[c] - P[c - sx]) / (xi - g.x[i]);
-    double bv = (i + 2 <= g.NX) ? fd(a.ocxR, i, sx, c - sx) : (P[c + sx] - P[c]) / (g.x[i + 2] - xi);
-    double cv = (j - 2 >= -1) ? fd(a.ocyL, j, sy, c - 2 * sy) : (P[c] - P[c - sy]) / (yj - g.y[j]);
+    const double* gx = g_axis(g, 0);
+    const double* gy = g_axis(g, 1);
+    const double* gz = g_axis(g, 2);
+    const double xi = gx[i + 1], yj = gy[j + 1], zk = gz[k + 1];
+    double av = (i - 2 >= -1) ? fd(a.ocxL, i, sx, c - 2 * sx) : (P[c] - P[c - sx]) / (xi - gx[i]);
+    double bv = (i + 2 <= g.NX) ? fd(a.ocxR, i, sx, c - sx) : (P[c + sx] - P[c]) / (gx[i + 2] - xi);
+    double cv = (j - 2 >= -1) ? fd(a.ocyL, j, sy, c - 2 * sy) : (P[c] - P[c - sy]) / (yj - gy[j]);
     double dv = (j + 2 <= g.NY) ? fd((i < g.NY - 1) ? a.ocyR3 : a.ocyR1, j, sy, c - sy)
-                                : (P[c + sy] - P[c]) / (g.y[j + 2] - yj);
-    double ev = (k - 2 >= -1) ? fd(a.oczL, k, sz, c - 2 * sz) : (P[c] - P[c - sz]) / (zk - g.z[k]);
-    double fv = (k + 2 <= g.NZ) ? fd(a.oczR, k, sz, c - sz) : (P[c + sz] - P[c]) / (g.z[k + 2] - zk);
-    const double u = a.V.c[1][c], v = a.V.c[2][c], w = a.V.c[3][c];
+                                : (P[c + sy] - P[c]) / (gy[j + 2] - yj);
+    double ev = (k - 2 >= -1) ? fd(a.oczL, k, sz, c - 2 * sz) : (P[c] - P[c - sz]) / (zk - gz[k]);
+    double fv = (k + 2 <= g.NZ) ? fd(a.oczR, k, sz, c - sz) : (P[c + sz] - P[c]) / (gz[k + 2] - zk);
+    const double u = a.V(1, c), v = a.V(2, c), w = a.V(3, c);
     double r = (u >= 0 ? av * u : bv * u) + (v >= 0 ? cv * v : dv * v) + (w >= 0 ? ev * w : fv * w);
     r = r * -1.0;
     double o;
@@ -393,7 +506,7 @@ __global__ void k_ls_bc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, 
       const int N = ax == 0 ? g.NX : (ax == 1 ? g.NY : g.NZ);
       const int a = ax == 0 ? i : (ax == 1 ? j : k);
       const int dirn = (side % 2 == 0) ? 1 : -1;
-      const double* X = ax == 0 ? g.x : (ax == 1 ? g.y : g.z);
+      const double* X = g_axis(g, ax);
       const long st = ax == 0 ? 1 : (ax == 1 ? g.SX : g.S);
       bool ok = (side % 2 == 0) ? (a + 2 < N) : (a - 2 >= 0);
       if (ok) {
@@ -429,7 +542,8 @@ __global__ void k_id_sweep(DevGrid g, Scheme s, const double* __restrict__ phi0,
                            DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     if (ID[c] == s.inactive) continue;
     double a = phi[c], b = phi0[c];
@@ -532,7 +646,8 @@ __global__ void k_changed(DevGrid g, Scheme s, const unsigned char* __restrict__
                           const unsigned char* __restrict__ IDn, int* changed, int cap, DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i = q % g.NX, j = (q / g.NX) % g.NY, k = q / ((long)g.NX * g.NY);
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     if (ID[c] == IDn[c] || ID[c] == s.inactive) continue;
     int slot = atomicAdd(&cnt->n_changed, 1);
@@ -563,7 +678,7 @@ __device__ bool local_update(const UpdArgs& a, long c, int id, bool upwind) {
   int counter = 0;
   double sum_weight = 0.0;
   double v[5];
-  for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+  for (int p = 0; p < 5; p++) v[p] = a.V(p, c);
   long r = c;
   const int i = (int)(r % g.SX) - 1; r /= g.SX;
   const int j = (int)(r % g.SY) - 1;
@@ -584,12 +699,12 @@ __device__ bool local_update(const UpdArgs& a, long c, int id, bool upwind) {
     if (a.L.err[slot] != 0) continue;
     const int sid = plus ? idp : idm;                // ID stored with the star state
     if (sid != id) continue;
-    const double vn = a.V.c[d + 1][nb];
+    const double vn = a.V(d + 1, nb);
     if (upwind && !(plus ? (vn < 0) : (vn > 0))) continue;
     const double* sv = plus ? a.L.vsp : a.L.vsm;
     double weight = 1.0;
     if (upwind) {
-      const double u1 = a.V.c[1][nb], u2 = a.V.c[2][nb], u3 = a.V.c[3][nb];
+      const double u1 = a.V(1, nb), u2 = a.V(2, nb), u3 = a.V(3, nb);
       const double nrm = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
       weight = (plus ? -vn : vn) / nrm;
     }
@@ -605,7 +720,7 @@ __device__ bool local_update(const UpdArgs& a, long c, int id, bool upwind) {
     for (int p = 0; p < 5; p++) v[p] *= cst;
   }
   if (counter > 0)
-    for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+    for (int p = 0; p < 5; p++) a.V(p, c) = v[p];
   return counter > 0;
 }
 
@@ -690,10 +805,13 @@ __global__ void k_fix_unresolved(UpdArgs a) {
     const int j = (int)(r % g.SY) - 1;
     const int k = (int)(r / g.SY) - 1;
     const int myid = a.ID[c];
-    const double x0[3] = {g.x[i + 1], g.y[j + 1], g.z[k + 1]};
+    const double* gx = g_axis(g, 0);
+    const double* gy = g_axis(g, 1);
+    const double* gz = g_axis(g, 2);
+    const double x0[3] = {gx[i + 1], gy[j + 1], gz[k + 1]};
     double sum_weight = 0.0, sum_weight2 = 0.0, vtmp[5] = {0, 0, 0, 0, 0};
     double v[5];
-    for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+    for (int p = 0; p < 5; p++) v[p] = a.V(p, c);
     for (int nk = k - 1; nk <= k + 1; nk++)
       for (int nj = j - 1; nj <= j + 1; nj++)
         for (int ni = i - 1; ni <= i + 1; ni++) {
@@ -703,11 +821,11 @@ __global__ void k_fix_unresolved(UpdArgs a) {
           if (nc == c) continue;
           if (unres(nc)) continue;
           double vn[5];
-          for (int p = 0; p < 5; p++) vn[p] = a.V.c[p][nc];
+          for (int p = 0; p < 5; p++) vn[p] = a.V(p, nc);
           double v1[3] = {vn[1], vn[2], vn[3]};
           double v1n = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
           if (v1n != 0) { double cs = 1.0 / v1n; v1[0] *= cs; v1[1] *= cs; v1[2] *= cs; }
-          double xx[3] = {x0[0] - g.x[ni + 1], x0[1] - g.y[nj + 1], x0[2] - g.z[nk + 1]};
+          double xx[3] = {x0[0] - gx[ni + 1], x0[1] - gy[nj + 1], x0[2] - gz[nk + 1]};
           double xn = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
           double cs = 1.0 / xn;
           xx[0] *= cs; xx[1] *= cs; xx[2] *= cs;
@@ -741,16 +859,16 @@ __global__ void k_fix_unresolved(UpdArgs a) {
               if (a.ID[nc] != myid) continue;
               if (nc == c) continue;
               if (unres(nc)) continue;
-              double dx0 = g.x[ni + 1] - x0[0], dx1 = g.y[nj + 1] - x0[1], dx2 = g.z[nk + 1] - x0[2];
+              double dx0 = gx[ni + 1] - x0[0], dx1 = gy[nj + 1] - x0[1], dx2 = gz[nk + 1] - x0[2];
               double dist = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
               sum_weight += 1.0 / dist;
-              density += (1.0 / dist) * a.V.c[0][nc];
+              density += (1.0 / dist) * a.V(0, nc);
             }
         if (sum_weight > 0) { v[0] = density / sum_weight; break; }
       }
       if (sum_weight == 0) v[0] = c_mats.m[myid].failsafe;
     }
-    for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+    for (int p = 0; p < 5; p++) a.V(p, c) = v[p];
   }
 }
 
@@ -764,11 +882,11 @@ __global__ void k_post_update(UpdArgs a) {
     long c = a.changed[q];
     a.done[c] = 0;
     double v[5];
-    for (int p = 0; p < 5; p++) v[p] = a.V.c[p][c];
+    for (int p = 0; p < 5; p++) v[p] = a.V(p, c);
     const DevEos& E = c_mats.m[a.ID[c]];
     if (eos_clip(E, v)) {
       atomicAdd(&a.cnt->clipped, 1ull);
-      for (int p = 0; p < 5; p++) a.V.c[p][c] = v[p];
+      for (int p = 0; p < 5; p++) a.V(p, c) = v[p];
     }
     if (eos_check5(E, v)) atomicOr(&a.cnt->error, 1);
     long r = c;
@@ -792,6 +910,11 @@ inline void mp_classify(MPBuffers& B, const DevGrid& g, const Scheme& s, const u
                                                                      cnt);
 }
 
+inline int mp_slab(MPBuffers& B) {
+  if (B.slab) return 0;
+  return hipMalloc(&B.slab, (long)B.lanes * 2 * B.pathcap * 3 * 8) == hipSuccess ? 0 : M2CX_E_OOM;
+}
+
 inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, const unsigned char* ID, int stage1,
                        DevCounters* cnt, hipStream_t st) {
   if (s.mp_flux == M2CX_MPFLUX_LLF) return;
@@ -799,11 +922,19 @@ inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, c
                                                    B.guess ? B.pguess : nullptr, cnt);
 }
 
+// plain copies as kernels (no memcpy nodes when a step is captured in a graph)
+__global__ void k_copy_u8(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) dst[c] = src[c];
+}
+__global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) dst[c] = src[c];
+}
+
 inline void mp_begin_step(MPBuffers& B, const DevGrid& g, const Scheme& s, const unsigned char* ID, DevCounters* cnt,
                           hipStream_t st) {
   mp_classify(B, g, s, ID, cnt, st);
-  (void)hipMemcpyAsync(B.IDn, ID, g.ncells, hipMemcpyDeviceToDevice, st);
-  if (B.nls) (void)hipMemcpyAsync(B.phin, B.phi, g.ncells * 8, hipMemcpyDeviceToDevice, st);
+  k_copy_u8<<<blocks_for(g.ncells), 256, 0, st>>>(B.IDn, ID, g.ncells);
+  if (B.nls) k_copy_f64<<<blocks_for(g.ncells), 256, 0, st>>>(B.phin, B.phi, g.ncells);
 }
 
 // which: -1 residual only, 0 FE, 1 RK stage 1, 2 RK2 final, 3 RK3 stage 2, 4 RK3 final
@@ -864,45 +995,6 @@ inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, u
   k_state_update<<<1, 1024, 0, st>>>(a);
   k_fix_unresolved<<<1, 64, 0, st>>>(a);
   k_post_update<<<256, 256, 0, st>>>(a);
-}
-
-inline int mp_riemann_batch(MPBuffers& B, const Scheme& s, int n, const double* dir, const double* Vm,
-                            const int* idm, const double* Vp, const int* idp, double* Vs, int* id, double* Vsm,
-                            double* Vsp, int* err, int* iters, int* fevals, DevCounters* cnt, hipStream_t st,
-                            std::string* msg) {
-  (void)s;
-  double *dd, *dVm, *dVp, *dVs, *dVsm, *dVsp;
-  int *didm, *didp, *did, *derr, *dit, *dfe;
-  const long N = n;
-  if (hipMalloc(&dd, N * 24) || hipMalloc(&dVm, N * 40) || hipMalloc(&dVp, N * 40) || hipMalloc(&dVs, N * 40) ||
-      hipMalloc(&dVsm, N * 40) || hipMalloc(&dVsp, N * 40) || hipMalloc(&didm, N * 4) || hipMalloc(&didp, N * 4) ||
-      hipMalloc(&did, N * 4) || hipMalloc(&derr, N * 4) || hipMalloc(&dit, N * 4) || hipMalloc(&dfe, N * 4)) {
-    if (msg) *msg = "out of device memory";
-    return M2CX_E_OOM;
-  }
-  (void)hipMemcpy(dd, dir, N * 24, hipMemcpyHostToDevice);
-  (void)hipMemcpy(dVm, Vm, N * 40, hipMemcpyHostToDevice);
-  (void)hipMemcpy(dVp, Vp, N * 40, hipMemcpyHostToDevice);
-  (void)hipMemcpy(didm, idm, N * 4, hipMemcpyHostToDevice);
-  (void)hipMemcpy(didp, idp, N * 4, hipMemcpyHostToDevice);
-  k_riemann_batch<<<(B.lanes + 127) / 128, 128, 0, st>>>(B.rc, n, dd, dVm, didm, dVp, didp, dVs, did, dVsm, dVsp,
-                                                         derr, dit, dfe, B.slab, B.lanes, B.pathcap, cnt);
-  hipError_t e = hipStreamSynchronize(st);
-  (void)hipMemcpy(Vs, dVs, N * 40, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Vsm, dVsm, N * 40, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Vsp, dVsp, N * 40, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(id, did, N * 4, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(err, derr, N * 4, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(iters, dit, N * 4, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(fevals, dfe, N * 4, hipMemcpyDeviceToHost);
-  for (void* p : {(void*)dd, (void*)dVm, (void*)dVp, (void*)dVs, (void*)dVsm, (void*)dVsp, (void*)didm, (void*)didp,
-                  (void*)did, (void*)derr, (void*)dit, (void*)dfe})
-    (void)hipFree(p);
-  if (e != hipSuccess) {
-    if (msg) *msg = hipGetErrorString(e);
-    return M2CX_E_HIP;
-  }
-  return 0;
 }
 
 }  // namespace m2c

@@ -147,14 +147,22 @@ __device__ __forceinline__ bool eos_would_clip(const DevEos& E, const double* V)
 }
 
 // ---------------------------------------------------------------- fluxes
+// Velocity component normal to axis d.  The empty asm pins the three
+// candidates in registers, so that the compiler cannot turn the select into
+// an indexed load from a private array (which would live in scratch memory).
+__device__ __forceinline__ double vsel(const double* V, int d) {
+  double a = V[1], b = V[2], c = V[3];
+  asm("" : "+v"(a), "+v"(b), "+v"(c));
+  return d == 0 ? a : (d == 1 ? b : c);
+}
+
 __device__ __forceinline__ void flux_phys(const DevEos& E, int d, const double* V, double* F) {
   double H = eos_H(E, V);
-  double m = V[0] * V[d + 1];
+  double m = V[0] * vsel(V, d);
   F[0] = m;
-  F[1] = m * V[1];
-  F[2] = m * V[2];
-  F[3] = m * V[3];
-  F[d + 1] = m * V[d + 1] + V[4];
+  F[1] = (d == 0) ? m * V[1] + V[4] : m * V[1];
+  F[2] = (d == 1) ? m * V[2] + V[4] : m * V[2];
+  F[3] = (d == 2) ? m * V[3] + V[4] : m * V[3];
   F[4] = m * H;
 }
 
@@ -202,7 +210,7 @@ __device__ __forceinline__ void roe_hat(const DevEos& E, const double* Vm, const
 
 __device__ __forceinline__ void hllc_star(const DevEos& E, int d, const double* V, double S, double Ss,
                                           double* F) {
-  double un = V[d + 1];
+  double un = vsel(V, d);
   double rs = V[0] * (S - un) / (S - Ss);
   double U[5];
   eos_p2c(E, V, U);
@@ -223,24 +231,24 @@ __device__ __forceinline__ void hllc_star(const DevEos& E, int d, const double* 
 __device__ __forceinline__ void flux_hllc(const DevEos& E, int d, const double* Vm, const double* Vp, double* F,
                                           int& err) {
   RoeHat h;
-  roe_hat(E, Vm, Vp, Vp[d + 1] - Vm[d + 1], h);
+  roe_hat(E, Vm, Vp, vsel(Vp, d) - vsel(Vm, d), h);
   double em = eos_e(E, Vm[0], Vm[4]);
   double ep = eos_e(E, Vp[0], Vp[4]);
   double cm2 = eos_c2(E, Vm[0], em);
   double cp2 = eos_c2(E, Vp[0], ep);
   if (cm2 < 0 || cp2 < 0) err |= 1;
   double cm = sqrt(cm2), cp = sqrt(cp2);
-  double un_hat = wavg(h.srm, Vm[d + 1], h.srp, Vp[d + 1]);
-  double a = Vm[d + 1] - cm, b = un_hat - h.c_hat;
+  double un_hat = wavg(h.srm, vsel(Vm, d), h.srp, vsel(Vp, d));
+  double a = vsel(Vm, d) - cm, b = un_hat - h.c_hat;
   double Sm = (b < a) ? b : a;                 // std::min
-  double a2 = Vp[d + 1] + cp, b2 = un_hat + h.c_hat;
+  double a2 = vsel(Vp, d) + cp, b2 = un_hat + h.c_hat;
   double Sp = (a2 < b2) ? b2 : a2;             // std::max
   if (0.0 <= Sm) {
     flux_phys(E, d, Vm, F);
   } else if (0.0 >= Sp) {
     flux_phys(E, d, Vp, F);
   } else {
-    double vm = Vm[d + 1], vp = Vp[d + 1];
+    double vm = vsel(Vm, d), vp = vsel(Vp, d);
     double Ss = (Vp[4] - Vm[4] + Vm[0] * vm * (Sm - vm) - Vp[0] * vp * (Sp - vp)) /
                 (Vm[0] * (Sm - vm) - Vp[0] * (Sp - vp));
     if (Ss >= 0.0) hllc_star(E, d, Vm, Sm, Ss, F);
@@ -275,10 +283,10 @@ __device__ __forceinline__ void flux_roe(const DevEos& E, int d, double del, con
   for (int p = 0; p < 5; p++)
 #pragma unroll
     for (int q = 0; q < 5; q++) r[p][q] = 0.0;
-  r[0][0] = 1; r[0][1] = uh; r[0][2] = vh; r[0][3] = wh; r[0][4] = Hh - un * c;
-  r[4][0] = 1; r[4][1] = uh; r[4][2] = vh; r[4][3] = wh; r[4][4] = Hh + un * c;
-  r[0][d + 1] = un - c;
-  r[4][d + 1] = un + c;
+  r[0][0] = 1; r[0][4] = Hh - un * c;
+  r[4][0] = 1; r[4][4] = Hh + un * c;
+  r[0][1] = (d == 0) ? un - c : uh; r[0][2] = (d == 1) ? un - c : vh; r[0][3] = (d == 2) ? un - c : wh;
+  r[4][1] = (d == 0) ? un + c : uh; r[4][2] = (d == 1) ? un + c : vh; r[4][3] = (d == 2) ? un + c : wh;
   // the entropy/shear vectors take the slot of the normal direction
   // (FluxFcnBase.h:139-154 and the G/H counterparts)
   if (d == 0) {
@@ -323,7 +331,7 @@ __device__ __forceinline__ void flux_llf(const DevEos& Em, const DevEos& Ep, int
   double ep = eos_e(Ep, Vp[0], Vp[4]);
   double cm2 = eos_c2(Em, Vm[0], em), cp2 = eos_c2(Ep, Vp[0], ep);
   if (cm2 < 0 || cp2 < 0) err |= 1;
-  double x = fabs(Vm[d + 1]) + sqrt(cm2), y = fabs(Vp[d + 1]) + sqrt(cp2);
+  double x = fabs(vsel(Vm, d)) + sqrt(cm2), y = fabs(vsel(Vp, d)) + sqrt(cp2);
   double a = (x < y) ? y : x;  // std::max
   double fm[5], fp[5], Um[5], Up[5];
   flux_phys(Em, d, Vm, fm);

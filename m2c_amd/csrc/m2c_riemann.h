@@ -46,6 +46,26 @@ struct RStats {
   bool trivial, fallback, fatal;
 };
 
+// Out-of-line EOS evaluations.  The solver evaluates e(rho, p), c^2 and the
+// Hugoniot function at dozens of call sites; lanes of a wave that follow
+// different branches walk through all of them, so one shared body per
+// function (instead of an inlined copy per site, each with every EOS kind)
+// keeps the instruction footprint of the kernel small.
+struct EC2 {
+  double e, c2;
+};
+__device__ __noinline__ EC2 ool_ec2(const DevEos* E, double rho, double p) {
+  EC2 r;
+  r.e = eos_e(*E, rho, p);
+  r.c2 = eos_c2(*E, rho, r.e);
+  return r;
+}
+__device__ __noinline__ double ool_e(const DevEos* E, double rho, double p) { return eos_e(*E, rho, p); }
+// HugoniotEquation (ExactRiemannSolverBase.h:66-83)
+__device__ __noinline__ double ool_hugo(const DevEos* E, double r, double ps, double he, double pavg, double oor) {
+  return eos_e(*E, r, ps) - he + pavg * (1.0 / r - oor);
+}
+
 struct DevRiemann {
   const DevMats& M;
   const RiemannCfg& P;
@@ -76,7 +96,7 @@ struct DevRiemann {
     if ((b - a) < 2 * tb_ * a) cc_ = a + (b - a) / 2;                         \
     else if (cc_ <= a + fabs(a) * tb_) cc_ = a + fabs(a) * tb_;              \
     else if (cc_ >= b - fabs(b) * tb_) cc_ = b - fabs(b) * tb_;              \
-    double fc_ = eos_e(E, cc_, ps) - he + pavg * (1.0 / cc_ - oor);           \
+    double fc_ = ool_hugo(&E, cc_, ps, he, pavg, oor);                       \
     if (fc_ == 0) { a = cc_; fa = 0; d = 0; fd = 0; }                         \
     else if (sgn(fa) * sgn(fc_) < 0) { d = b; fd = fb; b = cc_; fb = fc_; }   \
     else { d = a; fd = fa; a = cc_; fa = fc_; }                               \
@@ -87,10 +107,10 @@ struct DevRiemann {
   // TOMS748 of :1422-1621
   __device__ bool shock(const DevEos& E, double rho, double u, double p, double ps, double rhos0, double rhos1,
                         int wave, double& rhos, double& us) {
-    const double he = eos_e(E, rho, p);
+    const double he = ool_e(&E, rho, p);
     const double pavg = 0.5 * (p + ps);
     const double oor = 1.0 / rho;
-#define HUGO(r) (eos_e(E, (r), ps) - he + pavg * (1.0 / (r) - oor))
+#define HUGO(r) ool_hugo(&E, (r), ps, he, pavg, oor)
     double f0, f1;
     double drho = fmax_std(fabs(rhos0 - rhos1), 0.001 * rhos0);
     bool found0 = false, found1 = false;
@@ -263,40 +283,40 @@ struct DevRiemann {
   __device__ bool rk45(const DevEos& E, int wave, double rho_0, double u_0, double p_0, double dp, double& rho,
                        double& u, double& p, double& xi, double& uErr, double& rhoErr) {
     dp = -dp;
-    double e_0 = eos_e(E, rho_0, p_0);
-    double c_0_square = eos_c2(E, rho_0, e_0);
+    const EC2 q_0 = ool_ec2(&E, rho_0, p_0);
+    double c_0_square = q_0.c2;
     if (rho_0 <= 0 || c_0_square < 0) return false;
     double c_0 = sqrt(c_0_square);
     double rho_1 = rho_0 + 0.2 * dp / c_0_square;
     double p_1 = p_0 + 0.2 * dp;
-    double e_1 = eos_e(E, rho_1, p_1);
-    double c_1_square = eos_c2(E, rho_1, e_1);
+    const EC2 q_1 = ool_ec2(&E, rho_1, p_1);
+    double c_1_square = q_1.c2;
     if (rho_1 <= 0 || c_1_square < 0) return false;
     double rho_2 = rho_0 + 3. / 40. * dp / c_0_square + 9. / 40. * dp / c_1_square;
     double p_2 = p_1 + 3. / 10. * dp;  // the reference's abscissa (p0 + dp/2), :1814
-    double e_2 = eos_e(E, rho_2, p_2);
-    double c_2_square = eos_c2(E, rho_2, e_2);
+    const EC2 q_2 = ool_ec2(&E, rho_2, p_2);
+    double c_2_square = q_2.c2;
     if (rho_2 <= 0 || c_2_square < 0) return false;
     double c_2 = sqrt(c_2_square);
     double rho_3 = rho_0 + 3. / 10. * dp / c_0_square - 9. / 10. * dp / c_1_square + 6. / 5. * dp / c_2_square;
     double p_3 = p_0 + 3. / 5. * dp;
-    double e_3 = eos_e(E, rho_3, p_3);
-    double c_3_square = eos_c2(E, rho_3, e_3);
+    const EC2 q_3 = ool_ec2(&E, rho_3, p_3);
+    double c_3_square = q_3.c2;
     if (rho_3 <= 0 || c_3_square < 0) return false;
     double c_3 = sqrt(c_3_square);
     double rho_4 = rho_0 - 11. / 54. * dp / c_0_square + 2.5 * dp / c_1_square - 70. / 27. * dp / c_2_square +
                    35. / 27. * dp / c_3_square;
     double p_4 = p_0 + dp;
-    double e_4 = eos_e(E, rho_4, p_4);
-    double c_4_square = eos_c2(E, rho_4, e_4);
+    const EC2 q_4 = ool_ec2(&E, rho_4, p_4);
+    double c_4_square = q_4.c2;
     if (rho_4 <= 0 || c_4_square < 0) return false;
     double c_4 = sqrt(c_4_square);
     double rho_5 = rho_0 + 1631. / 55296. * dp / c_0_square + 175. / 512. * dp / c_1_square +
                    575. / 13824. * dp / c_2_square + 44275. / 110592. * dp / c_3_square +
                    253. / 4096. * dp / c_4_square;
     double p_5 = p_0 + 7. / 8. * dp;
-    double e_5 = eos_e(E, rho_5, p_5);
-    double c_5_square = eos_c2(E, rho_5, e_5);
+    const EC2 q_5 = ool_ec2(&E, rho_5, p_5);
+    double c_5_square = q_5.c2;
     if (rho_5 <= 0 || c_5_square < 0) return false;
     double c_5 = sqrt(c_5_square);
     double drho = (37. / 378. / c_0_square + 250. / 621. / c_2_square + 125. / 594. / c_3_square +
@@ -313,8 +333,8 @@ struct DevRiemann {
     rho = rho_0 + drho;
     p = p_0 + dp;
     u = (wave == 1) ? u_0 - du : u_0 + du;
-    double e = eos_e(E, rho, p);
-    double c = eos_c2(E, rho, e);
+    const EC2 qe = ool_ec2(&E, rho, p);
+    double c = qe.c2;
     if (rho <= 0 || c < 0) return false;
     c = sqrt(c);
     xi = (wave == 1) ? u - c : u + c;
@@ -325,26 +345,26 @@ struct DevRiemann {
   __device__ bool rk4(const DevEos& E, int wave, double rho_0, double u_0, double p_0, double dp, double& rho,
                       double& u, double& p, double& xi) {
     dp = -dp;
-    double e_0 = eos_e(E, rho_0, p_0);
-    double c_0_square = eos_c2(E, rho_0, e_0);
+    const EC2 q_0 = ool_ec2(&E, rho_0, p_0);
+    double c_0_square = q_0.c2;
     if (rho_0 <= 0 || c_0_square < 0) return false;
     double c_0 = sqrt(c_0_square);
     double rho_1 = rho_0 + 0.5 * dp / c_0_square;
     double p_1 = p_0 + 0.5 * dp;
-    double e_1 = eos_e(E, rho_1, p_1);
-    double c_1_square = eos_c2(E, rho_1, e_1);
+    const EC2 q_1 = ool_ec2(&E, rho_1, p_1);
+    double c_1_square = q_1.c2;
     if (rho_1 <= 0 || c_1_square < 0) return false;
     double c_1 = sqrt(c_1_square);
     double rho_2 = rho_0 + 0.5 * dp / c_1_square;
     double p_2 = p_1;
-    double e_2 = eos_e(E, rho_2, p_2);
-    double c_2_square = eos_c2(E, rho_2, e_2);
+    const EC2 q_2 = ool_ec2(&E, rho_2, p_2);
+    double c_2_square = q_2.c2;
     if (rho_2 <= 0 || c_2_square < 0) return false;
     double c_2 = sqrt(c_2_square);
     double rho_3 = rho_0 + dp / c_2_square;
     double p_3 = p_0 + dp;
-    double e_3 = eos_e(E, rho_3, p_3);
-    double c_3_square = eos_c2(E, rho_3, e_3);
+    const EC2 q_3 = ool_ec2(&E, rho_3, p_3);
+    double c_3_square = q_3.c2;
     if (rho_3 <= 0 || c_3_square < 0) return false;
     double c_3 = sqrt(c_3_square);
     double drho = 1.0 / 6.0 * dp * (1. / c_0_square + 2.0 * (1. / c_1_square + 1. / c_2_square) + 1. / c_3_square);
@@ -353,8 +373,8 @@ struct DevRiemann {
     rho = rho_0 + drho;
     p = p_0 + dp;
     u = (wave == 1) ? u_0 - du : u_0 + du;
-    double e = eos_e(E, rho, p);
-    double c = eos_c2(E, rho, e);
+    const EC2 qe = ool_ec2(&E, rho, p);
+    double c = qe.c2;
     if (rho <= 0 || c < 0) return false;
     c = sqrt(c);
     xi = (wave == 1) ? u - c : u + c;
@@ -362,7 +382,7 @@ struct DevRiemann {
   }
 
   // ComputeRhoUStar (:1213-1420 adaptive, :2523-2690 non-adaptive)
-  __device__ bool rhoustar(int wave, Path& path, double rho, double u, double p, double ps, int id, double rhos0,
+  __device__ __noinline__ bool rhoustar(int wave, Path& path, double rho, double u, double p, double ps, int id, double rhos0,
                            double rhos1, double& rhos, double& us, bool* trans_rare = nullptr,
                            double* Vrare = nullptr, bool count = true) {
     if (count && wave == 1) st.fevals++;
@@ -374,7 +394,8 @@ struct DevRiemann {
     double rhos_0 = rho, us_0 = u, ps_0 = p, xi_0;
     double rhos_1 = rho, us_1 = u, ps_1 = p, xi_1 = 0.0;
     double dp, dp_min_adaption = 0.0, dp_target = 0.0;
-    double e = eos_e(E, rho, p);
+    const EC2 q0 = ool_ec2(&E, rho, p);
+    const double e = q0.e;
     if (!nonadaptive) {
       dp = (p - ps) / nsteps;
       dp_min_adaption = dp / 2.5;
@@ -387,13 +408,25 @@ struct DevRiemann {
       dp = fmin_std(dp_target, drho_target * dpdrho);
     }
     const double pressure_endpoint_tol = P.tol_rare * fmax_std(1.0, fabs(p));
-    double c = eos_c2(E, rho, e);
+    double c = q0.c2;
     if (rho <= 0 || c < 0) return false;
     c = sqrt(c);
     if (path.n > 1) {
+      // The reference scans the cached path backwards for the last point
+      // with pressure above ps (:1243-1251).  Points are only appended below
+      // the last one (:1355), so pressures strictly decrease along the path and the
+      // points above ps form a prefix: a binary search over the prefix gives
+      // the same index with log2(n) dependent loads instead of up to n.
       int index0 = 0;
-      for (int j = path.n - 1; j >= 0; j--)
-        if (path.get(j, 0) > ps) { index0 = j; break; }
+      if (path.get(0, 0) > ps) {
+        int lo = 0, hi = path.n - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (path.get(mid, 0) > ps) lo = mid;
+          else hi = mid - 1;
+        }
+        index0 = lo;
+      }
       ps_0 = path.get(index0, 0);
       rhos_0 = path.get(index0, 1);
       us_0 = path.get(index0, 2);
@@ -483,7 +516,10 @@ struct DevRiemann {
     int found = 0;
     double Cl = rhol * cl, Cr = rhor * cr;
     p0 = (Cr * pl + Cl * pr + Cl * Cr * (ul - ur)) / (Cl + Cr);
-    if (p0g > 0) p0 = p0g;
+    // learned guess: acoustic theory is exact to second order for weak waves
+    // and more precise there than a BF16 network, so the prediction replaces
+    // it only where the two disagree by more than the network's precision
+    if (p0g > 0 && !(p0 > 0 && fabs(log(p0g / p0)) <= 0.02)) p0 = p0g;
     if (!rhoustar(1, path1, rhol, ul, pl, p0, idl, rhol, (p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0)) return found;
     if (!rhoustar(3, path3, rhor, ur, pr, p0, idr, rhor, (p0 > pr) ? rhor * 1.1 : rhor * 0.9, rhor0, ur0)) return found;
     found = 1;
@@ -646,8 +682,7 @@ struct DevRiemann {
       bool star = false;
       if (pl >= p2) {
         const DevEos& E = M.m[idl];
-        double el2 = eos_e(E, rhol2, p2);
-        double cl2 = eos_c2(E, rhol2, el2);
+        double cl2 = ool_ec2(&E, rhol2, p2).c2;
         if (rhol2 <= 0 || cl2 < 0) st.fatal = true;
         cl2 = sqrt(cl2);
         if (u2 - cl2 <= 0) star = true;
@@ -661,8 +696,7 @@ struct DevRiemann {
       bool star = false;
       if (pr >= p2) {
         const DevEos& E = M.m[idr];
-        double er2 = eos_e(E, rhor2, p2);
-        double cr2 = eos_c2(E, rhor2, er2);
+        double cr2 = ool_ec2(&E, rhor2, p2).c2;
         if (rhor2 <= 0 || cr2 < 0) st.fatal = true;
         cr2 = sqrt(cr2);
         if (u2 - cr2 >= 0) star = true;
@@ -705,12 +739,10 @@ struct DevRiemann {
     double pr = Vp[4];
     path1.reset(pl, rhol, ul);
     path3.reset(pr, rhor, ur);
-    double el = eos_e(M.m[idl], rhol, pl);
-    double cl = eos_c2(M.m[idl], rhol, el);
+    double cl = ool_ec2(&M.m[idl], rhol, pl).c2;
     if (rhol <= 0 || cl < 0) { st.fatal = true; return 1; }
     cl = sqrt(cl);
-    double er = eos_e(M.m[idr], rhor, pr);
-    double cr = eos_c2(M.m[idr], rhor, er);
+    double cr = ool_ec2(&M.m[idr], rhor, pr).c2;
     if (rhor <= 0 || cr < 0) { st.fatal = true; return 1; }
     cr = sqrt(cr);
     double p0(DBL_MIN), ul0(0.0), ur0(0.0), rhol0(DBL_MIN), rhor0(DBL_MIN);
@@ -797,7 +829,7 @@ struct DevRiemann {
     return 0;
   }
 
-  __device__ int solve(const double* dir, const double* Vm, int idl, const double* Vp, int idr, double* Vs, int& id,
+  __device__ __noinline__ int solve(const double* dir, const double* Vm, int idl, const double* Vp, int idr, double* Vs, int& id,
                        double* Vsm, double* Vsp, double p0g = -1.0) {
     nonadaptive = false;
     bool again;

@@ -59,12 +59,28 @@ def lib():
         L.m2cx_profile.argtypes = [P, C.c_int]
         L.m2cx_kernel_time.argtypes = [P, C.c_char_p, D, C.POINTER(C.c_int64)]
         L.m2cx_synchronize.argtypes = [P]
+        F = C.POINTER(C.c_float)
+        L.m2cx_mlp_set_params.argtypes = [P, F]
+        L.m2cx_mlp_get_params.argtypes = [P, F]
+        L.m2cx_mlp_features.argtypes = [P, C.c_int, D, D, I32, D, I32, D, F, F, I32]
+        L.m2cx_mlp_predict.argtypes = [P, C.c_int, F, F]
+        L.m2cx_mlp_loss_grad.argtypes = [P, C.c_int, F, F, F, F]
+        L.m2cx_mlp_train.argtypes = [P, C.c_int, F, F, C.c_int, C.c_int, C.c_float, C.c_uint32, F]
+        L.m2cx_ensemble_create.argtypes = [C.POINTER(P), C.c_int, C.POINTER(P)]
+        L.m2cx_ensemble_run_steps.argtypes = [P, C.c_int]
+        L.m2cx_ensemble_last_error.restype = C.c_char_p
+        L.m2cx_ensemble_last_error.argtypes = [P]
+        L.m2cx_ensemble_destroy.argtypes = [P]
         _lib = L
     return _lib
 
 
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
 def _d(a):
-    return a.ctypes.data_as(D)
+    return a.ctypes.data_as(D) if a is not None else None
 
 
 def _i(a):
@@ -194,6 +210,52 @@ class Solver:
                                               _i(mid), _d(Vsm), _d(Vsp), _i(err), _i(it), _i(fe)))
         return dict(Vs=Vs, id=mid, Vsm=Vsm, Vsp=Vsp, err=err, iters=it, fevals=fe)
 
+    # ---- learned p* guess ----
+    NPARAM = 8961
+
+    def mlp_set_params(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        assert theta.size == self.NPARAM
+        self._check(self.L.m2cx_mlp_set_params(self.h, _f(theta)))
+
+    def mlp_get_params(self):
+        out = np.zeros(self.NPARAM, dtype=np.float32)
+        self._check(self.L.m2cx_mlp_get_params(self.h, _f(out)))
+        return out
+
+    def mlp_features(self, dirs, Vm, idm, Vp, idp, pstar=None):
+        n = len(Vm)
+        dirs, Vm, Vp = (np.ascontiguousarray(x, dtype=np.float64) for x in (dirs, Vm, Vp))
+        idm, idp = (np.ascontiguousarray(x, dtype=np.int32) for x in (idm, idp))
+        X = np.zeros((n, 8), dtype=np.float32)
+        T = np.zeros(n, dtype=np.float32)
+        ok = np.zeros(n, dtype=np.int32)
+        ps = None if pstar is None else np.ascontiguousarray(pstar, dtype=np.float64)
+        self._check(self.L.m2cx_mlp_features(self.h, n, _d(dirs), _d(Vm), _i(idm), _d(Vp), _i(idp), _d(ps), _f(X),
+                                             _f(T), _i(ok)))
+        return X, (T if pstar is not None else None), ok.astype(bool)
+
+    def mlp_predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(len(X), dtype=np.float32)
+        self._check(self.L.m2cx_mlp_predict(self.h, len(X), _f(X), _f(y)))
+        return y
+
+    def mlp_loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.L.m2cx_mlp_loss_grad(self.h, len(X), _f(X), _f(T), _f(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_train(self, X, T, epochs, batch=4096, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        self._check(self.L.m2cx_mlp_train(self.h, len(X), _f(X), _f(T), epochs, batch, lr, seed, _f(hist)))
+        return hist
+
     # ---- measurement ----
     def profile(self, on=True):
         self._check(self.L.m2cx_profile(self.h, int(on)))
@@ -202,3 +264,38 @@ class Solver:
         ms, n = C.c_double(), C.c_int64()
         self._check(self.L.m2cx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Ensemble:
+    """Independent solvers advanced in lockstep, one Riemann launch per RK
+    stage for all of them, one captured HIP graph per time step."""
+
+    def __init__(self, solvers):
+        self.L = lib()
+        self.members = list(solvers)
+        arr = (C.c_void_p * len(self.members))(*[s.h for s in self.members])
+        h = C.c_void_p()
+        rc = self.L.m2cx_ensemble_create(arr, len(self.members), C.byref(h))
+        self.h = h
+        if rc < 0:
+            msg = self.L.m2cx_ensemble_last_error(self.h).decode() if self.h else ""
+            self.close()
+            raise M2CError("m2cx_ensemble_create error %d: %s" % (rc, msg))
+
+    def run_steps(self, n):
+        rc = self.L.m2cx_ensemble_run_steps(self.h, n)
+        if rc < 0:
+            raise M2CError("m2cx_ensemble_run_steps error %d: %s"
+                           % (rc, self.L.m2cx_ensemble_last_error(self.h).decode()))
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.m2cx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
