@@ -3,12 +3,20 @@
 
 Metric: FP64 cell-updates per second per RK stage (interior cells x RK stages
 x time steps / wall time of the stepping loop), plus Riemann main-loop
-iterations per interface face.  One "step" is one time step of the driver loop
+iterations per interface face, for the plain (acoustic) and the learned
+initial guess.  One "step" is one time step of the driver loop
 (Main.cpp:653-712): the CFL time step, all RK stages (interface Riemann
 solves, fused residual/RK kernels, level-set stages) and the end-of-step
 material update, with the state resident in HBM.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+Workloads (--config):
+  c2          256^3 two-material shock-bubble, RK3 (default; the headline)
+  c1          128x32x32 Sod tube, RK2
+  c2_single   the c2 grid with water only (single-material residual)
+  c4          uneven ensemble of shock-bubble runs (edges 64..192, --instances)
+              advanced in lockstep with one captured graph per step
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
 Prints one JSON line.  Only one GPU is used; --gpus N > 1 exits.
 """
 import argparse
@@ -26,6 +34,8 @@ sys.path.insert(0, ROOT)
 from m2c_amd import abi, configs  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+METRIC = "FP64 cell-updates/s per RK stage + Riemann Newton iters/face, learned vs plain"
+GUESS_PARAMS = os.path.join(ROOT, "m2c_amd", "data", "guess_params.npy")
 
 
 def stage_bytes(integrator, multimat):
@@ -36,12 +46,10 @@ def stage_bytes(integrator, multimat):
     when more than one material is present."""
     idb = 1 if multimat else 0
     if integrator == abi.TI_RK3:
-        per = [160 + idb, 200 + idb, 160 + idb]
-    elif integrator == abi.TI_RK2:
-        per = [160 + idb, 160 + idb]
-    else:
-        per = [120 + idb]
-    return per
+        return [160 + idb, 200 + idb, 160 + idb]
+    if integrator == abi.TI_RK2:
+        return [160 + idb, 160 + idb]
+    return [120 + idb]
 
 
 def build_config(name):
@@ -127,13 +135,54 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}))
 
 
+def riemann_stats(cnt):
+    calls = max(cnt["riemann_calls"], 1)
+    return {"faces": cnt["riemann_calls"], "trivial": cnt["riemann_trivial"],
+            "main_iters_per_face": cnt["riemann_main_iters"] / calls,
+            "fevals_per_face": cnt["riemann_fevals"] / calls,
+            "errors": cnt["riemann_errors"], "guess_fallbacks": cnt["riemann_guess_fallbacks"]}
+
+
+def run_single(cfg, steps, warmup, learned_params=None):
+    """Time `steps` steps of one solver; returns (result dict, solver)."""
+    from m2c_amd import runtime
+    (g, mats, nmat, s, V, ID, Phi) = cfg
+    if learned_params is not None:
+        s.riem_guess = abi.GUESS_MLP
+    solver = runtime.Solver(g, mats, nmat, s)
+    if learned_params is not None:
+        solver.mlp_set_params(learned_params)
+    solver.set_state(V, ID, Phi)
+    stages = {abi.TI_FE: 1, abi.TI_RK2: 2, abi.TI_RK3: 3}[s.integrator]
+    cells = g.nx * g.ny * g.nz
+    solver.run_steps(warmup)
+    solver.synchronize()
+    solver.reset_counters()
+    solver.profile(True)
+    t0 = time.perf_counter()
+    solver.run_steps(steps)
+    solver.synchronize()
+    el = time.perf_counter() - t0
+    cnt = solver.counters()
+    stage_ms, stage_n = solver.kernel_time("stage")
+    riem_ms, _ = solver.kernel_time("riemann")
+    mlp_ms, _ = solver.kernel_time("mlp")
+    res = {"value": cells * stages * steps / el, "ms_per_step": el * 1e3 / steps, "cells": cells,
+           "stages": stages, "stage_ms": stage_ms, "stage_launches": stage_n,
+           "riemann": dict(riemann_stats(cnt), riemann_ms_per_step=riem_ms / steps, mlp_ms_per_step=mlp_ms / steps)}
+    solver.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=os.environ.get("M2C_BENCH_CONFIG", "c2"))
+    ap.add_argument("--instances", type=int, default=64, help="ensemble size for --config c4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -144,60 +193,96 @@ def main():
         print("bench.py measures one GPU only: this build does not run the step across GPUs "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    from m2c_amd import runtime
+    if args.config == "c4":
+        return bench_ensemble(args)
     steps = args.steps if args.steps is not None else (100 if args.config == "c1" else 20)
-    (g, mats, nmat, s, V, ID, Phi), desc = build_config(args.config)
+    cfg, desc = build_config(args.config)
+    g, mats, nmat, s = cfg[:4]
     print("# bench: config=%s cells=%d steps=%d warmup=%d" % (args.config, g.nx * g.ny * g.nz, steps, args.warmup),
           file=sys.stderr, flush=True)
-    solver = runtime.Solver(g, mats, nmat, s)
-    solver.set_state(V, ID, Phi)
-    del V
-    stages = {abi.TI_FE: 1, abi.TI_RK2: 2, abi.TI_RK3: 3}[s.integrator]
-    cells = g.nx * g.ny * g.nz
-    solver.run_steps(args.warmup)
-    solver.synchronize()
-    solver.reset_counters()
-    solver.profile(True)
-    t0 = time.perf_counter()
-    solver.run_steps(steps)
-    solver.synchronize()
-    el = time.perf_counter() - t0
-    cnt = solver.counters()
-    stage_ms, stage_n = solver.kernel_time("stage")
-    riem_ms, riem_n = solver.kernel_time("riemann")
+    plain = run_single(cfg, steps, args.warmup)
     per = stage_bytes(s.integrator, nmat > 1)
+    cells = plain["cells"]
     avg_bytes = cells * sum(per) / len(per)
-    avg_stage_ms = stage_ms / max(stage_n, 1)
+    avg_stage_ms = plain["stage_ms"] / max(plain["stage_launches"], 1)
     achieved = avg_bytes / (avg_stage_ms * 1e-3) / 1e9
-    value = cells * stages * steps / el
-    iters_per_face = cnt["riemann_main_iters"] / max(cnt["riemann_calls"], 1)
-    fevals_per_face = cnt["riemann_fevals"] / max(cnt["riemann_calls"], 1)
     out = {
-        "metric": "FP64 cell-updates/s per RK stage + Riemann Newton iters/face, learned vs plain",
-        "value": value,
+        "metric": METRIC,
+        "value": plain["value"],
         "unit": "cell-stage-updates/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / steps,
+        "ms_per_step": plain["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.config, "description": desc, "cells": cells, "rk_stages": stages,
+        "config": {"workload": args.config, "description": desc, "cells": cells, "rk_stages": plain["stages"],
                    "riemann_guess": "acoustic (plain)"},
-        "riemann": {"faces": cnt["riemann_calls"], "trivial": cnt["riemann_trivial"],
-                    "main_iters_per_face": iters_per_face, "fevals_per_face": fevals_per_face,
-                    "errors": cnt["riemann_errors"], "riemann_ms_per_step": riem_ms / steps},
+        "riemann": plain["riemann"],
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused residual + RK stage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(args.config),
                      "bytes_per_cell_per_launch": per, "avg_launch_ms": avg_stage_ms},
     }
+    if nmat > 1 and not args.no_learned and os.path.exists(GUESS_PARAMS):
+        theta = np.load(GUESS_PARAMS)
+        cfg2, _ = build_config(args.config)
+        lr = run_single(cfg2, steps, args.warmup, learned_params=theta)
+        out["learned"] = {"value": lr["value"], "ms_per_step": lr["ms_per_step"], "riemann": lr["riemann"],
+                          "speedup_vs_plain": lr["value"] / plain["value"], "params": "m2c_amd/data/guess_params.npy"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config)
-    solver.close()
+    print(json.dumps(out))
+
+
+def bench_ensemble(args):
+    """config 4: uneven shock-bubble ensemble, lockstep with graph replay."""
+    from m2c_amd import runtime
+    steps = args.steps if args.steps is not None else 20
+    cases = configs.c4_instances(count=args.instances, seed=4)
+    print("# bench: ensemble of %d, cells %d, steps %d" % (len(cases), sum(c[0][0] ** 3 for c in cases), steps),
+          file=sys.stderr, flush=True)
+    learned = (not args.no_learned) and os.path.exists(GUESS_PARAMS)
+    theta = np.load(GUESS_PARAMS) if learned else None
+    members, cells = [], 0
+    for (n, r, m), (g, mats, nmat, s, V, ID, Phi) in cases:
+        if learned:
+            s.riem_guess = abi.GUESS_MLP
+        sol = runtime.Solver(g, mats, nmat, s)
+        if learned:
+            sol.mlp_set_params(theta)
+        sol.set_state(V, ID, Phi)
+        del V, ID, Phi
+        members.append(sol)
+        cells += g.nx * g.ny * g.nz
+    ens = runtime.Ensemble(members)
+    ens.run_steps(args.warmup)
+    for sol in members:
+        sol.reset_counters()
+    t0 = time.perf_counter()
+    ens.run_steps(steps)
+    el = time.perf_counter() - t0
+    calls = iters = fev = 0
+    for sol in members:
+        c = sol.counters()
+        calls += c["riemann_calls"]
+        iters += c["riemann_main_iters"]
+        fev += c["riemann_fevals"]
+    out = {"metric": METRIC, "value": cells * 3 * steps / el, "unit": "cell-stage-updates/s", "n_gpus": 1,
+           "steps": steps, "warmup": args.warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "c4", "instances": len(cases), "cells": cells, "rk_stages": 3,
+                      "riemann_guess": "learned (MLP)" if learned else "acoustic (plain)",
+                      "description": "uneven ensemble of shock-bubble runs (edges 64..192, r 0.05-0.25 L, "
+                                     "Mach 1.2-3.0), lockstep, one graph per step"},
+           "riemann": {"faces": calls, "main_iters_per_face": iters / max(calls, 1),
+                       "fevals_per_face": fev / max(calls, 1)}}
+    ens.close()
+    for sol in members:
+        sol.close()
     print(json.dumps(out))
 
 

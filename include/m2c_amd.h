@@ -188,6 +188,29 @@ int m2cx_download(m2cx_solver* h, int field, void* host);
 int m2cx_set_time(m2cx_solver* h, double t, int time_step);
 int m2cx_get_time(m2cx_solver* h, double* t, double* dt, int* time_step);
 
+/* Initial condition from geometric entities, on the device: the default
+ * state and ID everywhere, then each geometry in order overrides the
+ * interior cells it covers (SpaceInitializer::SetInitialCondition,
+ * SpaceInitializer.cpp:237-482); the level set of scheme.ls_materialid starts
+ * at the domain diagonal and takes the signed distance of the nearest
+ * surface of that material's geometries, negative inside
+ * (InitializePhiByDistance, :1390-1514); then the boundary fill.  Plane:
+ * cells with (x - point).n > 0; sphere: |x - point| - radius < 0 (side 0,
+ * interior) or > 0 (side 1). */
+enum { M2CX_GEOM_PLANE = 0, M2CX_GEOM_SPHERE = 3 };
+typedef struct {
+  int32_t kind;        /* M2CX_GEOM_*                          */
+  int32_t side;        /* sphere: 0 interior, 1 exterior        */
+  int32_t materialid;
+  int32_t reserved;
+  double point[3];     /* plane point / sphere centre           */
+  double normal[3];    /* plane normal (normalised here)        */
+  double radius;       /* sphere radius                         */
+  double state[5];     /* rho, u, v, w, p                       */
+} m2cx_geometry;
+int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id, const m2cx_geometry* geoms,
+                    int ngeom);
+
 /* ------------------------------------------------------------------ */
 /* SpaceOperator (SpaceOperator.h:107-149)                             */
 /* ------------------------------------------------------------------ */

@@ -18,6 +18,7 @@ MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
+GEOM_PLANE, GEOM_SPHERE = 0, 3
 OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY = \
     0, -1, -2, -3, -4, -5, -6, -7, -8
 
@@ -135,3 +136,24 @@ def default_scheme(**kw):
     for k, v in kw.items():
         setattr(s, k, v)
     return s
+
+
+class Geometry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("side", C.c_int32), ("materialid", C.c_int32), ("reserved", C.c_int32),
+                ("point", C.c_double * 3), ("normal", C.c_double * 3), ("radius", C.c_double),
+                ("state", C.c_double * 5)]
+
+
+def geometry(kind, point, normal=(1.0, 0.0, 0.0), radius=0.0, materialid=0, state=(1.0, 0.0, 0.0, 0.0, 1.0),
+             side=0):
+    """A plane (cells with (x - point).normal > 0) or a sphere (side 0:
+    interior) carrying an initial state and material ID."""
+    gm = Geometry()
+    gm.kind = {"plane": GEOM_PLANE, "sphere": GEOM_SPHERE}[kind]
+    gm.side = side
+    gm.materialid = materialid
+    gm.point[:] = list(point)
+    gm.normal[:] = list(normal)
+    gm.radius = radius
+    gm.state[:] = list(state)
+    return gm

@@ -73,6 +73,7 @@ def c1_sod(n=(128, 32, 32)):
 
 WATER_SG = dict(gamma=6.12, pc=3.43e8)
 AIR_SG = dict(gamma=1.4, pc=0.0)
+WATER_NASG = dict(gamma=1.19, pc=6.2178e8, b=6.7212e-4, q=-1.177788e6)
 
 
 def sg_post_shock(rho1, p1, gam, pc, mach):
@@ -131,3 +132,82 @@ def c4_instances(count=64, seed=4, edges=(64, 96, 128, 160, 192)):
         m = float(rng.uniform(1.2, 3.0))
         out.append(((n, r, m), c2_shock_bubble(n, radius=r, mach=m)))
     return out
+
+
+def nasg_post_shock(rho1, p1, gam, pinf, b, q, mach):
+    """Rankine-Hugoniot state behind a right-moving shock of Mach `mach`
+    entering quiescent Noble-Abel stiffened gas (e = (p + gam pinf)(v - b) /
+    (gam - 1) + q, VarFcnNASG.h:55-86): mass and momentum give p2 and v2 as
+    functions of the particle velocity u2, and the Hugoniot energy relation
+    e2 - e1 = (p1 + p2)(v1 - v2)/2 fixes u2 (one root in (0, Us(1 - b/v1)))."""
+    from scipy.optimize import brentq
+    v1 = 1.0 / rho1
+    e1 = (p1 + gam * pinf) * (v1 - b) / (gam - 1) + q
+    c1 = math.sqrt(gam * (p1 + pinf) * v1 * v1 / (v1 - b))
+    us = mach * c1
+
+    def hug(u2):
+        p2 = p1 + rho1 * us * u2
+        v2 = v1 * (us - u2) / us
+        e2 = (p2 + gam * pinf) * (v2 - b) / (gam - 1) + q
+        return e2 - e1 - 0.5 * (p1 + p2) * (v1 - v2)
+
+    hi = us * (1.0 - b / v1) * (1.0 - 1e-12)
+    lo = 1e-6 * us
+    u2 = brentq(hug, lo, hi, xtol=1e-15 * us, rtol=1e-15, maxiter=500)
+    p2 = p1 + rho1 * us * u2
+    rho2 = 1.0 / (v1 * (us - u2) / us)
+    return rho2, u2, p2
+
+
+def shock_bubble_geometries(L, radius, mach, shock_x, water, rho_w=1000.0, p_w=1.0e5, centre=None):
+    """The shock-bubble initial condition as geometric entities for
+    Solver.initialize: default quiescent water (material 0), a plane behind
+    which the post-shock state holds (x < shock_x), an air sphere (material
+    1, the level-set material)."""
+    if water.get("b", 0.0) or water.get("q", 0.0):
+        rho2, u2, p2 = nasg_post_shock(rho_w, p_w, water["gamma"], water["pc"], water.get("b", 0.0),
+                                       water.get("q", 0.0), mach)
+    else:
+        rho2, u2, p2 = sg_post_shock(rho_w, p_w, water["gamma"], water["pc"], mach)
+    c = centre if centre is not None else (0.5 * L, 0.5 * L, 0.5 * L)
+    plane = abi.geometry("plane", (shock_x, 0.0, 0.0), normal=(-1.0, 0.0, 0.0), materialid=0,
+                         state=(rho2, u2, 0.0, 0.0, p2))
+    sphere = abi.geometry("sphere", c, radius=radius, materialid=1, state=(1.2, 0.0, 0.0, 0.0, p_w))
+    return (rho_w, 0.0, 0.0, 0.0, p_w), 0, [plane, sphere]
+
+
+def c5_bubble_cloud(n=(512, 256, 256), nbubbles=200, seed=5, mach=2.0, shock_x=0.1, r_range=(0.005, 0.03),
+                    gap=0.002):
+    """Bubble cloud: NASG water (gamma 1.19, p_inf 6.2178e8, b 6.7212e-4,
+    q -1.177788e6) on [0, 2L] x [0, L] x [0, L] with `nbubbles` non-overlapping
+    ideal-gas bubbles (radius log-uniform in r_range x L, seeded centres by
+    rejection, all downstream of the shock), hit by a planar Mach-2 shock.
+    HLLC bulk, exact interface Riemann, RK3.  Returns (grid, mats, nmat,
+    scheme, default_state, default_id, geometries) for Solver.initialize."""
+    L = 1.0
+    lx = L * n[0] / n[2]
+    g = abi.grid(n, (0.0, 0.0, 0.0), (lx, L, L), (abi.BC_OUTLET,) * 6)
+    mats = abi.materials(abi.material("NASG", **WATER_NASG), abi.material("SG", **AIR_SG))
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK3, cfl=0.4,
+                           n_levelsets=1, ls_materialid=1, riem_tol_main=1e-10)
+    rng = np.random.default_rng(seed)
+    placed = []
+    tries = 0
+    while len(placed) < nbubbles and tries < 200000:
+        tries += 1
+        r = L * 10 ** rng.uniform(np.log10(r_range[0]), np.log10(r_range[1]))
+        cx = rng.uniform(shock_x * lx + r + gap, lx - r - gap)
+        cy = rng.uniform(r + gap, L - r - gap)
+        cz = rng.uniform(r + gap, L - r - gap)
+        ok = all((cx - x) ** 2 + (cy - y) ** 2 + (cz - z) ** 2 > (r + rr + gap) ** 2 for x, y, z, rr in placed)
+        if ok:
+            placed.append((cx, cy, cz, r))
+    rho_w, p_w = 1000.0, 1.0e5
+    rho2, u2, p2 = nasg_post_shock(rho_w, p_w, WATER_NASG["gamma"], WATER_NASG["pc"], WATER_NASG["b"],
+                                   WATER_NASG["q"], mach)
+    geoms = [abi.geometry("plane", (shock_x * lx, 0.0, 0.0), normal=(-1.0, 0.0, 0.0), materialid=0,
+                          state=(rho2, u2, 0.0, 0.0, p2))]
+    for cx, cy, cz, r in placed:
+        geoms.append(abi.geometry("sphere", (cx, cy, cz), radius=r, materialid=1, state=(1.2, 0.0, 0.0, 0.0, p_w)))
+    return g, mats, 2, s, (rho_w, 0.0, 0.0, 0.0, p_w), 0, geoms

@@ -1013,6 +1013,64 @@ int m2cx_ensemble_destroy(m2cx_ensemble* e) {
   return 0;
 }
 
+int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id, const m2cx_geometry* geoms,
+                    int ngeom) {
+  if (!h || !default_state || ngeom < 0 || (ngeom > 0 && !geoms)) return M2CX_E_ARG;
+  if (default_id < 0 || default_id > h->s.nmat) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  std::vector<InitGeom> G(ngeom);
+  for (int m = 0; m < ngeom; m++) {
+    const m2cx_geometry& a = geoms[m];
+    if ((a.kind != M2CX_GEOM_PLANE && a.kind != M2CX_GEOM_SPHERE) || a.materialid < 0 || a.materialid > h->s.nmat)
+      return fail(h, M2CX_E_ARG, "unsupported geometry");
+    InitGeom& b = G[m];
+    b.kind = a.kind;
+    b.interior = a.side == 0;
+    b.id = a.materialid;
+    for (int d = 0; d < 3; d++) b.x0[d] = a.point[d];
+    if (a.kind == M2CX_GEOM_PLANE) {
+      // GeoTools::DistanceFromPointToPlane normalises the direction with
+      // Vec3D::operator/=, a multiplication by the reciprocal (Vector3D.h:193-200)
+      const double nn = std::sqrt(a.normal[0] * a.normal[0] + a.normal[1] * a.normal[1] + a.normal[2] * a.normal[2]);
+      if (!(nn > 0)) return fail(h, M2CX_E_ARG, "plane normal is zero");
+      const double inv = 1.0 / nn;
+      for (int d = 0; d < 3; d++) b.n[d] = a.normal[d] * inv;
+    }
+    b.R = a.radius;
+    for (int p = 0; p < 5; p++) b.V[p] = a.state[p];
+  }
+  InitGeom* dG = nullptr;
+  double* dV0 = nullptr;
+  if (ngeom) {
+    HIPCHK(h, hipMalloc(&dG, ngeom * sizeof(InitGeom)));
+    HIPCHK(h, hipMemcpy(dG, G.data(), ngeom * sizeof(InitGeom), hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipMalloc(&dV0, 40));
+  HIPCHK(h, hipMemcpy(dV0, default_state, 40, hipMemcpyHostToDevice));
+  const m2cx_grid& gr = h->grid;
+  const double diag = std::sqrt((gr.xmax - gr.x0) * (gr.xmax - gr.x0) + (gr.ymax - gr.y0) * (gr.ymax - gr.y0) +
+                                (gr.zmax - gr.z0) * (gr.zmax - gr.z0));
+  double* phi = (h->multimat && h->sch.n_levelsets > 0) ? h->mp.phi : nullptr;
+  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  k_initialize<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, f5(h, h->V[0]), h->ID, phi, dV0, default_id, dG, ngeom,
+                                                        h->sch.ls_materialid, diag);
+  // boundary fill: state, ghost IDs, level set (ApplyBoundaryConditions,
+  // UpdateMaterialIDAtGhostNodes, LevelSetOperator::ApplyBoundaryConditions)
+  k_apply_bc<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, f5(h, h->V[0]));
+  const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
+  k_ghost_ids<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, h->ID);
+  if (phi) {
+    const MPBuffers& B = h->mp;
+    k_ls_bc<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, phi, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3],
+                                                    B.lsbc[4], B.lsbc[5]);
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  hipFree(dG);
+  hipFree(dV0);
+  return 0;
+}
+
 int m2cx_profile(m2cx_solver* h, int enable) {
   h->profile = enable != 0;
   return 0;

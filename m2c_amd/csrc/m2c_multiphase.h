@@ -899,6 +899,60 @@ __global__ void k_post_update(UpdArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Initial condition from geometric entities (SpaceInitializer.cpp:237-482 for
+// the state and material ID, :1390-1514 for the level set): the default state
+// everywhere, then every geometry in order overrides the interior cells it
+// covers; phi starts at the domain diagonal and takes the signed distance of
+// the nearest surface of the level-set material's geometries.
+// ---------------------------------------------------------------------------
+struct InitGeom {
+  int kind, interior, id, pad;  // kind 0 plane, 3 sphere
+  double x0[3], n[3], R;        // plane: point and unit normal; sphere: centre and radius
+  double V[5];
+};
+
+__global__ void k_initialize(DevGrid g, F5 V, unsigned char* ID, double* phi, const double* __restrict__ V0,
+                             int id0, const InitGeom* __restrict__ geo, int ngeo, int lsid, double diag) {
+  const long n = (long)g.NX * g.NY * g.NZ;
+  const double* gx = g_axis(g, 0);
+  const double* gy = g_axis(g, 1);
+  const double* gz = g_axis(g, 2);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
+    const long c = gidx(g, i, j, k);
+    const double x = gx[i + 1], y = gy[j + 1], z = gz[k + 1];
+    double v[5] = {V0[0], V0[1], V0[2], V0[3], V0[4]};
+    int id = id0;
+    double ph = diag;
+    for (int m = 0; m < ngeo; m++) {
+      const InitGeom& G = geo[m];
+      double dist;
+      bool inside;
+      if (G.kind == 0) {
+        dist = (x - G.x0[0]) * G.n[0] + (y - G.x0[1]) * G.n[1] + (z - G.x0[2]) * G.n[2];
+        inside = dist > 0;
+      } else {
+        const double a = x - G.x0[0], b = y - G.x0[1], e = z - G.x0[2];
+        dist = sqrt(a * a + b * b + e * e) - G.R;
+        inside = G.interior ? dist < 0 : dist > 0;
+      }
+      if (inside) {
+        for (int p = 0; p < 5; p++) v[p] = G.V[p];
+        id = G.id;
+      }
+      if (phi && G.id == lsid && fabs(dist) < fabs(ph)) {
+        if (G.kind == 0) ph = -dist;
+        else ph = G.interior ? dist : -dist;
+      }
+    }
+    store5(V, c, v);
+    ID[c] = (unsigned char)id;
+    if (phi) phi[c] = ph;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side orchestration
 // ---------------------------------------------------------------------------
 static inline int blocks_for(long n, int bs = 256) { return (int)std::min<long>((n + bs - 1) / bs, 256L * 64L); }

@@ -66,6 +66,7 @@ def lib():
         L.m2cx_mlp_predict.argtypes = [P, C.c_int, F, F]
         L.m2cx_mlp_loss_grad.argtypes = [P, C.c_int, F, F, F, F]
         L.m2cx_mlp_train.argtypes = [P, C.c_int, F, F, C.c_int, C.c_int, C.c_float, C.c_uint32, F]
+        L.m2cx_initialize.argtypes = [P, D, C.c_int, C.POINTER(abi.Geometry), C.c_int]
         L.m2cx_ensemble_create.argtypes = [C.POINTER(P), C.c_int, C.POINTER(P)]
         L.m2cx_ensemble_run_steps.argtypes = [P, C.c_int]
         L.m2cx_ensemble_last_error.restype = C.c_char_p
@@ -209,6 +210,13 @@ class Solver:
         self._check(self.L.m2cx_riemann_batch(self.h, n, _d(dirs), _d(Vm), _i(idm), _d(Vp), _i(idp), _d(Vs),
                                               _i(mid), _d(Vsm), _d(Vsp), _i(err), _i(it), _i(fe)))
         return dict(Vs=Vs, id=mid, Vsm=Vsm, Vsp=Vsp, err=err, iters=it, fevals=fe)
+
+    def initialize(self, default_state, default_id, geoms):
+        """Initial condition from planes and spheres on the device, then the
+        boundary fill (SpaceInitializer rules)."""
+        st = np.ascontiguousarray(default_state, dtype=np.float64)
+        arr = (abi.Geometry * max(len(geoms), 1))(*geoms)
+        self._check(self.L.m2cx_initialize(self.h, _d(st), default_id, arr, len(geoms)))
 
     # ---- learned p* guess ----
     NPARAM = 8961

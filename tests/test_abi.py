@@ -53,7 +53,7 @@ def test_argument_errors_need_no_gpu():
 
 
 _PREFIXES = ("EOS_", "BC_", "LSBC_", "FLUX_", "REC_", "LIM_", "SLOPE_", "TI_", "MPFLUX_", "MPREC_", "GUESS_",
-             "FIELD_", "E_")
+             "FIELD_", "E_", "GEOM_")
 
 
 def _consts():
@@ -69,7 +69,7 @@ def _c_layout(tmp_path):
     as the C compiler sees them."""
     checks = []
     for cname, py in (("m2cx_material", abi.Material), ("m2cx_grid", abi.Grid), ("m2cx_scheme", abi.Scheme),
-                      ("m2cx_counters", abi.Counters)):
+                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry)):
         checks.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             checks.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
@@ -87,7 +87,7 @@ def _c_layout(tmp_path):
 def test_ctypes_mirror_matches_header(tmp_path):
     got = _c_layout(tmp_path)
     for cname, py in (("m2cx_material", abi.Material), ("m2cx_grid", abi.Grid), ("m2cx_scheme", abi.Scheme),
-                      ("m2cx_counters", abi.Counters)):
+                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry)):
         assert got[f"{cname} size"] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[f"{cname}.{f}"] == getattr(py, f).offset, (cname, f)
