@@ -321,6 +321,17 @@ int m2cx_mlp_loss_grad(m2cx_solver* h, int n, const float* X, const float* T, fl
 int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int epochs, int batch, float lr,
                    uint32_t seed, float* loss_per_epoch);
 
+/* Stage-kernel tile shapes (no counterpart in the reference): variant 0
+ * 16x4x4, 1 32x4x2, 2 8x8x4, 3 16x8x2 cells per work-group.  The first
+ * m2cx_run_steps (and m2cx_ensemble_create) times the residual launch of each
+ * variant on the current state and keeps the fastest, cached per grid shape
+ * and scheme for the process (grids below 64^3 use variant 0).  Results do
+ * not depend on the variant.  m2cx_autotune re-times (ms_per_variant: 4
+ * values, may be NULL); m2cx_set_tile forces a variant. */
+#define M2CX_NUM_TILES 4
+int m2cx_autotune(m2cx_solver* h, int* variant, float* ms_per_variant);
+int m2cx_set_tile(m2cx_solver* h, int variant);
+
 /* ------------------------------------------------------------------ */
 /* Ensembles                                                           */
 /* ------------------------------------------------------------------ */

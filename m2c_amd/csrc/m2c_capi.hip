@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 #include "m2c_kernels.h"
@@ -52,6 +53,7 @@ struct m2cx_solver {
   MPBuffers mp{};
   bool multimat = false;
   hipStream_t stream_saved = nullptr;  // own stream while an ensemble borrows it
+  int tile = -1;                        // stage tile variant (-1: not tuned yet)
   // learned p* guess
   struct {
     bool ready = false;
@@ -365,25 +367,100 @@ static StageArgs stage_args(m2cx_solver* h, int vin, int vout, int mode) {
   return a;
 }
 
-static void launch_stage(m2cx_solver* h, int vin, int vout, int mode) {
-  StageArgs a = stage_args(h, vin, vout, mode);
+extern "C++" {
+// Stage-kernel tile shapes the autotuner chooses from (all 256 threads).
+struct TileVariant {
+  int bx, by, bz;
+};
+static const TileVariant kTiles[] = {{16, 4, 4}, {32, 4, 2}, {8, 8, 4}, {16, 8, 2}};
+static constexpr int kNumTiles = 4;
+
+template <int BX, int BY, int BZ>
+static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode) {
   // one work-group per tile, padded to a multiple of the XCD count (see
   // k_stage for the tile order)
-  const long ntiles = (long)((h->g.NX + TX - 1) / TX) * ((h->g.NY + TY - 1) / TY) * ((h->g.NZ + TZ - 1) / TZ);
+  const long ntiles = (long)((h->g.NX + BX - 1) / BX) * ((h->g.NY + BY - 1) / BY) * ((h->g.NZ + BZ - 1) / BZ);
   const long per = (ntiles + NXCD - 1) / NXCD;
-  dim3 grid((unsigned)(per * NXCD));
+  const dim3 grid((unsigned)(per * NXCD));
+  constexpr int nt = BX * BY * BZ;
+  switch (mode) {
+    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_STAGE1: k_stage<MODE_STAGE1, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_FE: k_stage<MODE_FE, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+  }
+}
+
+static void launch_stage_variant(m2cx_solver* h, const StageArgs& a, int mode, int v) {
+  switch (v) {
+    case 1: launch_stage_tile<32, 4, 2>(h, a, mode); break;
+    case 2: launch_stage_tile<8, 8, 4>(h, a, mode); break;
+    case 3: launch_stage_tile<16, 8, 2>(h, a, mode); break;
+    default: launch_stage_tile<16, 4, 4>(h, a, mode); break;
+  }
+}
+
+static void launch_stage(m2cx_solver* h, int vin, int vout, int mode) {
+  StageArgs a = stage_args(h, vin, vout, mode);
   hipEvent_t e0{};
   tic(h, "stage", &e0);
-  switch (mode) {
-    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL><<<grid, NT, 0, h->stream>>>(a); break;
-    case MODE_STAGE1: k_stage<MODE_STAGE1><<<grid, NT, 0, h->stream>>>(a); break;
-    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL><<<grid, NT, 0, h->stream>>>(a); break;
-    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2><<<grid, NT, 0, h->stream>>>(a); break;
-    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL><<<grid, NT, 0, h->stream>>>(a); break;
-    case MODE_FE: k_stage<MODE_FE><<<grid, NT, 0, h->stream>>>(a); break;
-  }
+  launch_stage_variant(h, a, mode, h->tile < 0 ? 0 : h->tile);
   toc(h, "stage", e0);
 }
+
+// Tile-shape autotuner: times the residual launch of every variant on the
+// current state (results are identical across variants: each face flux is
+// computed from the same inputs whichever thread does it, and each cell
+// accumulates its six faces in the same order) and keeps the fastest.  The
+// choice is cached per grid shape and scheme for the process.
+static std::map<std::tuple<int, int, int, int, int, int>, int>& tile_cache() {
+  static std::map<std::tuple<int, int, int, int, int, int>, int> c;
+  return c;
+}
+
+static int autotune(m2cx_solver* h, float* ms_out) {
+  const auto key = std::make_tuple(h->g.NX, h->g.NY, h->g.NZ, h->s.flux, h->s.limiter, (int)h->multimat);
+  auto it = tile_cache().find(key);
+  if (it != tile_cache().end() && !ms_out) {
+    h->tile = it->second;
+    return 0;
+  }
+  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  if (nin < 64L * 64 * 64 && !ms_out) {  // small grids: the default shape
+    h->tile = 0;
+    return 0;
+  }
+  int err0 = 0;
+  HIPCHK(h, hipMemcpy(&err0, &h->cnt->error, sizeof(int), hipMemcpyDeviceToHost));
+  StageArgs a = stage_args(h, 0, -1, MODE_RESIDUAL);
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < kNumTiles; v++) {
+    launch_stage_variant(h, a, MODE_RESIDUAL, v);  // warm
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    for (int r = 0; r < 3; r++) launch_stage_variant(h, a, MODE_RESIDUAL, v);
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    if (ms_out) ms_out[v] = ms / 3;
+    if (ms < best_ms) { best_ms = ms; best = v; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // the residual launches may have flagged the state: restore the flags
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(&h->cnt->error, &err0, sizeof(int), hipMemcpyHostToDevice));
+  h->tile = best;
+  tile_cache()[key] = best;
+  return 0;
+}
+}  // extern "C++"
 
 // Interface Riemann problems of this stage (multi-material only).
 static MLPDev mlp_dev(const m2cx_solver* h) {
@@ -552,6 +629,8 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
   if (!h) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
   if (int rc = ensure_slab(h)) return rc;
+  if (h->tile < 0)
+    if (int rc = autotune(h, nullptr)) return rc;
   const bool clamp = h->sch.max_time > 0;
   for (int q = 0; q < nsteps; q++) {
     if (clamp) {
@@ -971,6 +1050,8 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
     EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 1) * sizeof(int)));
     EHIPCHK(e, hipMalloc(&e->slab, (long)e->lanes * 2 * e->pathcap * 3 * 8));
   }
+  for (m2cx_solver* h : e->m)
+    if (h->tile < 0 && autotune(h, nullptr)) return efail(e, M2CX_E_HIP, "autotune: " + h->err);
   // every member works on the ensemble stream while it is captured and run
   for (m2cx_solver* h : e->m) h->stream = e->stream;
   EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -1068,6 +1149,24 @@ int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id,
   HIPCHK(h, hipStreamSynchronize(h->stream));
   hipFree(dG);
   hipFree(dV0);
+  return 0;
+}
+
+int m2cx_autotune(m2cx_solver* h, int* variant, float* ms_per_variant) {
+  if (!h) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  float ms[kNumTiles];
+  int rc = autotune(h, ms);
+  if (rc) return rc;
+  if (variant) *variant = h->tile;
+  if (ms_per_variant)
+    for (int v = 0; v < kNumTiles; v++) ms_per_variant[v] = ms[v];
+  return 0;
+}
+
+int m2cx_set_tile(m2cx_solver* h, int variant) {
+  if (!h || variant < 0 || variant >= kNumTiles) return M2CX_E_ARG;
+  h->tile = variant;
   return 0;
 }
 

@@ -16,7 +16,6 @@
 
 namespace m2c {
 
-constexpr int TX = 16, TY = 4, TZ = 4, NT = TX * TY * TZ;  // stage tile: 256 threads
 
 struct DevGrid {
   int NX, NY, NZ, SX, SY, SZ;

@@ -106,24 +106,34 @@ __device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, 
   }
 }
 
-// LDS slot of the face flux of axis D whose plus cell has tile-local
-// position (li, lj, lk); x faces: (TX+1) x TY x TZ, y: TX x (TY+1) x TZ,
-// z: TX x TY x (TZ+1), stored one after another.
 constexpr int NXCD = 8;  // XCDs of an MI355X
-constexpr int NFX = (TX + 1) * TY * TZ, NFY = TX * (TY + 1) * TZ, NFZ = TX * TY * (TZ + 1);
-constexpr int NFACE = NFX + NFY + NFZ;
-__device__ __forceinline__ int face_slot(int D, int li, int lj, int lk) {
-  if (D == 0) return (lk * TY + lj) * (TX + 1) + li;
-  if (D == 1) return NFX + (lk * (TY + 1) + lj) * TX + li;
-  return NFX + NFY + (lk * TY + lj) * TX + li;
-}
+
+// LDS slot of the face flux of axis D whose plus cell has tile-local
+// position (li, lj, lk) in a BX x BY x BZ tile; x faces: (BX+1) x BY x BZ,
+// y: BX x (BY+1) x BZ, z: BX x BY x (BZ+1), stored one after another.
+template <int BX, int BY, int BZ>
+struct TileShape {
+  static constexpr int NT = BX * BY * BZ;
+  static constexpr int NFX = (BX + 1) * BY * BZ, NFY = BX * (BY + 1) * BZ, NFZ = BX * BY * (BZ + 1);
+  static constexpr int NFACE = NFX + NFY + NFZ;
+  // plus faces of the last layer, one per thread: y, then z, then x
+  static constexpr int EY = BX * BZ, EZ = BX * BY, EX = BY * BZ;
+  static_assert(EY + EZ + EX <= NT, "one extra face per thread at most");
+  __device__ static __forceinline__ int slot(int D, int li, int lj, int lk) {
+    if (D == 0) return (lk * BY + lj) * (BX + 1) + li;
+    if (D == 1) return NFX + (lk * (BY + 1) + lj) * BX + li;
+    return NFX + NFY + (lk * BY + lj) * BX + li;
+  }
+};
 
 __constant__ DevMats c_mats;
 
-template <int MODE>
-__global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
-  __shared__ double sF[5][NFACE];
-  __shared__ int sMask[NT];
+template <int MODE, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
+  using TS = TileShape<BX, BY, BZ>;
+  constexpr int TX = BX, TY = BY, TZ = BZ;
+  __shared__ double sF[5][TS::NFACE];
+  __shared__ int sMask[TS::NT];
   const DevGrid& g = a.g;
   const Scheme& s = a.s;
   const DevMats& M = c_mats;
@@ -150,18 +160,20 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
   // the first position past the domain end, whose minus face closes the last
   // cell); job 3 (threads < 144) is one plus face of the tile's last layer:
   // y faces for t < 64, z faces for t < 128, x faces for t < 144.
-  const int njobs = (t < 128 + TY * TZ) ? 4 : 3;
+  const int njobs = (t < TS::EY + TS::EZ + TS::EX) ? 4 : 3;
 #pragma unroll 1
   for (int job = 0; job < njobs; job++) {
     int D, li, lj, lk;  // axis and tile-local position of the face's plus cell
     if (job < 3) {
       D = job; li = tx; lj = ty; lk = tz;
-    } else if (t < 64) {
+    } else if (t < TS::EY) {
       D = 1; li = t % TX; lj = TY; lk = t / TX;
-    } else if (t < 128) {
-      D = 2; li = (t - 64) % TX; lj = (t - 64) / TX; lk = TZ;
+    } else if (t < TS::EY + TS::EZ) {
+      const int e = t - TS::EY;
+      D = 2; li = e % TX; lj = e / TX; lk = TZ;
     } else {
-      D = 0; li = TX; lj = (t - 128) % TY; lk = (t - 128) / TY;
+      const int e = t - TS::EY - TS::EZ;
+      D = 0; li = TX; lj = e % TY; lk = e / TY;
     }
     const int fi = ib + li, fj = jb + lj, fk = kb + lk;
     const bool ok = (D == 0) ? (fi <= g.NX && fj < g.NY && fk < g.NZ)
@@ -171,7 +183,7 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
     double f[5];
     int bm, bp;
     face_flux(a, M, D, fi, fj, fk, f, bm, bp, err);
-    const int slot = face_slot(D, li, lj, lk);
+    const int slot = TS::slot(D, li, lj, lk);
 #pragma unroll
     for (int p = 0; p < 5; p++) sF[p][slot] = f[p];
     // check bits: the plus cell's minus face (slot 2D), the minus cell's plus
@@ -206,9 +218,9 @@ __global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
   const double ax = dyj * dzk;
   const double ay = dxi * dzk;
   const double az = dxi * dyj;
-  const int fxm = face_slot(0, tx, ty, tz), fxp = face_slot(0, tx + 1, ty, tz);
-  const int fym = face_slot(1, tx, ty, tz), fyp = face_slot(1, tx, ty + 1, tz);
-  const int fzm = face_slot(2, tx, ty, tz), fzp = face_slot(2, tx, ty, tz + 1);
+  const int fxm = TS::slot(0, tx, ty, tz), fxp = TS::slot(0, tx + 1, ty, tz);
+  const int fym = TS::slot(1, tx, ty, tz), fyp = TS::slot(1, tx, ty + 1, tz);
+  const int fzm = TS::slot(2, tx, ty, tz), fzp = TS::slot(2, tx, ty, tz + 1);
   // minus faces of an interface take localflux2 from the Riemann list
   const bool mp = s.mp_flux != M2CX_MPFLUX_LLF;
   const int idx_m = a.ID[c - 1], idy_m = a.ID[c - g.SX], idz_m = a.ID[c - g.S];
@@ -293,6 +305,7 @@ __global__ void k_resync(DevGrid g, CF5 V, F5 U, const unsigned char* __restrict
 }
 
 __global__ void k_reset_stage(DevCounters* cnt) { cnt->stage_clipped = 0; }
+__global__ void k_reset_error(DevCounters* cnt) { cnt->error = 0; }
 
 // SpaceOperator::ApplyBoundaryConditions on a state field (face ghosts)
 __global__ void k_apply_bc(DevGrid g, F5 V) {

@@ -66,6 +66,8 @@ def lib():
         L.m2cx_mlp_predict.argtypes = [P, C.c_int, F, F]
         L.m2cx_mlp_loss_grad.argtypes = [P, C.c_int, F, F, F, F]
         L.m2cx_mlp_train.argtypes = [P, C.c_int, F, F, C.c_int, C.c_int, C.c_float, C.c_uint32, F]
+        L.m2cx_autotune.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.m2cx_set_tile.argtypes = [P, C.c_int]
         L.m2cx_initialize.argtypes = [P, D, C.c_int, C.POINTER(abi.Geometry), C.c_int]
         L.m2cx_ensemble_create.argtypes = [C.POINTER(P), C.c_int, C.POINTER(P)]
         L.m2cx_ensemble_run_steps.argtypes = [P, C.c_int]
@@ -210,6 +212,15 @@ class Solver:
         self._check(self.L.m2cx_riemann_batch(self.h, n, _d(dirs), _d(Vm), _i(idm), _d(Vp), _i(idp), _d(Vs),
                                               _i(mid), _d(Vsm), _d(Vsp), _i(err), _i(it), _i(fe)))
         return dict(Vs=Vs, id=mid, Vsm=Vsm, Vsp=Vsp, err=err, iters=it, fevals=fe)
+
+    def autotune(self):
+        v = C.c_int()
+        ms = (C.c_float * 4)()
+        self._check(self.L.m2cx_autotune(self.h, C.byref(v), ms))
+        return v.value, list(ms)
+
+    def set_tile(self, variant):
+        self._check(self.L.m2cx_set_tile(self.h, variant))
 
     def initialize(self, default_state, default_id, geoms):
         """Initial condition from planes and spheres on the device, then the

@@ -138,3 +138,32 @@ def test_time_step_bitwise():
     ID = np.zeros(V.shape[:3], dtype=np.int32)
     gpu, o = _pair(g, mats, 1, s, V, ID)
     assert gpu.compute_time_step() == o.time_step()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_tile_variants_bitwise(variant):
+    """Every stage-kernel tile shape gives the oracle's residual and steps,
+    on a grid that no tile shape divides."""
+    n = (37, 13, 11)
+    g = abi.grid(n, (0, 0, 0), (1.0, 0.4, 0.3), (abi.BC_OUTLET, abi.BC_SYMMETRY, abi.BC_SLIPWALL,
+                                                  abi.BC_STICKWALL, abi.BC_OUTLET, abi.BC_OUTLET))
+    mats = abi.materials(abi.material("SG", gamma=1.4))
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK3)
+    V = _random_state(n, 9, True)
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    configs.fill_ghosts(V, ID, None, g)
+    gpu, o = _pair(g, mats, 1, s, V, ID)
+    gpu.set_tile(variant)
+    assert np.array_equal(_interior(gpu.compute_residual()), _interior(o.residual()))
+    gpu.run_steps(3)
+    o.run_steps(3)
+    assert np.array_equal(_interior(gpu.get_state()[0]), _interior(o.get_state()[0]))
+
+
+def test_autotune_reports_all_variants():
+    g, mats, nmat, s, V, ID, Phi = configs.c1_sod((64, 64, 64))
+    from m2c_amd import runtime
+    gpu = runtime.Solver(g, mats, nmat, s)
+    gpu.set_state(V, ID, Phi)
+    v, ms = gpu.autotune()
+    assert 0 <= v < 4 and all(m > 0 for m in ms) and ms[v] == min(ms)
