@@ -51,7 +51,8 @@ enum { M2CX_FLUX_ROE = 0, M2CX_FLUX_LLF = 1, M2CX_FLUX_HLLC = 2 };
 enum { M2CX_REC_CONSTANT = 0, M2CX_REC_LINEAR = 1 };
 enum { M2CX_LIM_NONE = 0, M2CX_LIM_GENERALIZED_MINMOD = 1, M2CX_LIM_VANALBADA = 2 };
 enum { M2CX_SLOPE_ZERO = 0, M2CX_SLOPE_NONZERO = 1 };
-enum { M2CX_VAR_PRIMITIVE = 0 };
+enum { M2CX_VAR_PRIMITIVE = 0, M2CX_VAR_CONSERVATIVE = 1, M2CX_VAR_PRIMITIVE_CHARACTERISTIC = 2,
+       M2CX_VAR_CONSERVATIVE_CHARACTERISTIC = 3 };
 
 /* ExplicitData::Type (IoData.h:931) */
 enum { M2CX_TI_FORWARD_EULER = 0, M2CX_TI_RK2 = 1, M2CX_TI_RK3 = 2 };
@@ -107,7 +108,7 @@ typedef struct {
   int32_t flux;                /* M2CX_FLUX_*                                  */
   int32_t rec_type;            /* M2CX_REC_*                                   */
   int32_t limiter;             /* M2CX_LIM_*                                   */
-  int32_t var_type;            /* M2CX_VAR_PRIMITIVE                           */
+  int32_t var_type;            /* M2CX_VAR_*                                   */
   int32_t slope_near_interface;/* M2CX_SLOPE_*                                 */
   int32_t integrator;          /* M2CX_TI_*                                    */
   double gmm_coeff;            /* GeneralizedMinModCoefficient (1.2)           */

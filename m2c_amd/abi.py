@@ -13,6 +13,7 @@ FLUX_ROE, FLUX_LLF, FLUX_HLLC = 0, 1, 2
 REC_CONSTANT, REC_LINEAR = 0, 1
 LIM_NONE, LIM_GMM, LIM_VANALBADA = 0, 1, 2
 SLOPE_ZERO, SLOPE_NONZERO = 0, 1
+VAR_PRIMITIVE, VAR_CONSERVATIVE, VAR_PRIMITIVE_CHARACTERISTIC, VAR_CONSERVATIVE_CHARACTERISTIC = 0, 1, 2, 3
 TI_FE, TI_RK2, TI_RK3 = 0, 1, 2
 MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
@@ -107,7 +108,7 @@ def default_scheme(**kw):
     s.flux = FLUX_HLLC
     s.rec_type = REC_LINEAR
     s.limiter = LIM_GMM
-    s.var_type = 0
+    s.var_type = VAR_PRIMITIVE
     s.slope_near_interface = SLOPE_NONZERO
     s.integrator = TI_RK2
     s.gmm_coeff = 1.2

@@ -41,6 +41,7 @@ struct m2cx_solver {
   double* R[5] = {};
   unsigned char* ID = nullptr;
   unsigned char* IDn = nullptr;
+  unsigned char* rflag = nullptr;  // primitive fallback of the reconstruction (Scheme::rflag)
   double* geo = nullptr;  // x, y, z, dx, dy, dz (ghosted), CoeffA, CoeffB per axis
   int* coefK = nullptr;   // CoeffK per axis
   double* fd3 = nullptr;  // level-set FD3 coefficient tables
@@ -137,7 +138,8 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (!grid || !mats || !sch || !out || nmat < 1 || nmat > M2C_MAX_MAT) return M2CX_E_ARG;
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return M2CX_E_ARG;
   if ((long)grid->nx * grid->ny * grid->nz >= (1L << 31)) return M2CX_E_ARG;  // 32-bit cell numbering
-  if (sch->var_type != M2CX_VAR_PRIMITIVE || sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
+  if (sch->var_type < M2CX_VAR_PRIMITIVE || sch->var_type > M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) return M2CX_E_ARG;
+  if (sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
   auto* h = new m2cx_solver;
   *out = h;
   h->grid = *grid;
@@ -200,6 +202,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   s.integrator = sch->integrator; s.mp_flux = sch->mp_flux; s.mp_recon = sch->mp_recon;
   s.nmat = nmat; s.inactive = nmat;
   s.gmm = sch->gmm_coeff; s.del = sch->entropy_fix; s.cfl = sch->cfl;
+  s.var_type = sch->var_type;
   DevMats& M = h->M;
   M.nmat = nmat;
   for (int q = 0; q < nmat; q++) M.m[q] = make_eos(mats[q]);
@@ -222,6 +225,11 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   }
   HIPCHK(h, hipMalloc(&h->ID, n));
   HIPCHK(h, hipMalloc(&h->IDn, n));
+  if (sch->var_type != M2CX_VAR_PRIMITIVE && sch->rec_type == M2CX_REC_LINEAR) {
+    HIPCHK(h, hipMalloc(&h->rflag, n));
+    HIPCHK(h, hipMemsetAsync(h->rflag, 0, n, h->stream));
+    s.rflag = h->rflag;
+  }
   HIPCHK(h, hipMemsetAsync(h->ID, 0, n, h->stream));
   HIPCHK(h, hipMemsetAsync(h->IDn, 0, n, h->stream));
   HIPCHK(h, hipMalloc(&h->staging, n * 5 * 8));
@@ -251,7 +259,7 @@ int m2cx_destroy(m2cx_solver* h) {
   // exchange whole fields)
   for (int f = 0; f < 3; f++) hipFree(h->V[f][0]);
   hipFree(h->Un[0]); hipFree(h->U1[0]); hipFree(h->R[0]);
-  hipFree(h->ID); hipFree(h->IDn);
+  hipFree(h->ID); hipFree(h->IDn); hipFree(h->rflag);
   hipFree(h->geo); hipFree(h->coefK);
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
@@ -375,7 +383,7 @@ struct TileVariant {
 static const TileVariant kTiles[] = {{16, 4, 4}, {32, 4, 2}, {8, 8, 4}, {16, 8, 2}};
 static constexpr int kNumTiles = 4;
 
-template <int BX, int BY, int BZ>
+template <int BX, int BY, int BZ, bool GEN>
 static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode) {
   // one work-group per tile, padded to a multiple of the XCD count (see
   // k_stage for the tile order)
@@ -384,21 +392,28 @@ static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode) {
   const dim3 grid((unsigned)(per * NXCD));
   constexpr int nt = BX * BY * BZ;
   switch (mode) {
-    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_STAGE1: k_stage<MODE_STAGE1, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_FE: k_stage<MODE_FE, BX, BY, BZ><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_STAGE1: k_stage<MODE_STAGE1, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_FE: k_stage<MODE_FE, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
   }
 }
 
+// Reconstruction in conservative or characteristic variables runs one tile
+// shape (16x4x4) of the general kernel; primitive variables choose the tuned
+// tile of the primitive-only kernel.
 static void launch_stage_variant(m2cx_solver* h, const StageArgs& a, int mode, int v) {
+  if (a.s.var_type != M2CX_VAR_PRIMITIVE && a.s.rec_type == M2CX_REC_LINEAR) {
+    launch_stage_tile<16, 4, 4, true>(h, a, mode);
+    return;
+  }
   switch (v) {
-    case 1: launch_stage_tile<32, 4, 2>(h, a, mode); break;
-    case 2: launch_stage_tile<8, 8, 4>(h, a, mode); break;
-    case 3: launch_stage_tile<16, 8, 2>(h, a, mode); break;
-    default: launch_stage_tile<16, 4, 4>(h, a, mode); break;
+    case 1: launch_stage_tile<32, 4, 2, false>(h, a, mode); break;
+    case 2: launch_stage_tile<8, 8, 4, false>(h, a, mode); break;
+    case 3: launch_stage_tile<16, 8, 2, false>(h, a, mode); break;
+    default: launch_stage_tile<16, 4, 4, false>(h, a, mode); break;
   }
 }
 
@@ -421,6 +436,10 @@ static std::map<std::tuple<int, int, int, int, int, int>, int>& tile_cache() {
 }
 
 static int autotune(m2cx_solver* h, float* ms_out) {
+  if (h->rflag && !ms_out) {  // one tile shape for the general reconstruction
+    h->tile = 0;
+    return 0;
+  }
   const auto key = std::make_tuple(h->g.NX, h->g.NY, h->g.NZ, h->s.flux, h->s.limiter, (int)h->multimat);
   auto it = tile_cache().find(key);
   if (it != tile_cache().end() && !ms_out) {
@@ -483,7 +502,16 @@ static void guess_stage(m2cx_solver* h, int vin) {
   }
 }
 
+// Cells falling back to primitive reconstruction for this stage's input
+// (conservative / characteristic variable types only).
+static void recon_stage(m2cx_solver* h, int vin) {
+  if (!h->rflag) return;
+  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  k_recon_flags<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[vin]), h->ID, h->rflag, h->cnt);
+}
+
 static void interface_stage(m2cx_solver* h, int vin, int stage1) {
+  recon_stage(h, vin);
   if (!h->multimat) return;
   guess_stage(h, vin);
   hipEvent_t e0{};
@@ -975,7 +1003,7 @@ struct m2cx_ensemble {
   int nriem = 0;              // members with interface faces
   int* doff = nullptr;
   double* slab = nullptr;
-  int lanes = 256 * 256, pathcap = 512;
+  int lanes = RIEM_SLOTS, pathcap = 512;  // integration-path scratch: one slot per wave
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::string err;
@@ -1001,11 +1029,14 @@ static void ensemble_step(m2cx_ensemble* e) {
   StageSpec sp[3];
   stage_specs(e->ti, sp);
   for (int q = 0; q < e->nstages; q++) {
-    for (m2cx_solver* h : e->m) guess_stage(h, sp[q].vin);
+    for (m2cx_solver* h : e->m) {
+      recon_stage(h, sp[q].vin);
+      guess_stage(h, sp[q].vin);
+    }
     if (e->nriem) {
       k_riem_offsets<<<1, 1, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff);
-      k_riemann_multi<<<(e->lanes + 127) / 128, 128, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1,
-                                                                     e->slab, e->lanes, e->pathcap);
+      k_riemann_multi<<<RIEM_SLOTS, 64, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1, e->slab,
+                                                        e->lanes, e->pathcap);
       k_riem_count<<<e->nriem, 256, 0, e->stream>>>(e->dmem[q]);
     }
     for (m2cx_solver* h : e->m) stage_after_riemann(h, sp[q]);
@@ -1047,7 +1078,7 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
       EHIPCHK(e, hipMalloc(&e->dmem[q], tab[q].size() * sizeof(RiemMember)));
       EHIPCHK(e, hipMemcpy(e->dmem[q], tab[q].data(), tab[q].size() * sizeof(RiemMember), hipMemcpyHostToDevice));
     }
-    EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 1) * sizeof(int)));
+    EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 2) * sizeof(int)));
     EHIPCHK(e, hipMalloc(&e->slab, (long)e->lanes * 2 * e->pathcap * 3 * 8));
   }
   for (m2cx_solver* h : e->m)

@@ -57,7 +57,12 @@ __device__ __forceinline__ const int* g_coefK(const DevGrid& g, int D) {
 struct Scheme {
   int flux, rec_type, limiter, slope_zero, integrator;
   int mp_flux, mp_recon, nmat, inactive;
+  int var_type;                 // M2CX_VAR_*
   double gmm, del, cfl;
+  // per cell (ghosted index): 1 where a reconstruction in var_type fails
+  // CheckState on some face and the cell falls back to primitive variables
+  // (Reconstructor.cpp:722-778); written by k_recon_flags each stage
+  const unsigned char* rflag;
 };
 
 __device__ __forceinline__ long gidx(const DevGrid& g, int i, int j, int k) {
@@ -96,6 +101,106 @@ __device__ __forceinline__ void store5(const F5& f, long c, const double* v) {
 }
 
 // ---------------------------------------------------------------------------
+// Limited slope of interior cell (i,j,k) along axis D for the reconstruction
+// variables `vtype` (Reconstructor.cpp:472-648): one-sided differences of V
+// or U = P2C(V) (each neighbour with its own material, :400-409), zero next
+// to an inactive cell; for the characteristic types they are taken to
+// characteristic variables about the cell state, limited component by
+// component, and brought back.  q0 receives the cell value in the
+// reconstructed variables (V or U).  Returns false when c0^2 < 0 at the cell
+// (the reference aborts).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool recon_slope(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
+                                            const unsigned char* __restrict__ ID, int vtype, int D, int i, int j,
+                                            int k, const double* v, double* q0, double* sig) {
+  const long c = gidx(g, i, j, k);
+  const long st = axis_stride(g, D);
+  const int myid = ID[c], idm = ID[c - st], idp = ID[c + st];
+  const DevEos& E = M.m[myid];
+  double qm[5], qp[5];
+  load5(V, c - st, qm);
+  load5(V, c + st, qp);
+  const bool cons = (vtype == M2CX_VAR_CONSERVATIVE || vtype == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC);
+  const bool chr = (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC || vtype == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC);
+  if (cons) {
+    double t[5];
+    eos_p2c(M.m[idm], qm, t);
+#pragma unroll
+    for (int p = 0; p < 5; p++) qm[p] = t[p];
+    eos_p2c(M.m[idp], qp, t);
+#pragma unroll
+    for (int p = 0; p < 5; p++) qp[p] = t[p];
+    eos_p2c(E, v, q0);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 5; p++) q0[p] = v[p];
+  }
+  double d0[5], d1[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    d0[p] = (idm == s.inactive) ? 0.0 : q0[p] - qm[p];
+    d1[p] = (idp == s.inactive) ? 0.0 : qp[p] - q0[p];
+  }
+  CharBasis b;
+  bool ok = true;
+  if (chr) {
+    ok = char_setup(E, v, D, b);
+    double w[5];
+    if (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC) {
+      char_from_prim(b, v, d0, w);
+#pragma unroll
+      for (int p = 0; p < 5; p++) d0[p] = w[p];
+      char_from_prim(b, v, d1, w);
+#pragma unroll
+      for (int p = 0; p < 5; p++) d1[p] = w[p];
+    } else {
+      char_left(b, v, d0, w);
+#pragma unroll
+      for (int p = 0; p < 5; p++) d0[p] = w[p];
+      char_left(b, v, d1, w);
+#pragma unroll
+      for (int p = 0; p < 5; p++) d1[p] = w[p];
+    }
+  }
+  const int a = (D == 0) ? i : (D == 1 ? j : k);
+  const double A = g_coefA(g, D)[a];
+  const double B = g_coefB(g, D)[a];
+  const int K = g_coefK(g, D)[a];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    if (s.limiter == M2CX_LIM_GENERALIZED_MINMOD) sig[p] = lim_gmm(A, B, s.gmm, d0[p], d1[p]);
+    else if (s.limiter == M2CX_LIM_VANALBADA) sig[p] = lim_va(A, B, K, d0[p], d1[p]);
+    else sig[p] = 0.5 * (d0[p] + d1[p]);
+  }
+  if (chr) {
+    double w[5];
+    if (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC) char_to_prim(b, v, sig, w);
+    else char_right(b, v, sig, w);
+#pragma unroll
+    for (int p = 0; p < 5; p++) sig[p] = w[p];
+  }
+  if (s.slope_zero && (myid != idm || myid != idp)) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) sig[p] = 0.0;
+  }
+  return ok;
+}
+
+// One face (side 0 minus, 1 plus) from recon_slope's output (:665-714).
+__device__ __forceinline__ void recon_face(const DevEos& E, int vtype, const double* v, const double* q0,
+                                           const double* sig, int side, double* out) {
+  if (vtype == M2CX_VAR_CONSERVATIVE || vtype == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) {
+    double u2[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) u2[p] = side ? q0[p] + 0.5 * sig[p] : q0[p] - 0.5 * sig[p];
+    eos_c2p(E, u2, out);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 5; p++) out[p] = side ? v[p] + 0.5 * sig[p] : v[p] - 0.5 * sig[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reconstructed face state of an interior cell q along axis D (side 0: the
 // minus face, vl/vb/vk; side 1: the plus face, vr/vt/vf).
 //
@@ -106,7 +211,14 @@ __device__ __forceinline__ void store5(const F5& f, long c, const double* v) {
 // the same as: a face reverts to the cell value iff it fails CheckState or
 // clipping would modify it.  `fail`/`fixable` report the two conditions so
 // the owning cell can detect the abort case.
+//
+// For the other variable types a failing face sends the whole cell back to
+// primitive variables (the goto RETRY of :722-778); s.rflag marks those cells
+// (k_recon_flags).  The faces of an unmarked cell all pass CheckState, so
+// only clipping can revert them.  GEN = false compiles the primitive path
+// alone.
 // ---------------------------------------------------------------------------
+template <bool GEN = true>
 __device__ __forceinline__ void face_state(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
                                            const unsigned char* __restrict__ ID, int D, int i, int j, int k,
                                            int side, double* out, bool& fail, bool& unfixable) {
@@ -125,6 +237,17 @@ __device__ __forceinline__ void face_state(const DevGrid& g, const Scheme& s, co
       bool f = eos_check5(E, out), w = eos_would_clip(E, out);
       fail = f;
       unfixable = f && !w;
+    }
+    return;
+  }
+  if (GEN && s.var_type != M2CX_VAR_PRIMITIVE && s.rflag[c] == 0) {
+    const DevEos& E = M.m[myid];
+    double q0[5], sig[5];
+    recon_slope(g, s, M, V, ID, s.var_type, D, i, j, k, v, q0, sig);
+    recon_face(E, s.var_type, v, q0, sig, side, out);
+    if (eos_would_clip(E, out)) {
+#pragma unroll
+      for (int p = 0; p < 5; p++) out[p] = v[p];
     }
     return;
   }
@@ -181,6 +304,7 @@ __device__ __forceinline__ void ghost_face(int bc, int D, const double* vghost, 
 // Both states of the face between cell (i,j,k) - e_D and (i,j,k), i.e. the
 // minus face of (i,j,k).  `bits_m`/`bits_p` return fail/unfixable bits for the
 // two owning cells (bit 0 fail, bit 1 unfixable).
+template <bool GEN = true>
 __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, const DevMats& M, const CF5& V,
                                           const unsigned char* __restrict__ ID, int D, int i, int j, int k,
                                           double* Vm, double* Vp, int& bits_m, int& bits_p) {
@@ -192,7 +316,7 @@ __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, con
   if (a == 0) {
     // ghost on the minus side
     double fp[5], vg[5];
-    face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
+    face_state<GEN>(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
     bits_p = (f ? 1 : 0) | (u ? 2 : 0);
     load5(V, gidx(g, im, jm, km), vg);
     for (int p = 0; p < 5; p++) fp[p] = Vp[p];
@@ -200,15 +324,15 @@ __device__ __forceinline__ void face_pair(const DevGrid& g, const Scheme& s, con
   } else if (a == N) {
     // ghost on the plus side
     double fm[5], vg[5];
-    face_state(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
+    face_state<GEN>(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
     bits_m = (f ? 1 : 0) | (u ? 2 : 0);
     load5(V, gidx(g, i, j, k), vg);
     for (int p = 0; p < 5; p++) fm[p] = Vm[p];
     ghost_face(g.bc[2 * D + 1], D, vg, fm, Vp);
   } else {
-    face_state(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
+    face_state<GEN>(g, s, M, V, ID, D, im, jm, km, 1, Vm, f, u);
     bits_m = (f ? 1 : 0) | (u ? 2 : 0);
-    face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
+    face_state<GEN>(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
     bits_p = (f ? 1 : 0) | (u ? 2 : 0);
   }
 }

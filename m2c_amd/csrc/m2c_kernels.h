@@ -15,6 +15,7 @@ struct DevCounters {
   unsigned int dt_ticket;
   int nif;            // interface faces in the current list
   int n_remaining, n_changed, n_unresolved, pad;
+  unsigned int riem_next, pad3;  // face queue of the interface Riemann launch
   double t, dt, cfl, dts, tmax;
   int time_step, pad2;
 };
@@ -80,6 +81,7 @@ __device__ __forceinline__ void write_ghosts(const DevGrid& g, const F5& V, int 
 
 // Flux through the minus face (axis D) of cell (i,j,k); returns f1 (applied to
 // the minus-side cell) and the reconstruction-check bits of the two cells.
+template <bool GEN>
 __device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, int D, int i, int j, int k,
                                           double* f1, int& bm, int& bp, int& err) {
   const DevGrid& g = a.g;
@@ -88,7 +90,7 @@ __device__ __forceinline__ void face_flux(const StageArgs& a, const DevMats& M, 
   const long st = axis_stride(g, D);
   const int myid = a.ID[c], nid = a.ID[c - st];
   double Vm[5], Vp[5];
-  face_pair(g, s, M, a.Vin, a.ID, D, i, j, k, Vm, Vp, bm, bp);
+  face_pair<GEN>(g, s, M, a.Vin, a.ID, D, i, j, k, Vm, Vp, bm, bp);
   if (nid == myid) {
     if (myid != s.inactive) flux_num(M.m[myid], s.flux, s.del, D, Vm, Vp, f1, err);
     else for (int p = 0; p < 5; p++) f1[p] = 0.0;
@@ -128,7 +130,9 @@ struct TileShape {
 
 __constant__ DevMats c_mats;
 
-template <int MODE, int BX, int BY, int BZ>
+// GEN: reconstruction in any variable type (false: primitive variables only,
+// the common case, compiled without the characteristic code)
+template <int MODE, int BX, int BY, int BZ, bool GEN>
 __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   using TS = TileShape<BX, BY, BZ>;
   constexpr int TX = BX, TY = BY, TZ = BZ;
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
     if (!ok) continue;
     double f[5];
     int bm, bp;
-    face_flux(a, M, D, fi, fj, fk, f, bm, bp, err);
+    face_flux<GEN>(a, M, D, fi, fj, fk, f, bm, bp, err);
     const int slot = TS::slot(D, li, lj, lk);
 #pragma unroll
     for (int p = 0; p < 5; p++) sF[p][slot] = f[p];
@@ -286,6 +290,41 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   if (MODE == MODE_STAGE1 || MODE == MODE_RK3_STAGE2) store5(a.U1, c, u);
   store5(a.Vout, c, v);
   write_ghosts(g, a.Vout, i, j, k, v);
+}
+
+// Cells whose reconstruction in the scheme's variable type fails CheckState
+// on one of their six faces; they are reconstructed in primitive variables
+// instead (Reconstructor.cpp:722-778).  c0^2 < 0 in a characteristic
+// transform is the reference's abort (FluxFcnBase.h:315-320).
+__global__ void k_recon_flags(DevGrid g, Scheme s, CF5 V, const unsigned char* __restrict__ ID,
+                              unsigned char* __restrict__ flag, DevCounters* cnt) {
+  const DevMats& M = c_mats;
+  const long n = (long)g.NX * g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
+    const long c = gidx(g, i, j, k);
+    const int myid = ID[c];
+    if (myid == s.inactive) {
+      flag[c] = 0;
+      continue;
+    }
+    const DevEos& E = M.m[myid];
+    double v[5];
+    load5(V, c, v);
+    bool fails = false, ok = true;
+#pragma unroll 1
+    for (int D = 0; D < 3; D++) {
+      double q0[5], sig[5], f[5];
+      ok = recon_slope(g, s, M, V, ID, s.var_type, D, i, j, k, v, q0, sig) && ok;
+      recon_face(E, s.var_type, v, q0, sig, 0, f);
+      fails = fails || eos_check5(E, f);
+      recon_face(E, s.var_type, v, q0, sig, 1, f);
+      fails = fails || eos_check5(E, f);
+    }
+    flag[c] = fails ? 1 : 0;
+    if (!ok) atomicOr(&cnt->error, 1);
+  }
 }
 
 // U = P2C(V) on interior cells, only if the preceding stage clipped a cell

@@ -269,17 +269,35 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   return flags;
 }
 
-__global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
-                                                 const unsigned char* __restrict__ ID, IfaceList L, int stage1,
-                                                 double* slab, int lanes, int pathcap, const float* guess,
-                                                 DevCounters* cnt) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane >= lanes) return;
+// One wave per interface face.  The exact solver's iteration counts differ
+// from face to face by two orders of magnitude, and faces sharing a wave run
+// the union of their control paths one after another; with one face per wave
+// (all 64 lanes evaluate it, the solve being a single serial chain) every
+// face runs at the speed of its own chain.  A wave takes the next face from
+// the queue cnt->riem_next (reset by k_riem_queue_reset before the launch)
+// until the list is exhausted; `slot` is the wave's integration-path scratch.
+constexpr int RIEM_SLOTS = 2048;  // waves of one interface launch
+
+__global__ void k_riem_queue_reset(DevCounters* cnt) { cnt->riem_next = 0; }
+
+__device__ __forceinline__ int riem_next_face(unsigned int* queue) {
+  int q = 0;
+  if (threadIdx.x == 0) q = (int)atomicAdd(queue, 1u);
+  return __shfl(q, 0);
+}
+
+__global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
+                                                const unsigned char* __restrict__ ID, IfaceList L, int stage1,
+                                                double* slab, int stride, int pathcap, const float* guess,
+                                                DevCounters* cnt) {
+  const int slot = blockIdx.x;
   const int nif = min(cnt->nif, L.cap);
-  DevRiemann R(c_mats, rc, slab + lane, lanes, pathcap);
+  DevRiemann R(c_mats, rc, slab + slot, stride, pathcap);
   unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
   int fatal = 0, ovf = 0;
-  for (int q = lane; q < nif; q += lanes) {
+  for (;;) {
+    const int q = riem_next_face(&cnt->riem_next);
+    if (q >= nif) break;
     const int f = riemann_face(g, s, V, ID, L, q, stage1, guess, R, fatal, ovf);
     calls++;
     triv += f & 1;
@@ -289,6 +307,7 @@ __global__ void __launch_bounds__(128) k_riemann(DevGrid g, Scheme s, RiemannCfg
     its += R.st.it_main;
     fev += R.st.fevals;
   }
+  if (threadIdx.x != 0) return;
   if (calls) {
     atomicAdd(&cnt->riemann_calls, calls);
     atomicAdd(&cnt->riemann_trivial, triv);
@@ -319,6 +338,8 @@ struct RiemMember {
   DevCounters* cnt;
 };
 
+// off[m]: first face of member m in the concatenated list; off[nm]: total;
+// off[nm + 1]: the face queue of k_riemann_multi, reset here
 __global__ void k_riem_offsets(const RiemMember* mem, int nm, int* off) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int acc = 0;
@@ -327,15 +348,18 @@ __global__ void k_riem_offsets(const RiemMember* mem, int nm, int* off) {
     acc += min(mem[m].cnt->nif, mem[m].L.cap);
   }
   off[nm] = acc;
+  off[nm + 1] = 0;
 }
 
-__global__ void __launch_bounds__(128) k_riemann_multi(const RiemMember* __restrict__ mem, int nm,
-                                                       const int* __restrict__ off, int stage1, double* slab,
-                                                       int lanes, int pathcap) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane >= lanes) return;
+// one wave per face over the concatenated list, as k_riemann
+__global__ void __launch_bounds__(64) k_riemann_multi(const RiemMember* __restrict__ mem, int nm, int* off,
+                                                      int stage1, double* slab, int stride, int pathcap) {
+  const int slot = blockIdx.x;
   const int total = off[nm];
-  for (int q = lane; q < total; q += lanes) {
+  int fatal = 0, ovf = 0;
+  for (;;) {
+    const int q = riem_next_face(reinterpret_cast<unsigned int*>(off + nm + 1));
+    if (q >= total) break;
     int lo = 0, hi = nm - 1;  // member m with off[m] <= q < off[m+1]
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -343,11 +367,14 @@ __global__ void __launch_bounds__(128) k_riemann_multi(const RiemMember* __restr
       else hi = mid - 1;
     }
     const RiemMember& B = mem[lo];
-    DevRiemann R(c_mats, B.rc, slab + lane, lanes, pathcap);
-    int fatal = 0, ovf = 0;
+    DevRiemann R(c_mats, B.rc, slab + slot, stride, pathcap);
+    fatal = 0;
+    ovf = 0;
     riemann_face(B.g, B.s, B.V, B.ID, B.L, q - off[lo], stage1, B.guess, R, fatal, ovf);
-    if (fatal) atomicOr(&B.cnt->error, 2);
-    if (ovf) atomicOr(&B.cnt->error, 4);
+    if (threadIdx.x == 0) {
+      if (fatal) atomicOr(&B.cnt->error, 2);
+      if (ovf) atomicOr(&B.cnt->error, 4);
+    }
   }
 }
 
@@ -972,8 +999,9 @@ inline int mp_slab(MPBuffers& B) {
 inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, const unsigned char* ID, int stage1,
                        DevCounters* cnt, hipStream_t st) {
   if (s.mp_flux == M2CX_MPFLUX_LLF) return;
-  k_riemann<<<(B.lanes + 127) / 128, 128, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
-                                                   B.guess ? B.pguess : nullptr, cnt);
+  k_riem_queue_reset<<<1, 1, 0, st>>>(cnt);
+  k_riemann<<<RIEM_SLOTS, 64, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
+                                       B.guess ? B.pguess : nullptr, cnt);
 }
 
 // plain copies as kernels (no memcpy nodes when a step is captured in a graph)

@@ -9,6 +9,7 @@
 //   fluxes         FluxFcnBase.h:83-123,287-303, FluxFcnHLLC.h:58-245,
 //                  FluxFcnGenRoe.h:64-221, FluxFcnLLF.h:47-185
 //   limiters       Reconstructor.h:96-111
+//   characteristic FluxFcnBase.h:309-517
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cfloat>
@@ -374,6 +375,109 @@ __device__ __forceinline__ double lim_va(double A, double B, int k, double dq0, 
   if (k == 2) { a = dq0 * dq0; b = dq1 * dq1; }
   else { a = ipow_dev(dq0, k); b = ipow_dev(dq1, k); }
   return B * (dq1 * a + dq0 * b) / (a + A * b);
+}
+
+
+// ---------------------------------------------------------------- characteristic variables
+// Eigenvectors of the flux Jacobian along axis d about a cell state V0
+// (FluxFcnBase.h:309-517): L = R^{-1} (left), R (right) and Q = dU/dV.  The
+// entries and the sums of products are formed in the reference's order, so
+// every product rounds as it does there.
+struct CharBasis {
+  double e0, c0, dpdrho0, Gamma0, kin0, un0;
+  int d;
+};
+// false when c0^2 < 0, where the reference aborts (FluxFcnBase.h:315-320)
+__device__ __forceinline__ bool char_setup(const DevEos& E, const double* V0, int d, CharBasis& b) {
+  b.e0 = eos_e(E, V0[0], V0[4]);
+  const double c2 = eos_c2(E, V0[0], b.e0);
+  b.c0 = sqrt(c2);
+  b.dpdrho0 = eos_dpdrho(E, V0[0], b.e0);
+  b.Gamma0 = eos_Gamma(E, V0[0]);
+  b.kin0 = 0.5 * (V0[1] * V0[1] + V0[2] * V0[2] + V0[3] * V0[3]);
+  b.un0 = vsel(V0, d);
+  b.d = d;
+  return !(c2 < 0);
+}
+__device__ __forceinline__ void matvec5(const double (&M)[5][5], const double* x, double* y) {
+#pragma unroll
+  for (int r = 0; r < 5; r++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc += M[r][q] * x[q];
+    y[r] = acc;
+  }
+}
+// y = L x (FluxFcnBase.h:336-358)
+__device__ __forceinline__ void char_left(const CharBasis& b, const double* V0, const double* x, double* y) {
+  const double al = V0[0] * b.Gamma0 / (V0[4] * b.Gamma0 + V0[0] * b.dpdrho0);
+  const double be = (b.e0 - b.kin0 + V0[4] / V0[0]) * al;
+  double L[5][5];
+  L[0][0] = 0.5 * (1.0 - be + b.un0 / b.c0);
+  L[4][0] = 0.5 * (1.0 - be - b.un0 / b.c0);
+  L[0][4] = al / 2.0;
+  L[4][4] = al / 2.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const double nr = (r == b.d) ? 1.0 : 0.0;
+    L[0][1 + r] = -0.5 * (al * V0[1 + r] + nr / b.c0);
+    L[4][1 + r] = -0.5 * (al * V0[1 + r] - nr / b.c0);
+    L[1 + r][0] = -V0[1 + r] + (be + b.un0) * nr;
+    L[1 + r][4] = -al * nr;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double nq = (q == b.d) ? 1.0 : 0.0;
+      L[1 + r][1 + q] = (r == q ? 1.0 : 0.0) + al * nr * V0[1 + q] - nr * nq;
+    }
+  }
+  matvec5(L, x, y);
+}
+// y = R x (FluxFcnBase.h:392-417)
+__device__ __forceinline__ void char_right(const CharBasis& b, const double* V0, const double* x, double* y) {
+  const double H0 = b.e0 + b.kin0 + V0[4] / V0[0];
+  const double par = b.e0 + b.kin0 - b.dpdrho0 / b.Gamma0 - b.un0;
+  double R[5][5];
+  R[0][0] = 1.0;
+  R[0][4] = 1.0;
+  R[4][0] = H0 - b.un0 * b.c0;
+  R[4][4] = H0 + b.un0 * b.c0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const double nr = (r == b.d) ? 1.0 : 0.0;
+    R[0][1 + r] = nr;
+    R[1 + r][0] = V0[1 + r] - b.c0 * nr;
+    R[1 + r][4] = V0[1 + r] + b.c0 * nr;
+    R[4][1 + r] = V0[1 + r] + par * nr;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double nq = (q == b.d) ? 1.0 : 0.0;
+      R[1 + r][1 + q] = (r == q ? 1.0 : 0.0) + V0[1 + r] * nq - nr * nq;
+    }
+  }
+  matvec5(R, x, y);
+}
+// primitive difference -> characteristic: L Q dV (FluxFcnBase.h:309-362)
+__device__ __forceinline__ void char_from_prim(const CharBasis& b, const double* V0, const double* dV, double* dW) {
+  const double q51 = b.e0 + b.kin0 - b.dpdrho0 / b.Gamma0;
+  double t[5];
+  t[0] = dV[0];
+  t[1] = V0[1] * dV[0] + V0[0] * dV[1];
+  t[2] = V0[2] * dV[0] + V0[0] * dV[2];
+  t[3] = V0[3] * dV[0] + V0[0] * dV[3];
+  t[4] = q51 * dV[0] + V0[0] * V0[1] * dV[1] + V0[0] * V0[2] * dV[2] + V0[0] * V0[3] * dV[3] +
+         1.0 / b.Gamma0 * dV[4];
+  char_left(b, V0, t, dW);
+}
+// characteristic -> primitive difference: Q^{-1} R dW (FluxFcnBase.h:367-421)
+__device__ __forceinline__ void char_to_prim(const CharBasis& b, const double* V0, const double* dW, double* dV) {
+  double t[5];
+  char_right(b, V0, dW, t);
+  dV[0] = t[0];
+  dV[1] = -V0[1] / V0[0] * t[0] + t[1] / V0[0];
+  dV[2] = -V0[2] / V0[0] * t[0] + t[2] / V0[0];
+  dV[3] = -V0[3] / V0[0] * t[0] + t[3] / V0[0];
+  dV[4] = ((-b.e0 + b.kin0) * b.Gamma0 + b.dpdrho0) * t[0] -
+          b.Gamma0 * (V0[1] * t[1] + V0[2] * t[2] + V0[3] * t[3] - t[4]);
 }
 
 }  // namespace m2c

@@ -16,6 +16,11 @@ def _sound_speed(mat, rho, p):
     return np.sqrt(np.maximum(gam * (p + pc) / rho, 1e-300))
 
 
+def sound_speeds(mats, ids, V):
+    """c of each state V[q] (rho, u, v, w, p) of material ids[q]."""
+    return np.array([_sound_speed(mats[int(i)], v[0], v[4]) for i, v in zip(ids, V)])
+
+
 def sample_problems(mats, n, seed, ref_density=None, p_level=(1e4, 1e6)):
     """n two-material Riemann problems between materials 0 and 1 (random
     orientation).  Returns dirs, Vm, idm, Vp, idp (float64 / int32)."""

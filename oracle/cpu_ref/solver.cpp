@@ -206,8 +206,195 @@ void Solver::ApplyBoundaryConditions(std::vector<double>& Vio) {
 }
 
 // ---------------------------------------------------------------------------
-// Reconstructor::Reconstruct, primitive variables (Reconstructor.cpp:335-918)
+// Characteristic variables of the Euler equations along axis `dir`, about the
+// cell state V0 (FluxFcnBase.h:309-517).  L = R^{-1} holds the left and R
+// the right eigenvectors of dF/dU; Q = dU/dV.  Entries and sums are formed in
+// the reference's order, so every product rounds the same way.
 // ---------------------------------------------------------------------------
+struct CharBasis {
+  double e0, c0, dpdrho0, Gamma0, kin0, un0, n[3];
+};
+
+// false: c0^2 < 0, where the reference aborts (FluxFcnBase.h:315-320)
+static bool CharSetup(const Eos& E, const double* V0, int dir, CharBasis& b) {
+  b.e0 = E.GetInternalEnergyPerUnitMass(V0[0], V0[4]);
+  double c2 = E.ComputeSoundSpeedSquare(V0[0], b.e0);
+  if (c2 < 0) return false;
+  b.c0 = std::sqrt(c2);
+  b.dpdrho0 = E.GetDpdrho(V0[0], b.e0);
+  b.Gamma0 = E.GetBigGamma(V0[0], b.e0);
+  b.kin0 = 0.5 * (V0[1] * V0[1] + V0[2] * V0[2] + V0[3] * V0[3]);
+  b.un0 = V0[dir + 1];
+  for (int q = 0; q < 3; q++) b.n[q] = (q == dir) ? 1.0 : 0.0;
+  return true;
+}
+
+// FluxFcnBase.h:336-352 (and :445-461)
+static void LeftEigenvectors(const CharBasis& b, const double* V0, double L[5][5]) {
+  const double al = V0[0] * b.Gamma0 / (V0[4] * b.Gamma0 + V0[0] * b.dpdrho0);
+  const double be = (b.e0 - b.kin0 + V0[4] / V0[0]) * al;
+  L[0][0] = 0.5 * (1.0 - be + b.un0 / b.c0);
+  L[4][0] = 0.5 * (1.0 - be - b.un0 / b.c0);
+  L[0][4] = al / 2.0;
+  L[4][4] = al / 2.0;
+  for (int r = 0; r < 3; r++) {
+    L[0][1 + r] = -0.5 * (al * V0[1 + r] + b.n[r] / b.c0);
+    L[4][1 + r] = -0.5 * (al * V0[1 + r] - b.n[r] / b.c0);
+    L[1 + r][0] = -V0[1 + r] + (be + b.un0) * b.n[r];
+    L[1 + r][4] = -al * b.n[r];
+    for (int q = 0; q < 3; q++) L[1 + r][1 + q] = (double)(r == q) + al * b.n[r] * V0[1 + q] - b.n[r] * b.n[q];
+  }
+}
+
+// FluxFcnBase.h:392-410 (and :497-510)
+static void RightEigenvectors(const CharBasis& b, const double* V0, double R[5][5]) {
+  const double H0 = b.e0 + b.kin0 + V0[4] / V0[0];
+  const double par = b.e0 + b.kin0 - b.dpdrho0 / b.Gamma0 - b.un0;
+  R[0][0] = 1.0;
+  R[0][4] = 1.0;
+  R[4][0] = H0 - b.un0 * b.c0;
+  R[4][4] = H0 + b.un0 * b.c0;
+  for (int r = 0; r < 3; r++) {
+    R[0][1 + r] = b.n[r];
+    R[1 + r][0] = V0[1 + r] - b.c0 * b.n[r];
+    R[1 + r][4] = V0[1 + r] + b.c0 * b.n[r];
+    R[4][1 + r] = V0[1 + r] + par * b.n[r];
+    for (int q = 0; q < 3; q++) R[1 + r][1 + q] = (double)(r == q) + V0[1 + r] * b.n[q] - b.n[r] * b.n[q];
+  }
+}
+
+static void MatVec5(const double M[5][5], const double* x, double* y) {
+  for (int r = 0; r < 5; r++) {
+    double acc = 0.0;
+    for (int q = 0; q < 5; q++) acc += M[r][q] * x[q];
+    y[r] = acc;
+  }
+}
+
+// primitive difference -> characteristic (FluxFcnBase.h:309-362): L Q dV
+static void PrimToChar(const CharBasis& b, const double* V0, const double* dV, double* dW) {
+  double t[5], L[5][5];
+  const double q51 = b.e0 + b.kin0 - b.dpdrho0 / b.Gamma0;
+  t[0] = dV[0];
+  t[1] = V0[1] * dV[0] + V0[0] * dV[1];
+  t[2] = V0[2] * dV[0] + V0[0] * dV[2];
+  t[3] = V0[3] * dV[0] + V0[0] * dV[3];
+  t[4] = q51 * dV[0] + V0[0] * V0[1] * dV[1] + V0[0] * V0[2] * dV[2] + V0[0] * V0[3] * dV[3] + 1.0 / b.Gamma0 * dV[4];
+  LeftEigenvectors(b, V0, L);
+  MatVec5(L, t, dW);
+}
+
+// characteristic -> primitive difference (FluxFcnBase.h:367-421): Q^{-1} R dW
+static void CharToPrim(const CharBasis& b, const double* V0, const double* dW, double* dV) {
+  double t[5], R[5][5];
+  RightEigenvectors(b, V0, R);
+  MatVec5(R, dW, t);
+  dV[0] = t[0];
+  dV[1] = -V0[1] / V0[0] * t[0] + t[1] / V0[0];
+  dV[2] = -V0[2] / V0[0] * t[0] + t[2] / V0[0];
+  dV[3] = -V0[3] / V0[0] * t[0] + t[3] / V0[0];
+  dV[4] = ((-b.e0 + b.kin0) * b.Gamma0 + b.dpdrho0) * t[0] -
+          b.Gamma0 * (V0[1] * t[1] + V0[2] * t[2] + V0[3] * t[3] - t[4]);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructor::Reconstruct (Reconstructor.cpp:335-918).  Slopes are limited
+// in primitive, conservative or characteristic variables
+// (ReconstructionData::VariableType, IoData.h:670-671).
+// ---------------------------------------------------------------------------
+void Solver::ReconstructCell(const std::vector<double>& Vin, long c, int vtype, double* const* fl) {
+  int i, j, k;
+  m.ijk(c, i, j, k);
+  const double* v = at5(Vin, c);
+  const int myid = ID[c];
+  const Eos& E = vf[myid];
+  const bool cons = (vtype == M2CX_VAR_CONSERVATIVE || vtype == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC);
+  const bool chr = (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC || vtype == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC);
+  const std::vector<double>& Q = cons ? Urec : Vin;  // the variables whose slopes are limited
+  const double* q0 = at5(Q, c);
+  long nb[6] = {m.idx(i - 1, j, k), m.idx(i + 1, j, k), m.idx(i, j - 1, k),
+                m.idx(i, j + 1, k), m.idx(i, j, k - 1), m.idx(i, j, k + 1)};
+  // Step 1.1: one-sided differences (:472-574); zero next to an inactive cell
+  double dq[6][5];
+  for (int p = 0; p < 5; p++) {
+    dq[0][p] = (ID[nb[0]] == inactive_id) ? 0.0 : q0[p] - Q[nb[0] * 5 + p];
+    dq[1][p] = (ID[nb[1]] == inactive_id) ? 0.0 : Q[nb[1] * 5 + p] - q0[p];
+    dq[2][p] = (ID[nb[2]] == inactive_id) ? 0.0 : q0[p] - Q[nb[2] * 5 + p];
+    dq[3][p] = (ID[nb[3]] == inactive_id) ? 0.0 : Q[nb[3] * 5 + p] - q0[p];
+    dq[4][p] = (ID[nb[4]] == inactive_id) ? 0.0 : q0[p] - Q[nb[4] * 5 + p];
+    dq[5][p] = (ID[nb[5]] == inactive_id) ? 0.0 : Q[nb[5] * 5 + p] - q0[p];
+  }
+  CharBasis cb[3];
+  if (chr) {
+    for (int a = 0; a < 3; a++)
+      if (!CharSetup(E, v, a, cb[a])) {
+        char b[160];
+        snprintf(b, sizeof b, "c0^2 < 0 in a characteristic transform at (%d,%d,%d): state violates hyperbolicity",
+                 i, j, k);
+        Fail(b);
+        for (int a2 = 0; a2 < 3; a2++) CharSetup(vf[inactive_id], v, a2, cb[a2]);
+        break;
+      }
+    for (int f = 0; f < 6; f++) {
+      double w[5];
+      if (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC) PrimToChar(cb[f / 2], v, dq[f], w);
+      else {
+        double L[5][5];
+        LeftEigenvectors(cb[f / 2], v, L);
+        MatVec5(L, dq[f], w);
+      }
+      for (int p = 0; p < 5; p++) dq[f][p] = w[p];
+    }
+  }
+  // Step 2.2: limited slopes, component by component (:577-611)
+  double alpha = sch.gmm_coeff;
+  double sig[3][5];
+  for (int p = 0; p < 5; p++)
+    for (int a = 0; a < 3; a++) {
+      double A = A3[c * 3 + a], B = B3[c * 3 + a];
+      double d0 = dq[2 * a][p], d1 = dq[2 * a + 1][p];
+      if (sch.limiter == M2CX_LIM_GENERALIZED_MINMOD) sig[a][p] = GeneralizedMinMod(A, B, alpha, d0, d1);
+      else if (sch.limiter == M2CX_LIM_VANALBADA) sig[a][p] = VanAlbada(A, B, K3[c * 3 + a], d0, d1);
+      else sig[a][p] = 0.5 * (d0 + d1);
+    }
+  // Step 2.3: back to primitive / conservative differences (:613-633)
+  if (chr) {
+    for (int a = 0; a < 3; a++) {
+      double w[5];
+      if (vtype == M2CX_VAR_PRIMITIVE_CHARACTERISTIC) CharToPrim(cb[a], v, sig[a], w);
+      else {
+        double R[5][5];
+        RightEigenvectors(cb[a], v, R);
+        MatVec5(R, sig[a], w);
+      }
+      for (int p = 0; p < 5; p++) sig[a][p] = w[p];
+    }
+  }
+  // Step 2.4 (:638-648)
+  if (sch.slope_near_interface == M2CX_SLOPE_ZERO) {
+    for (int a = 0; a < 3; a++)
+      if (ID[c] != ID[nb[2 * a]] || ID[c] != ID[nb[2 * a + 1]])
+        for (int p = 0; p < 5; p++) sig[a][p] = 0.0;
+  }
+  // Step 2.5: face values (:665-714)
+  if (!cons) {
+    for (int p = 0; p < 5; p++) {
+      fl[0][p] = v[p] - 0.5 * sig[0][p];
+      fl[1][p] = v[p] + 0.5 * sig[0][p];
+      fl[2][p] = v[p] - 0.5 * sig[1][p];
+      fl[3][p] = v[p] + 0.5 * sig[1][p];
+      fl[4][p] = v[p] - 0.5 * sig[2][p];
+      fl[5][p] = v[p] + 0.5 * sig[2][p];
+    }
+  } else {
+    for (int f = 0; f < 6; f++) {
+      double u2[5];
+      for (int p = 0; p < 5; p++) u2[p] = (f & 1) ? q0[p] + 0.5 * sig[f / 2][p] : q0[p] - 0.5 * sig[f / 2][p];
+      E.ConservativeToPrimitive(u2, fl[f]);
+    }
+  }
+}
+
 void Solver::Reconstruct(const std::vector<double>& Vin) {
   const long n = m.ncells();
   if (sch.rec_type == M2CX_REC_CONSTANT) {
@@ -219,7 +406,11 @@ void Solver::Reconstruct(const std::vector<double>& Vin) {
     }
     return;
   }
-  double alpha = sch.gmm_coeff;
+  // V -> U over the ghosted domain, each cell with its own material (:400-409)
+  if (sch.var_type == M2CX_VAR_CONSERVATIVE || sch.var_type == M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) {
+    Urec.assign(n * 5, 0.0);
+    for (long c = 0; c < n; c++) vf[ID[c]].PrimitiveToConservative(at5(Vin, c), at5(Urec, c));
+  }
   for (int k = 0; k < m.NZ; k++)
     for (int j = 0; j < m.NY; j++)
       for (int i = 0; i < m.NX; i++) {
@@ -232,44 +423,23 @@ void Solver::Reconstruct(const std::vector<double>& Vin) {
             for (int p = 0; p < 5; p++) fl[f][p] = v[p];
           continue;
         }
-        long nb[6] = {m.idx(i - 1, j, k), m.idx(i + 1, j, k), m.idx(i, j - 1, k),
-                      m.idx(i, j + 1, k), m.idx(i, j, k - 1), m.idx(i, j, k + 1)};
-        double dq[6][5];
-        for (int p = 0; p < 5; p++) {
-          dq[0][p] = (ID[nb[0]] == inactive_id) ? 0.0 : v[p] - Vin[nb[0] * 5 + p];
-          dq[1][p] = (ID[nb[1]] == inactive_id) ? 0.0 : Vin[nb[1] * 5 + p] - v[p];
-          dq[2][p] = (ID[nb[2]] == inactive_id) ? 0.0 : v[p] - Vin[nb[2] * 5 + p];
-          dq[3][p] = (ID[nb[3]] == inactive_id) ? 0.0 : Vin[nb[3] * 5 + p] - v[p];
-          dq[4][p] = (ID[nb[4]] == inactive_id) ? 0.0 : v[p] - Vin[nb[4] * 5 + p];
-          dq[5][p] = (ID[nb[5]] == inactive_id) ? 0.0 : Vin[nb[5] * 5 + p] - v[p];
-        }
-        double sig[3][5];
-        for (int p = 0; p < 5; p++)
-          for (int a = 0; a < 3; a++) {
-            double A = A3[c * 3 + a], B = B3[c * 3 + a];
-            double d0 = dq[2 * a][p], d1 = dq[2 * a + 1][p];
-            if (sch.limiter == M2CX_LIM_GENERALIZED_MINMOD) sig[a][p] = GeneralizedMinMod(A, B, alpha, d0, d1);
-            else if (sch.limiter == M2CX_LIM_VANALBADA) sig[a][p] = VanAlbada(A, B, K3[c * 3 + a], d0, d1);
-            else sig[a][p] = 0.5 * (d0 + d1);
-          }
-        if (sch.slope_near_interface == M2CX_SLOPE_ZERO) {
-          for (int a = 0; a < 3; a++)
-            if (ID[c] != ID[nb[2 * a]] || ID[c] != ID[nb[2 * a + 1]])
-              for (int p = 0; p < 5; p++) sig[a][p] = 0.0;
-        }
-        for (int p = 0; p < 5; p++) {
-          fl[0][p] = v[p] - 0.5 * sig[0][p];
-          fl[1][p] = v[p] + 0.5 * sig[0][p];
-          fl[2][p] = v[p] - 0.5 * sig[1][p];
-          fl[3][p] = v[p] + 0.5 * sig[1][p];
-          fl[4][p] = v[p] - 0.5 * sig[2][p];
-          fl[5][p] = v[p] + 0.5 * sig[2][p];
-        }
-        // Step 2.6: only the first failing face (l,r,b,t,k,f order) reverts
-        // to constant reconstruction (the else-if chain of :722-778)
+        ReconstructCell(Vin, c, sch.var_type, fl);
+        if (fatal) return;
+        // Step 2.6 (:722-778): the else-if chain stops at the first failing
+        // face.  Other variable types retry the whole cell in primitive
+        // variables; in primitive variables that face becomes constant.
         for (int f = 0; f < 6; f++)
           if (vf[myid].CheckState(fl[f])) {
-            for (int p = 0; p < 5; p++) fl[f][p] = v[p];
+            if (sch.var_type != M2CX_VAR_PRIMITIVE) {
+              ReconstructCell(Vin, c, M2CX_VAR_PRIMITIVE, fl);
+              for (int f2 = 0; f2 < 6; f2++)
+                if (vf[myid].CheckState(fl[f2])) {
+                  for (int p = 0; p < 5; p++) fl[f2][p] = v[p];
+                  break;
+                }
+            } else {
+              for (int p = 0; p < 5; p++) fl[f][p] = v[p];
+            }
             break;
           }
       }

@@ -19,6 +19,11 @@ struct Mesh {
   std::vector<double> x, y, z, dx, dy, dz;  // indexed i+1 (ghosts included)
   long idx(int i, int j, int k) const { return ((long)(k + 1) * PY + (j + 1)) * PX + (i + 1); }
   long ncells() const { return (long)PX * PY * PZ; }
+  void ijk(long c, int& i, int& j, int& k) const {
+    i = (int)(c % PX) - 1;
+    j = (int)((c / PX) % PY) - 1;
+    k = (int)(c / ((long)PX * PY)) - 1;
+  }
   bool inside(int i, int j, int k) const { return i >= 0 && i < NX && j >= 0 && j < NY && k >= 0 && k < NZ; }
   double X(int i) const { return x[i + 1]; }
   double Y(int j) const { return y[j + 1]; }
@@ -59,6 +64,7 @@ class Solver {
   std::vector<double> riem_log;
   // fields (AoS like SpaceVariable3D, ghost width 1)
   std::vector<double> V, V1, V2, Un, U1, R, Vl, Vr, Vb, Vt, Vk, Vf, vol;
+  std::vector<double> Urec;  // conservative copy of V for conservative reconstruction
   std::vector<double> A3, B3;  // Reconstructor CoeffA/CoeffB (3 per cell)
   std::vector<int> K3;         // CoeffK
   std::vector<int> ID, IDn;
@@ -78,6 +84,7 @@ class Solver {
 
   // SpaceOperator
   void Reconstruct(const std::vector<double>& Vin);
+  void ReconstructCell(const std::vector<double>& Vin, long c, int vtype, double* const* fl);
   void CheckReconstructedStates(const std::vector<double>& Vin);
   void ComputeAdvectionFluxes(const std::vector<double>& Vin, std::vector<double>& F, bool capture);
   void ComputeResidual(const std::vector<double>& Vin, std::vector<double>& Rout, bool capture);

@@ -11,24 +11,41 @@ BC_NAMES = {"Inlet": abi.BC_INLET, "Farfield": abi.BC_INLET, "Inlet2": abi.BC_IN
             "Outlet": abi.BC_OUTLET}
 FLUX = {"Roe": abi.FLUX_ROE, "LocalLaxFriedrichs": abi.FLUX_LLF, "HLLC": abi.FLUX_HLLC}
 LIM = {"None": abi.LIM_NONE, "GeneralizedMinMod": abi.LIM_GMM, "VanAlbada": abi.LIM_VANALBADA}
+VAR = {"Primitive": abi.VAR_PRIMITIVE, "Conservative": abi.VAR_CONSERVATIVE,
+       "PrimitiveCharacteristic": abi.VAR_PRIMITIVE_CHARACTERISTIC,
+       "ConservativeCharacteristic": abi.VAR_CONSERVATIVE_CHARACTERISTIC}
 TI = {"ForwardEuler": abi.TI_FE, "RungeKutta2": abi.TI_RK2, "RungeKutta3": abi.TI_RK3}
+
+
+# MaterialModelData keys (IoData.cpp:1074-1080).  The deck parser resolves a
+# name in the innermost enclosing "under" block that knows it (Parser.y:66-76),
+# so these may also be written inside the EOS sub-block.
+_CUTOFFS = {"DensityCutOff": "rhomin", "PressureCutOff": "pmin", "DensityUpperLimit": "rhomax",
+            "PressureUpperLimit": "pmax", "DensityPrescribedAtFailure": "failsafe_density"}
 
 
 def _mat(md):
     eos = md.get("EquationOfState", "StiffenedGas")
     if eos == "StiffenedGas":
-        sg = md.get("StiffenedGasModel", {})
-        return abi.material("SG", gamma=sg.get("SpecificHeatRatio", 1.4), pc=sg.get("PressureConstant", 0.0))
-    if eos == "NobleAbelStiffenedGas":
-        sg = md.get("NobleAbelStiffenedGasModel", {})
-        return abi.material("NASG", gamma=sg.get("SpecificHeatRatio", 1.4), pc=sg.get("PressureConstant", 0.0),
-                            b=sg.get("VolumeConstant", 0.0), q=sg.get("EnergyConstant", 0.0))
-    if eos == "MieGruneisen":
-        mg = md.get("MieGruneisenModel", {})  # defaults IoData.cpp:708-724
-        return abi.material("MG", rho0=mg.get("ReferenceDensity", 8.96e-3), c0=mg.get("BulkSpeedOfSound", 3.933e6),
-                            Gamma0=mg.get("ReferenceGamma", 1.99), s=mg.get("HugoniotSlope", 1.5),
-                            e0=mg.get("ReferenceSpecificInternalEnergy", 0.0))
-    raise NotImplementedError(eos)
+        sub = md.get("StiffenedGasModel", {})
+        m = abi.material("SG", gamma=sub.get("SpecificHeatRatio", 1.4), pc=sub.get("PressureConstant", 0.0))
+    elif eos == "NobleAbelStiffenedGas":
+        sub = md.get("NobleAbelStiffenedGasModel", {})
+        m = abi.material("NASG", gamma=sub.get("SpecificHeatRatio", 1.4), pc=sub.get("PressureConstant", 0.0),
+                         b=sub.get("VolumeConstant", 0.0), q=sub.get("EnergyConstant", 0.0))
+    elif eos == "MieGruneisen":
+        sub = md.get("MieGruneisenModel", {})  # defaults IoData.cpp:708-724
+        m = abi.material("MG", rho0=sub.get("ReferenceDensity", 8.96e-3), c0=sub.get("BulkSpeedOfSound", 3.933e6),
+                         Gamma0=sub.get("ReferenceGamma", 1.99), s=sub.get("HugoniotSlope", 1.5),
+                         e0=sub.get("ReferenceSpecificInternalEnergy", 0.0))
+    else:
+        raise NotImplementedError(eos)
+    for key, field in _CUTOFFS.items():
+        if key in sub:
+            setattr(m, field, sub[key])
+        if key in md:
+            setattr(m, field, md[key])
+    return m
 
 
 def supported_eos(deck):
@@ -70,6 +87,7 @@ def setup(deck):
     s.rec_type = abi.REC_LINEAR if rec.get("Type", "Linear") == "Linear" else abi.REC_CONSTANT
     s.limiter = LIM[rec.get("Limiter", "GeneralizedMinMod")]
     s.gmm_coeff = rec.get("GeneralizedMinModCoefficient", 1.2)
+    s.var_type = VAR[rec.get("VariableType", "Primitive")]
     s.slope_near_interface = abi.SLOPE_ZERO if rec.get("SlopeNearInterface", "NonZero") == "Zero" else abi.SLOPE_NONZERO
     s.entropy_fix = ns.get("EntropyFixCoefficient", 0.2)
     s.integrator = TI[ti.get("Explicit", {}).get("Type", "RungeKutta2")]
@@ -123,3 +141,15 @@ def _axis(N, a0, a1):
     c[0] = c[1] - h
     c[N + 1] = c[N] + h
     return c
+
+
+def probe_rows(deck, g, V, ID, Phi):
+    """The deck's first line plot (Output.LinePlot[0]) sampled from a state in
+    the reference's format: rows of coordinate, rho, u, v, w, p, T, ID, phi."""
+    from m2c_amd import output
+    lp = deck["Output"]["LinePlot[0]"]
+    p0 = [lp.get("X0", 0.0), lp.get("Y0", 0.0), lp.get("Z0", 0.0)]
+    p1 = [lp.get("Xmax", 0.0), lp.get("Ymax", 0.0), lp.get("Zmax", 0.0)]
+    _, rows = output.line_plot_rows(g, V, np.asarray(ID, dtype=np.float64), [Phi], p0, p1,
+                                    int(lp["NumberOfPoints"]))
+    return np.array(rows)

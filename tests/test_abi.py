@@ -52,7 +52,7 @@ def test_argument_errors_need_no_gpu():
     assert isinstance(lib.m2cx_last_error(None), bytes)
 
 
-_PREFIXES = ("EOS_", "BC_", "LSBC_", "FLUX_", "REC_", "LIM_", "SLOPE_", "TI_", "MPFLUX_", "MPREC_", "GUESS_",
+_PREFIXES = ("EOS_", "BC_", "LSBC_", "FLUX_", "REC_", "LIM_", "SLOPE_", "VAR_", "TI_", "MPFLUX_", "MPREC_", "GUESS_",
              "FIELD_", "E_", "GEOM_")
 
 

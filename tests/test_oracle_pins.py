@@ -173,34 +173,110 @@ def test_exact_fixture_with_rarefaction(fixtures, name):
 
 
 # --------------------------------------------------------------------------
-# full pipeline: Test1 (Roe, van Albada) and Test6 (Roe, GMM 2.0), RK2,
-# level set FD3, exact interface Riemann with linear reconstruction,
-# RiemannSolution phase change, Inlet/Inlet2/Symmetry boundaries.
+# full pipeline against the reference's line plots: RK2/RK3, level set FD3,
+# exact interface Riemann, RiemannSolution phase change, Inlet/Inlet2/
+# Symmetry boundaries, and the four reconstruction variable types:
+# Primitive (Test1 van Albada, Test6 GMM 2.0), ConservativeCharacteristic
+# (Test2, Test3 Roe GMM; Test8 HLLC van Albada with MG/MG; Test9, Test10 RK3
+# MG/MG), Conservative (Test5 HLLC GMM 1.0).  The tolerance is a few times
+# the deviation seen, relative to max|column|; the files print 9 digits.
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("name,tol", [("Test1", 3e-8), ("Test6", 3e-9)])
-def test_line_fixture_full_pipeline(fixtures, name, tol):
-    d = fixtures[name]["deck"]
+LINE_TOL = {"Test1": 5e-8, "Test2": 1.5e-7, "Test3": 2e-8, "Test5": 1e-6, "Test6": 3e-9, "Test8": 6e-7,
+            "Test9": 4e-8, "Test10": 8e-8}
+
+
+def _replay(fx, name, nmax=None):
+    d = fx[name]["deck"]
     g, mats, nmat, s, V, ID, Phi = dk.setup(d)
     o = ol.Oracle(g, mats, nmat, s)
     o.set_state(V, ID, Phi)
     o.apply_bc()
     o.update_ghost_ids()
     o.ls_apply_bc()
-    steps = fixtures[name]["log_steps"]
+    steps = fx[name]["log_steps"]
     dts = []
-    for _ in range(len(steps) + 3):
+    for _ in range(len(steps) + 3 if nmax is None else nmax):
         o.run_steps(1)
         t, dt, st = o.time()
         dts.append(dt)
         if t >= s.max_time:
             break
+    return d, g, o, np.array(dts), st
+
+
+@pytest.mark.parametrize("name", sorted(LINE_TOL))
+def test_line_fixture_full_pipeline(fixtures, name):
+    d, g, o, dts, st = _replay(fixtures, name)
+    steps = fixtures[name]["log_steps"]
     assert st == len(steps)
     ref_dt = np.array([x[2] for x in steps])
-    assert np.max(np.abs(np.array(dts) - ref_dt) / ref_dt) < 6e-7  # %e prints 7 digits
+    # the last step is clamped to MaxTime - t, a difference of printed times
+    assert np.max(np.abs(dts[:-1] - ref_dt[:-1]) / ref_dt[:-1]) < 6e-7  # %e prints 7 digits
     Vf, IDf, Phif = o.get_state()
     rows = np.array(fixtures[name]["line_rows"])
-    got = {1: Vf[1, 1, 1:-1, 0], 2: Vf[1, 1, 1:-1, 1], 5: Vf[1, 1, 1:-1, 4], 8: Phif[1, 1, 1:-1]}
-    for col, mine in got.items():
+    mine = dk.probe_rows(d, g, Vf, IDf, Phif)
+    for col in (1, 2, 5, 7, 8):  # rho, u, p, material ID, phi
         ref = rows[:, col]
-        assert np.max(np.abs(mine - ref)) <= tol * np.max(np.abs(ref)), col
-    assert np.array_equal(IDf[1, 1, 1:-1], rows[:, 7].astype(np.int32))
+        assert np.max(np.abs(mine[:, col] - ref)) <= LINE_TOL[name] * np.max(np.abs(ref)), (name, col)
+
+
+# PrimitiveCharacteristic (Test11 LLF with exact interface fluxes, Test12 LLF
+# SG/MG) next to vacuum: the solid pulls away and its pressure sits on the
+# cut-off, so clipping decisions amplify rounding differences (built with FMA
+# contraction, the oracle's time steps already leave these logs by step 10).
+# The time steps are pinned while the runs are still deterministic; the
+# transform pair itself is pinned by the invariants below.
+@pytest.mark.parametrize("name,nsteps", [("Test11", 11), ("Test12", 4)])
+def test_primitive_characteristic_fixture_early_steps(fixtures, name, nsteps):
+    d, g, o, dts, st = _replay(fixtures, name, nsteps)
+    ref_dt = np.array([x[2] for x in fixtures[name]["log_steps"][:nsteps]])
+    assert np.max(np.abs(dts - ref_dt) / ref_dt) < 6e-7
+
+
+def _smooth_state(n, rho0, p0, seed):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    x = np.linspace(0, 1, nx + 2)[None, None, :]
+    y = np.linspace(0, 1, ny + 2)[None, :, None]
+    z = np.linspace(0, 1, nz + 2)[:, None, None]
+    V = np.zeros((nz + 2, ny + 2, nx + 2, 5))
+    ph = rng.uniform(0, 1, 4)
+    V[..., 0] = rho0 * (1.0 + 0.2 * np.sin(5 * x + 2 * y + ph[0]) * np.cos(3 * z))
+    V[..., 1] = 0.3 * np.cos(4 * y + z + ph[1])
+    V[..., 2] = -0.2 * np.sin(3 * x + 2 * z + ph[2])
+    V[..., 3] = 0.25 * np.cos(6 * x * y + ph[3])
+    V[..., 4] = p0 * (1.0 + 0.3 * np.cos(4 * z + x))
+    return V
+
+
+@pytest.mark.parametrize("eos", ["SG", "NASG", "MG"])
+def test_characteristic_transforms_cancel_without_limiter(eos):
+    """Without a limiter the slope is linear in the differences, so limiting
+    in characteristic variables (L Q dV, then Q^-1 R sigma; L dU, then R
+    sigma) must give the primitive (conservative) residual up to rounding:
+    a wrong entry of L, R, Q or Q^-1 breaks the cancellation."""
+    if eos == "SG":
+        mat, rho0, p0 = abi.material("SG", gamma=1.4, pc=0.5), 1.0, 1.0
+    elif eos == "NASG":  # order-one parameters keep L and R well conditioned
+        mat, rho0, p0 = abi.material("NASG", gamma=1.19, pc=2.0, b=0.1, q=-0.5), 1.0, 1.0
+    else:
+        mat, rho0, p0 = abi.material("MG", rho0=1.0, c0=2.0, s=1.5, Gamma0=2.0), 1.05, 1.0
+    n = (10, 8, 6)
+    g = abi.grid(n, (0, 0, 0), (1.0, 0.8, 0.6))
+    mats = abi.materials(mat)
+    V = _smooth_state(n, rho0, p0, 5)
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    res = {}
+    for vt in range(4):
+        s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_NONE, var_type=vt)
+        o = ol.Oracle(g, mats, 1, s)
+        o.set_state(V, ID)
+        o.apply_bc()
+        res[vt] = o.residual()[1:-1, 1:-1, 1:-1]
+    scale = np.max(np.abs(res[0]), axis=(0, 1, 2))
+    for a, b in ((abi.VAR_PRIMITIVE_CHARACTERISTIC, abi.VAR_PRIMITIVE),
+                 (abi.VAR_CONSERVATIVE_CHARACTERISTIC, abi.VAR_CONSERVATIVE)):
+        err = np.max(np.abs(res[a] - res[b]), axis=(0, 1, 2)) / scale
+        assert np.all(err < 1e-9), (eos, a, err)
+    # and the variable types differ from each other (the test is not vacuous)
+    assert np.max(np.abs(res[abi.VAR_CONSERVATIVE] - res[abi.VAR_PRIMITIVE])) > 1e-6 * np.max(scale)
