@@ -1003,7 +1003,7 @@ struct m2cx_ensemble {
   int nriem = 0;              // members with interface faces
   int* doff = nullptr;
   double* slab = nullptr;
-  int lanes = RIEM_SLOTS, pathcap = 512;  // integration-path scratch: one slot per wave
+  int lanes = RIEM_LANES, pathcap = 512;  // integration-path scratch: one slot per lane
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::string err;
@@ -1035,8 +1035,8 @@ static void ensemble_step(m2cx_ensemble* e) {
     }
     if (e->nriem) {
       k_riem_offsets<<<1, 1, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff);
-      k_riemann_multi<<<RIEM_SLOTS, 64, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1, e->slab,
-                                                        e->lanes, e->pathcap);
+      k_riemann_multi<<<e->lanes / 64, 64, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1, e->slab,
+                                                           e->lanes, e->pathcap);
       k_riem_count<<<e->nriem, 256, 0, e->stream>>>(e->dmem[q]);
     }
     for (m2cx_solver* h : e->m) stage_after_riemann(h, sp[q]);
@@ -1078,7 +1078,7 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
       EHIPCHK(e, hipMalloc(&e->dmem[q], tab[q].size() * sizeof(RiemMember)));
       EHIPCHK(e, hipMemcpy(e->dmem[q], tab[q].data(), tab[q].size() * sizeof(RiemMember), hipMemcpyHostToDevice));
     }
-    EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 2) * sizeof(int)));
+    EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 1) * sizeof(int)));
     EHIPCHK(e, hipMalloc(&e->slab, (long)e->lanes * 2 * e->pathcap * 3 * 8));
   }
   for (m2cx_solver* h : e->m)

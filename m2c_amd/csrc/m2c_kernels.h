@@ -15,7 +15,6 @@ struct DevCounters {
   unsigned int dt_ticket;
   int nif;            // interface faces in the current list
   int n_remaining, n_changed, n_unresolved, pad;
-  unsigned int riem_next, pad3;  // face queue of the interface Riemann launch
   double t, dt, cfl, dts, tmax;
   int time_step, pad2;
 };

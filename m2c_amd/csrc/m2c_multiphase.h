@@ -18,6 +18,11 @@
 
 namespace m2c {
 
+// Lanes of an interface Riemann launch: 1024 waves, one per SIMD of the 256
+// CUs (the exact solver's register use leaves room for one resident wave per
+// SIMD).  Each lane owns 2 x 512 integration-path points (24 KiB).
+constexpr int RIEM_LANES = 1024 * 64;
+
 struct MPBuffers {
   int* fidx[3] = {};
   IfaceList L{};
@@ -130,7 +135,7 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
                     sch.riem_min_pressure, sch.riem_failure_threshold, sch.riem_pressure_at_failure};
   B.guess = sch.riem_guess;
   // per-lane integration-path scratch: lanes x 2 paths x cap points x 3
-  B.lanes = 256 * 128;
+  B.lanes = RIEM_LANES;
   B.pathcap = 512;
   // the integration-path slab is allocated on first use (mp_slab): solvers
   // that run in an ensemble share the ensemble's slab instead
@@ -269,36 +274,29 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   return flags;
 }
 
-// One wave per interface face.  The exact solver's iteration counts differ
-// from face to face by two orders of magnitude, and faces sharing a wave run
-// the union of their control paths one after another; with one face per wave
-// (all 64 lanes evaluate it, the solve being a single serial chain) every
-// face runs at the speed of its own chain.  A wave takes the next face from
-// the queue cnt->riem_next (reset by k_riem_queue_reset before the launch)
-// until the list is exhausted; `slot` is the wave's integration-path scratch.
-constexpr int RIEM_SLOTS = 2048;  // waves of one interface launch
-
-__global__ void k_riem_queue_reset(DevCounters* cnt) { cnt->riem_next = 0; }
-
-__device__ __forceinline__ int riem_next_face(unsigned int* queue) {
-  int q = 0;
-  if (threadIdx.x == 0) q = (int)atomicAdd(queue, 1u);
-  return __shfl(q, 0);
-}
-
+// One lane per interface face, the faces dealt across the waves: face q goes
+// to wave q % W, lane (q / W) % 64 (W waves of 64 lanes).  The exact solver's
+// work per face varies by orders of magnitude and with it its control flow
+// (faces in near equilibrium run the secant loop to its iteration limit, the
+// rarefaction side being flat within the integration tolerance; strong ones
+// spend their time in long Hugoniot and rarefaction solves), and lanes of a
+// wave that follow different paths run one after another.  Neighbouring faces
+// tend to need the same kind of solve; dealt out this way every wave holds a
+// small, even share of the expensive faces.  Each lane owns slot `gid` of the
+// integration-path scratch (stride: all lanes of the launch).
 __global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
                                                 const unsigned char* __restrict__ ID, IfaceList L, int stage1,
-                                                double* slab, int stride, int pathcap, const float* guess,
+                                                double* slab, int lanes, int pathcap, const float* guess,
                                                 DevCounters* cnt) {
-  const int slot = blockIdx.x;
+  const int W = gridDim.x, wave = blockIdx.x, lane = threadIdx.x;
+  const int gid = wave * 64 + lane;
+  if (gid >= lanes) return;
   const int nif = min(cnt->nif, L.cap);
-  DevRiemann R(c_mats, rc, slab + slot, stride, pathcap);
+  DevRiemann R(c_mats, rc, slab + gid, lanes, pathcap);
   unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
   int fatal = 0, ovf = 0;
-  for (;;) {
-    const int q = riem_next_face(&cnt->riem_next);
-    if (q >= nif) break;
-    const int f = riemann_face(g, s, V, ID, L, q, stage1, guess, R, fatal, ovf);
+  for (long q = (long)lane * W + wave; q < nif; q += 64L * W) {
+    const int f = riemann_face(g, s, V, ID, L, (int)q, stage1, guess, R, fatal, ovf);
     calls++;
     triv += f & 1;
     errs += (f >> 1) & 1;
@@ -307,7 +305,6 @@ __global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg 
     its += R.st.it_main;
     fev += R.st.fevals;
   }
-  if (threadIdx.x != 0) return;
   if (calls) {
     atomicAdd(&cnt->riemann_calls, calls);
     atomicAdd(&cnt->riemann_trivial, triv);
@@ -338,8 +335,6 @@ struct RiemMember {
   DevCounters* cnt;
 };
 
-// off[m]: first face of member m in the concatenated list; off[nm]: total;
-// off[nm + 1]: the face queue of k_riemann_multi, reset here
 __global__ void k_riem_offsets(const RiemMember* mem, int nm, int* off) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int acc = 0;
@@ -348,18 +343,17 @@ __global__ void k_riem_offsets(const RiemMember* mem, int nm, int* off) {
     acc += min(mem[m].cnt->nif, mem[m].L.cap);
   }
   off[nm] = acc;
-  off[nm + 1] = 0;
 }
 
-// one wave per face over the concatenated list, as k_riemann
-__global__ void __launch_bounds__(64) k_riemann_multi(const RiemMember* __restrict__ mem, int nm, int* off,
-                                                      int stage1, double* slab, int stride, int pathcap) {
-  const int slot = blockIdx.x;
+// the concatenated list dealt across waves as in k_riemann
+__global__ void __launch_bounds__(64) k_riemann_multi(const RiemMember* __restrict__ mem, int nm,
+                                                      const int* __restrict__ off, int stage1, double* slab,
+                                                      int lanes, int pathcap) {
+  const int W = gridDim.x, wave = blockIdx.x, lane = threadIdx.x;
+  const int gid = wave * 64 + lane;
+  if (gid >= lanes) return;
   const int total = off[nm];
-  int fatal = 0, ovf = 0;
-  for (;;) {
-    const int q = riem_next_face(reinterpret_cast<unsigned int*>(off + nm + 1));
-    if (q >= total) break;
+  for (long q = (long)lane * W + wave; q < total; q += 64L * W) {
     int lo = 0, hi = nm - 1;  // member m with off[m] <= q < off[m+1]
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -367,14 +361,11 @@ __global__ void __launch_bounds__(64) k_riemann_multi(const RiemMember* __restri
       else hi = mid - 1;
     }
     const RiemMember& B = mem[lo];
-    DevRiemann R(c_mats, B.rc, slab + slot, stride, pathcap);
-    fatal = 0;
-    ovf = 0;
-    riemann_face(B.g, B.s, B.V, B.ID, B.L, q - off[lo], stage1, B.guess, R, fatal, ovf);
-    if (threadIdx.x == 0) {
-      if (fatal) atomicOr(&B.cnt->error, 2);
-      if (ovf) atomicOr(&B.cnt->error, 4);
-    }
+    DevRiemann R(c_mats, B.rc, slab + gid, lanes, pathcap);
+    int fatal = 0, ovf = 0;
+    riemann_face(B.g, B.s, B.V, B.ID, B.L, (int)q - off[lo], stage1, B.guess, R, fatal, ovf);
+    if (fatal) atomicOr(&B.cnt->error, 2);
+    if (ovf) atomicOr(&B.cnt->error, 4);
   }
 }
 
@@ -999,9 +990,8 @@ inline int mp_slab(MPBuffers& B) {
 inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, const unsigned char* ID, int stage1,
                        DevCounters* cnt, hipStream_t st) {
   if (s.mp_flux == M2CX_MPFLUX_LLF) return;
-  k_riem_queue_reset<<<1, 1, 0, st>>>(cnt);
-  k_riemann<<<RIEM_SLOTS, 64, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
-                                       B.guess ? B.pguess : nullptr, cnt);
+  k_riemann<<<B.lanes / 64, 64, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
+                                         B.guess ? B.pguess : nullptr, cnt);
 }
 
 // plain copies as kernels (no memcpy nodes when a step is captured in a graph)

@@ -46,23 +46,22 @@ struct RStats {
   bool trivial, fallback, fatal;
 };
 
-// Out-of-line EOS evaluations.  The solver evaluates e(rho, p), c^2 and the
-// Hugoniot function at dozens of call sites; lanes of a wave that follow
-// different branches walk through all of them, so one shared body per
-// function (instead of an inlined copy per site, each with every EOS kind)
-// keeps the instruction footprint of the kernel small.
+// EOS evaluations of the solver: e(rho, p) with c^2, e alone, and the
+// Hugoniot function.  Inlined: as out-of-line calls every evaluation saved
+// and restored live registers through scratch memory, which dominated the
+// latency of the solver's serial chain.
 struct EC2 {
   double e, c2;
 };
-__device__ __noinline__ EC2 ool_ec2(const DevEos* E, double rho, double p) {
+__device__ __forceinline__ EC2 ool_ec2(const DevEos* E, double rho, double p) {
   EC2 r;
   r.e = eos_e(*E, rho, p);
   r.c2 = eos_c2(*E, rho, r.e);
   return r;
 }
-__device__ __noinline__ double ool_e(const DevEos* E, double rho, double p) { return eos_e(*E, rho, p); }
+__device__ __forceinline__ double ool_e(const DevEos* E, double rho, double p) { return eos_e(*E, rho, p); }
 // HugoniotEquation (ExactRiemannSolverBase.h:66-83)
-__device__ __noinline__ double ool_hugo(const DevEos* E, double r, double ps, double he, double pavg, double oor) {
+__device__ __forceinline__ double ool_hugo(const DevEos* E, double r, double ps, double he, double pavg, double oor) {
   return eos_e(*E, r, ps) - he + pavg * (1.0 / r - oor);
 }
 
@@ -508,6 +507,9 @@ struct DevRiemann {
   }
 
   // ----------------------------------------------------------- bracketing
+  // wave strength from which the learned p* guess replaces the acoustic one
+  static constexpr double LEARNED_MIN_STRENGTH = 0.3;
+
   // FindInitialFeasiblePointsByAcousticTheory (:1107-1158); `p0g` replaces the
   // acoustic p0 when positive (learned guess), see m2cx_scheme.riem_guess.
   __device__ int acoustic(double rhol, double ul, double pl, double cl, int idl, double rhor, double ur, double pr,
@@ -516,10 +518,14 @@ struct DevRiemann {
     int found = 0;
     double Cl = rhol * cl, Cr = rhor * cr;
     p0 = (Cr * pl + Cl * pr + Cl * Cr * (ul - ur)) / (Cl + Cr);
-    // learned guess: acoustic theory is exact to second order for weak waves
-    // and more precise there than a BF16 network, so the prediction replaces
-    // it only where the two disagree by more than the network's precision
-    if (p0g > 0 && !(p0 > 0 && fabs(log(p0g / p0)) <= 0.02)) p0 = p0g;
+    // learned guess: acoustic theory is exact to second order in the wave
+    // strength |p0 - p| / (rho c^2) and beats the network where both waves
+    // are weak; the prediction replaces it for strong waves and where the
+    // acoustic pressure is not positive
+    if (p0g > 0) {
+      const double sl = fabs(p0 - pl) / (Cl * cl), sr = fabs(p0 - pr) / (Cr * cr);
+      if (!(p0 > 0) || !(fmax_std(sl, sr) < LEARNED_MIN_STRENGTH)) p0 = p0g;
+    }
     if (!rhoustar(1, path1, rhol, ul, pl, p0, idl, rhol, (p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0)) return found;
     if (!rhoustar(3, path3, rhor, ur, pr, p0, idr, rhor, (p0 > pr) ? rhor * 1.1 : rhor * 0.9, rhor0, ur0)) return found;
     found = 1;
