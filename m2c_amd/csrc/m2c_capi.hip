@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -33,6 +34,7 @@ struct m2cx_solver {
   DevGrid g{};
   Scheme s{};
   DevMats M{};
+  int mset = -1;  // slot of M in c_mats
   long ncells = 0;
   // fields
   double* V[3][5] = {};   // V, V1, V2
@@ -72,6 +74,46 @@ struct m2cx_solver {
 static int fail(m2cx_solver* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
+}
+
+// Slots of c_mats per device: solvers with byte-identical material tables
+// share one; a slot is rewritten only when it is free (no live solver uses it).
+struct MatSlot {
+  DevMats m;
+  int refs = 0;
+};
+static std::mutex g_slot_mu;
+static std::map<int, std::vector<MatSlot>> g_slots;
+
+static int mat_slot_acquire(m2cx_solver* h) {
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  std::vector<MatSlot>& v = g_slots[h->device];
+  int slot = -1;
+  for (int i = 0; i < (int)v.size() && slot < 0; i++)
+    if (v[i].refs > 0 && std::memcmp(&v[i].m, &h->M, sizeof(DevMats)) == 0) slot = i;
+  if (slot < 0)
+    for (int i = 0; i < (int)v.size() && slot < 0; i++)
+      if (v[i].refs == 0) slot = i;
+  if (slot < 0 && (int)v.size() < MAX_MAT_SETS) {
+    v.emplace_back();
+    slot = (int)v.size() - 1;
+  }
+  if (slot < 0) return fail(h, M2CX_E_CAPACITY, "too many distinct material tables on one device");
+  if (v[slot].refs == 0) {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_mats), &h->M, sizeof(DevMats), slot * sizeof(DevMats));
+    if (e != hipSuccess) return fail(h, M2CX_E_HIP, std::string("material table upload: ") + hipGetErrorString(e));
+    v[slot].m = h->M;
+  }
+  v[slot].refs++;
+  h->mset = slot;
+  return 0;
+}
+
+static void mat_slot_release(m2cx_solver* h) {
+  if (h->mset < 0) return;
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  g_slots[h->device][h->mset].refs--;
+  h->mset = -1;
 }
 
 #define HIPCHK(h, x)                                                                  \
@@ -204,12 +246,14 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   s.gmm = sch->gmm_coeff; s.del = sch->entropy_fix; s.cfl = sch->cfl;
   s.var_type = sch->var_type;
   DevMats& M = h->M;
+  std::memset(&M, 0, sizeof M);
   M.nmat = nmat;
   for (int q = 0; q < nmat; q++) M.m[q] = make_eos(mats[q]);
   m2cx_material dm{};
   dm.eos = M2CX_EOS_DUMMY;
   M.m[nmat] = make_eos(dm);
-  HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_mats), &M, sizeof(DevMats)));
+  if (int rc = mat_slot_acquire(h)) return rc;
+  s.mset = h->mset;
   // fields
   const long n = h->ncells;
   auto field = [&](double** comp) -> int {
@@ -263,6 +307,7 @@ int m2cx_destroy(m2cx_solver* h) {
   hipFree(h->geo); hipFree(h->coefK);
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
+  mat_slot_release(h);
   for (void* p : {(void*)h->mlp.theta, (void*)h->mlp.m, (void*)h->mlp.v, (void*)h->mlp.wt1, (void*)h->mlp.wt2,
                   (void*)h->mlp.wt3, (void*)h->mlp.wt4, (void*)h->mlp.w2, (void*)h->mlp.w3, (void*)h->mlp.guess})
     hipFree(p);
@@ -570,7 +615,7 @@ int m2cx_clip(m2cx_solver* h, int v_field) {
   double** f = field_ptr(h, v_field);
   if (!f) return M2CX_E_ARG;
   k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
-  k_clip<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, f5(h, f), h->ID, h->cnt);
+  k_clip<<<grid_blocks((long)h->g.NX * h->g.NY * h->g.NZ), 256, 0, h->stream>>>(h->g, h->s.mset, f5(h, f), h->ID, h->cnt);
   int rc = check_error(h);
   if (rc) return rc;
   DevCounters c;
@@ -603,7 +648,8 @@ static void stage_after_riemann(m2cx_solver* h, const StageSpec& sp) {
   if (sp.resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
   launch_stage(h, sp.vin, sp.vout, sp.mode);
   if (sp.resync)
-    k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, cf5(h, h->V[sp.vout]), f5(h, h->U1), h->ID, h->cnt);
+    k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s.mset, cf5(h, h->V[sp.vout]), f5(h, h->U1), h->ID,
+                                                       h->cnt);
   levelset_stage(h, sp.vin, sp.ls_which);
 }
 
@@ -761,7 +807,7 @@ int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir, const double* V
     HIPCHK(h, hipMalloc(&P, N * 8));
     HIPCHK(h, hipMalloc(&pc, N * 8));
     HIPCHK(h, hipMalloc(&ok, N * 4));
-    k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(n, dd, dVm, didm, dVp, didp, nullptr, X, P, pc, nullptr,
+    k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(h->s.mset, n, dd, dVm, didm, dVp, didp, nullptr, X, P, pc, nullptr,
                                                            ok);
     k_mlp_predict<<<(n + 15) / 16, 64, 0, h->stream>>>(n, X, mlp_dev(h), y);
     k_mlp_compose<<<(n + 255) / 256, 256, 0, h->stream>>>(n, y, P, pc, ok, dguess);
@@ -769,7 +815,7 @@ int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir, const double* V
     hipFree(X); hipFree(y); hipFree(P); hipFree(pc); hipFree(ok);
   }
   MPBuffers& B = h->mp;
-  k_riemann_batch<<<(B.lanes + 127) / 128, 128, 0, h->stream>>>(B.rc, n, dd, dVm, didm, dVp, didp, dVs, did, dVsm,
+  k_riemann_batch<<<(B.lanes + 127) / 128, 128, 0, h->stream>>>(h->s.mset, B.rc, n, dd, dVm, didm, dVp, didp, dVs, did, dVsm,
                                                                  dVsp, derr, dit, dfe, B.slab, B.lanes, B.pathcap,
                                                                  dguess, h->cnt);
   HIPCHK(h, hipGetLastError());
@@ -854,7 +900,7 @@ int m2cx_mlp_features(m2cx_solver* h, int n, const double* dir, const double* Vm
   HIPCHK(h, hipMemcpy(dVp, Vp, N * 40, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(didm, idm, N * 4, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(didp, idp, N * 4, hipMemcpyHostToDevice));
-  k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(n, dd, dVm, didm, dVp, didp, dps, dX, P, pc, dT, dok);
+  k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(h->s.mset, n, dd, dVm, didm, dVp, didp, dps, dX, P, pc, dT, dok);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(X, dX, N * 32, hipMemcpyDeviceToHost));

@@ -58,6 +58,7 @@ struct Scheme {
   int flux, rec_type, limiter, slope_zero, integrator;
   int mp_flux, mp_recon, nmat, inactive;
   int var_type;                 // M2CX_VAR_*
+  int mset;                     // slot of the solver's material table in c_mats
   double gmm, del, cfl;
   // per cell (ghosted index): 1 where a reconstruction in var_type fails
   // CheckState on some face and the cell falls back to primitive variables

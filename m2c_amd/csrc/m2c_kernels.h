@@ -127,7 +127,10 @@ struct TileShape {
   }
 };
 
-__constant__ DevMats c_mats;
+// Material tables of the live solvers, one slot per distinct table (solvers
+// with equal tables share a slot; Scheme::mset selects it).
+constexpr int MAX_MAT_SETS = 32;
+__constant__ DevMats c_mats[MAX_MAT_SETS];
 
 // GEN: reconstruction in any variable type (false: primitive variables only,
 // the common case, compiled without the characteristic code)
@@ -139,7 +142,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   __shared__ int sMask[TS::NT];
   const DevGrid& g = a.g;
   const Scheme& s = a.s;
-  const DevMats& M = c_mats;
+  const DevMats& M = c_mats[a.s.mset];
   const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
   const int t = threadIdx.x;
   // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
@@ -297,7 +300,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
 // transform is the reference's abort (FluxFcnBase.h:315-320).
 __global__ void k_recon_flags(DevGrid g, Scheme s, CF5 V, const unsigned char* __restrict__ ID,
                               unsigned char* __restrict__ flag, DevCounters* cnt) {
-  const DevMats& M = c_mats;
+  const DevMats& M = c_mats[s.mset];
   const long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     int i, j, k;
@@ -328,7 +331,7 @@ __global__ void k_recon_flags(DevGrid g, Scheme s, CF5 V, const unsigned char* _
 
 // U = P2C(V) on interior cells, only if the preceding stage clipped a cell
 // (TimeIntegrator.cpp:569-572, 622-625).
-__global__ void k_resync(DevGrid g, CF5 V, F5 U, const unsigned char* __restrict__ ID, const DevCounters* cnt) {
+__global__ void k_resync(DevGrid g, int mset, CF5 V, F5 U, const unsigned char* __restrict__ ID, const DevCounters* cnt) {
   if (cnt->stage_clipped == 0) return;
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
@@ -337,7 +340,7 @@ __global__ void k_resync(DevGrid g, CF5 V, F5 U, const unsigned char* __restrict
     long c = gidx(g, i, j, k);
     double v[5], u[5];
     load5(V, c, v);
-    eos_p2c(c_mats.m[ID[c]], v, u);
+    eos_p2c(c_mats[mset].m[ID[c]], v, u);
     store5(U, c, u);
   }
 }
@@ -360,7 +363,7 @@ __global__ void k_apply_bc(DevGrid g, F5 V) {
 }
 
 // ClipDensityAndPressure on interior cells (SpaceOperator.cpp:646-693)
-__global__ void k_clip(DevGrid g, F5 V, const unsigned char* __restrict__ ID, DevCounters* cnt) {
+__global__ void k_clip(DevGrid g, int mset, F5 V, const unsigned char* __restrict__ ID, DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     int i, j, k;
@@ -368,7 +371,7 @@ __global__ void k_clip(DevGrid g, F5 V, const unsigned char* __restrict__ ID, De
     long c = gidx(g, i, j, k);
     double v[5];
     for (int p = 0; p < 5; p++) v[p] = V(p, c);
-    const DevEos& E = c_mats.m[ID[c]];
+    const DevEos& E = c_mats[mset].m[ID[c]];
     if (eos_clip(E, v)) {
       atomicAdd(&cnt->clipped, 1ull);
       atomicAdd(&cnt->stage_clipped, 1ull);
@@ -398,7 +401,7 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
     long c = gidx(g, i, j, k);
     int id = ID[c];
     if (id == s.inactive) continue;
-    const DevEos& E = c_mats.m[id];
+    const DevEos& E = c_mats[s.mset].m[id];
     double v[5];
     load5(V, c, v);
     double e = eos_e(E, v[0], v[4]);
@@ -479,7 +482,7 @@ __global__ void k_flux_batch(Scheme s, int n, int d, const double* Vm, const dou
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   int err = 0;
-  flux_num(c_mats.m[id[q]], s.flux, s.del, d, Vm + 5 * q, Vp + 5 * q, F + 5 * q, err);
+  flux_num(c_mats[s.mset].m[id[q]], s.flux, s.del, d, Vm + 5 * q, Vp + 5 * q, F + 5 * q, err);
   if (err) atomicOr(&cnt->error, 1);
 }
 

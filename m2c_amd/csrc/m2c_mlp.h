@@ -171,10 +171,10 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevGrid g, Scheme s, CF5 V, co
         const int d = L.dir[q];
         const long st = axis_stride(g, d);
         double Vm[5], Vp[5];
-        if_states(g, s, c_mats, V, ID, d, c, Vm, Vp);
+        if_states(g, s, c_mats[s.mset], V, ID, d, c, Vm, Vp);
         const double dir[3] = {(d == 0) ? 1.0 : 0.0, (d == 1) ? 1.0 : 0.0, (d == 2) ? 1.0 : 0.0};
         float x[8];
-        ok = mlp_features(c_mats, dir, Vm, ID[c - st], Vp, ID[c], x, P, pc);
+        ok = mlp_features(c_mats[s.mset], dir, Vm, ID[c - st], Vp, ID[c], x, P, pc);
         if (ok)
           for (int f = 0; f < 8; f++) xs[l * 32 + f] = (__bf16)x[f];
       }
@@ -198,14 +198,14 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevGrid g, Scheme s, CF5 V, co
 
 // Batched features / targets of explicit problems (training data and tests):
 // X[n][8], P[n], pcmin[n]; T[n] = log((p* + pcmin)/P) when pstar is given.
-__global__ void k_mlp_features(int n, const double* dir, const double* Vm, const int* idm, const double* Vp,
+__global__ void k_mlp_features(int mset, int n, const double* dir, const double* Vm, const int* idm, const double* Vp,
                                const int* idp, const double* pstar, float* X, double* P, double* pcmin, float* T,
                                int* ok) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   float x[8];
   double Pq = 1.0, pc = 0.0;
-  int good = mlp_features(c_mats, dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], x, Pq, pc);
+  int good = mlp_features(c_mats[mset], dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], x, Pq, pc);
   for (int f = 0; f < 8; f++) X[8L * q + f] = good ? x[f] : 0.0f;
   P[q] = Pq;
   pcmin[q] = pc;

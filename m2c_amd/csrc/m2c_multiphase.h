@@ -220,7 +220,7 @@ __device__ __forceinline__ void if_states(const DevGrid& g, const Scheme& s, con
 __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, const CF5& V,
                                             const unsigned char* __restrict__ ID, const IfaceList& L, int q,
                                             int stage1, const float* guess, DevRiemann& R, int& fatal, int& ovf) {
-  const DevMats& M = c_mats;
+  const DevMats& M = c_mats[s.mset];
   const long c = L.cell[q];
   const int d = L.dir[q];
   const long st = axis_stride(g, d);
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg 
   const int gid = wave * 64 + lane;
   if (gid >= lanes) return;
   const int nif = min(cnt->nif, L.cap);
-  DevRiemann R(c_mats, rc, slab + gid, lanes, pathcap);
+  DevRiemann R(c_mats[s.mset], rc, slab + gid, lanes, pathcap);
   unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
   int fatal = 0, ovf = 0;
   for (long q = (long)lane * W + wave; q < nif; q += 64L * W) {
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(64) k_riemann_multi(const RiemMember* __restri
       else hi = mid - 1;
     }
     const RiemMember& B = mem[lo];
-    DevRiemann R(c_mats, B.rc, slab + gid, lanes, pathcap);
+    DevRiemann R(c_mats[B.s.mset], B.rc, slab + gid, lanes, pathcap);
     int fatal = 0, ovf = 0;
     riemann_face(B.g, B.s, B.V, B.ID, B.L, (int)q - off[lo], stage1, B.guess, R, fatal, ovf);
     if (fatal) atomicOr(&B.cnt->error, 2);
@@ -403,14 +403,14 @@ __global__ void __launch_bounds__(256) k_riem_count(const RiemMember* mem) {
 }
 
 // point-kernel entry: n problems given explicitly
-__global__ void __launch_bounds__(128) k_riemann_batch(RiemannCfg rc, int n, const double* dir, const double* Vm,
+__global__ void __launch_bounds__(128) k_riemann_batch(int mset, RiemannCfg rc, int n, const double* dir, const double* Vm,
                                                        const int* idm, const double* Vp, const int* idp,
                                                        double* Vs, int* id, double* Vsm, double* Vsp, int* err,
                                                        int* iters, int* fevals, double* slab, int lanes,
                                                        int pathcap, const float* guess, DevCounters* cnt) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= lanes) return;
-  DevRiemann R(c_mats, rc, slab + lane, lanes, pathcap);
+  DevRiemann R(c_mats[mset], rc, slab + lane, lanes, pathcap);
   for (int q = lane; q < n; q += lanes) {
     R.st = RStats{0, 0, false, false, false};
     R.path1.overflow = R.path3.overflow = false;
@@ -884,7 +884,7 @@ __global__ void k_fix_unresolved(UpdArgs a) {
             }
         if (sum_weight > 0) { v[0] = density / sum_weight; break; }
       }
-      if (sum_weight == 0) v[0] = c_mats.m[myid].failsafe;
+      if (sum_weight == 0) v[0] = c_mats[a.s.mset].m[myid].failsafe;
     }
     for (int p = 0; p < 5; p++) a.V(p, c) = v[p];
   }
@@ -901,7 +901,7 @@ __global__ void k_post_update(UpdArgs a) {
     a.done[c] = 0;
     double v[5];
     for (int p = 0; p < 5; p++) v[p] = a.V(p, c);
-    const DevEos& E = c_mats.m[a.ID[c]];
+    const DevEos& E = c_mats[a.s.mset].m[a.ID[c]];
     if (eos_clip(E, v)) {
       atomicAdd(&a.cnt->clipped, 1ull);
       for (int p = 0; p < 5; p++) a.V(p, c) = v[p];

@@ -507,8 +507,8 @@ struct DevRiemann {
   }
 
   // ----------------------------------------------------------- bracketing
-  // wave strength from which the learned p* guess replaces the acoustic one
-  static constexpr double LEARNED_MIN_STRENGTH = 0.3;
+  // wave strengths between which the learned p* guess replaces the acoustic one
+  static constexpr double LEARNED_MIN_STRENGTH = 0.3, LEARNED_MAX_STRENGTH = 10.0;
 
   // FindInitialFeasiblePointsByAcousticTheory (:1107-1158); `p0g` replaces the
   // acoustic p0 when positive (learned guess), see m2cx_scheme.riem_guess.
@@ -519,12 +519,14 @@ struct DevRiemann {
     double Cl = rhol * cl, Cr = rhor * cr;
     p0 = (Cr * pl + Cl * pr + Cl * Cr * (ul - ur)) / (Cl + Cr);
     // learned guess: acoustic theory is exact to second order in the wave
-    // strength |p0 - p| / (rho c^2) and beats the network where both waves
-    // are weak; the prediction replaces it for strong waves and where the
-    // acoustic pressure is not positive
+    // strength s = |p0 - p| / (rho c^2) and beats the network where both
+    // waves are weak; the prediction replaces it where the acoustic pressure
+    // is not positive and for strong waves up to the strength the network
+    // was trained on
     if (p0g > 0) {
       const double sl = fabs(p0 - pl) / (Cl * cl), sr = fabs(p0 - pr) / (Cr * cr);
-      if (!(p0 > 0) || !(fmax_std(sl, sr) < LEARNED_MIN_STRENGTH)) p0 = p0g;
+      const double sw = fmax_std(sl, sr);
+      if (!(p0 > 0) || (sw >= LEARNED_MIN_STRENGTH && sw < LEARNED_MAX_STRENGTH)) p0 = p0g;
     }
     if (!rhoustar(1, path1, rhol, ul, pl, p0, idl, rhol, (p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0)) return found;
     if (!rhoustar(3, path3, rhor, ur, pr, p0, idr, rhor, (p0 > pr) ? rhor * 1.1 : rhor * 0.9, rhor0, ur0)) return found;

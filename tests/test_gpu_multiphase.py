@@ -135,3 +135,36 @@ def test_levelset_residual_bitwise():
     rg = gpu.levelset_residual()
     ro = o.ls_residual()
     assert np.array_equal(rg[1:-1, 1:-1, 1:-1], ro[1:-1, 1:-1, 1:-1])
+
+
+def test_solvers_with_different_materials_side_by_side(fixtures):
+    """Solvers with different material tables in one process keep their own
+    tables: a water/air shock-bubble and the air/air Test1 deck stepped in
+    turns, each bit-exact against its oracle."""
+    ga, ma, na, sa, Va, IDa, Pa = configs.c2_shock_bubble(16)
+    d = fixtures["Test1"]["deck"]
+    gb, mb, nb, sb, Vb, IDb, Pb = dk.setup(d)
+    ob = ol.Oracle(gb, mb, nb, sb)
+    ob.set_state(Vb, IDb, Pb)
+    ob.apply_bc()
+    ob.update_ghost_ids()
+    ob.ls_apply_bc()
+    Vb, IDb, Pb = ob.get_state()
+    oa = ol.Oracle(ga, ma, na, sa)
+    oa.set_state(Va, IDa, Pa)
+    A = _rt().Solver(ga, ma, na, sa)
+    A.set_state(Va, IDa, Pa)
+    B = _rt().Solver(gb, mb, nb, sb)
+    B.set_state(Vb, IDb, Pb)
+    for step in range(3):
+        A.run_steps(1)
+        B.run_steps(1)
+        oa.run_steps(1)
+        ob.run_steps(1)
+        for gpu, o in ((A, oa), (B, ob)):
+            Vg, IDg, Pg = gpu.get_state()
+            Vo, IDo, Po = o.get_state()
+            assert np.array_equal(Vg[1:-1, 1:-1, 1:-1], Vo[1:-1, 1:-1, 1:-1]), step
+            assert np.array_equal(IDg[1:-1, 1:-1, 1:-1], IDo[1:-1, 1:-1, 1:-1]), step
+    A.close()
+    B.close()
