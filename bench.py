@@ -143,12 +143,25 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}))
 
 
-def riemann_stats(cnt):
+def _quantile(hist, q):
+    """Smallest count k with at least a fraction q of the faces at <= k."""
+    c = np.cumsum(hist.astype(np.float64))
+    if c[-1] == 0:
+        return 0
+    return int(np.searchsorted(c, q * c[-1]))
+
+
+def riemann_stats(cnt, hist=None):
     calls = max(cnt["riemann_calls"], 1)
-    return {"faces": cnt["riemann_calls"], "trivial": cnt["riemann_trivial"],
-            "main_iters_per_face": cnt["riemann_main_iters"] / calls,
-            "fevals_per_face": cnt["riemann_fevals"] / calls,
-            "errors": cnt["riemann_errors"], "guess_fallbacks": cnt["riemann_guess_fallbacks"]}
+    out = {"faces": cnt["riemann_calls"], "trivial": cnt["riemann_trivial"],
+           "main_iters_per_face": cnt["riemann_main_iters"] / calls,
+           "fevals_per_face": cnt["riemann_fevals"] / calls,
+           "errors": cnt["riemann_errors"], "guess_fallbacks": cnt["riemann_guess_fallbacks"]}
+    if hist is not None:
+        it, fe = hist
+        out.update({"main_iters_p50": _quantile(it, 0.5), "main_iters_p99": _quantile(it, 0.99),
+                    "fevals_p50": _quantile(fe, 0.5), "fevals_p99": _quantile(fe, 0.99)})
+    return out
 
 
 def run_single(cfg, steps, warmup, learned_params=None):
@@ -175,12 +188,14 @@ def run_single(cfg, steps, warmup, learned_params=None):
     solver.synchronize()
     el = time.perf_counter() - t0
     cnt = solver.counters()
+    hist = solver.riemann_histogram()
     stage_ms, stage_n = solver.kernel_time("stage")
     riem_ms, _ = solver.kernel_time("riemann")
     mlp_ms, _ = solver.kernel_time("mlp")
     res = {"value": cells * stages * steps / el, "ms_per_step": el * 1e3 / steps, "cells": cells,
            "stages": stages, "stage_ms": stage_ms, "stage_launches": stage_n,
-           "riemann": dict(riemann_stats(cnt), riemann_ms_per_step=riem_ms / steps, mlp_ms_per_step=mlp_ms / steps)}
+           "riemann": dict(riemann_stats(cnt, hist), riemann_ms_per_step=riem_ms / steps,
+                           mlp_ms_per_step=mlp_ms / steps)}
     solver.close()
     return res
 
@@ -206,7 +221,8 @@ def main():
         sys.exit(2)
     if args.config == "c4":
         return bench_ensemble(args)
-    steps = args.steps if args.steps is not None else (100 if args.config == "c1" else 20)
+    # the configurations' own run lengths: 100 steps (c1), 200 (c2, c2_single); c5's 300 are cut to 50
+    steps = args.steps if args.steps is not None else {"c1": 100, "c2": 200, "c2_single": 200}.get(args.config, 50)
     cfg, desc = build_config(args.config)
     g, mats, nmat, s = cfg[:4]
     print("# bench: config=%s cells=%d steps=%d warmup=%d" % (args.config, g.nx * g.ny * g.nz, steps, args.warmup),
@@ -252,7 +268,7 @@ def main():
 def bench_ensemble(args):
     """config 4: uneven shock-bubble ensemble, lockstep with graph replay."""
     from m2c_amd import runtime
-    steps = args.steps if args.steps is not None else 20
+    steps = args.steps if args.steps is not None else 100  # the configuration: 100 steps each
     cases = configs.c4_instances(count=args.instances, seed=4)
     print("# bench: ensemble of %d, cells %d, steps %d" % (len(cases), sum(c[0][0] ** 3 for c in cases), steps),
           file=sys.stderr, flush=True)
@@ -277,11 +293,15 @@ def bench_ensemble(args):
     ens.run_steps(steps)
     el = time.perf_counter() - t0
     calls = iters = fev = 0
+    hit = hfe = None
     for sol in members:
         c = sol.counters()
         calls += c["riemann_calls"]
         iters += c["riemann_main_iters"]
         fev += c["riemann_fevals"]
+        a, b = sol.riemann_histogram()
+        hit = a if hit is None else hit + a
+        hfe = b if hfe is None else hfe + b
     out = {"metric": METRIC, "value": cells * 3 * steps / el, "unit": "cell-stage-updates/s", "n_gpus": 1,
            "steps": steps, "warmup": args.warmup, "ms_per_step": el * 1e3 / steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -290,7 +310,9 @@ def bench_ensemble(args):
                       "description": "uneven ensemble of shock-bubble runs (edges 64..192, r 0.05-0.25 L, "
                                      "Mach 1.2-3.0), lockstep, one graph per step"},
            "riemann": {"faces": calls, "main_iters_per_face": iters / max(calls, 1),
-                       "fevals_per_face": fev / max(calls, 1)}}
+                       "fevals_per_face": fev / max(calls, 1), "main_iters_p50": _quantile(hit, 0.5),
+                       "main_iters_p99": _quantile(hit, 0.99), "fevals_p50": _quantile(hfe, 0.5),
+                       "fevals_p99": _quantile(hfe, 0.99)}}
     ens.close()
     for sol in members:
         sol.close()

@@ -262,6 +262,15 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps);
 int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out);
 int m2cx_reset_counters(m2cx_solver* h);
 
+/* Distribution of the interface Riemann work since the last reset:
+ * iters[k] = faces whose secant main loop ran k iterations, fevals[k] = faces
+ * with k f(p) evaluations (ExactRiemannSolverBase.cpp:219, :741-743,
+ * :1122-1152), the last bin counting k >= M2CX_RIEMANN_HIST_BINS - 1.  All
+ * zero for a single-material solver.  No reference counterpart: the reference
+ * reports only failures (SpaceOperator.cpp:2031-2034). */
+enum { M2CX_RIEMANN_HIST_BINS = 512 };
+int m2cx_get_riemann_histogram(m2cx_solver* h, uint64_t* iters, uint64_t* fevals);
+
 /* ------------------------------------------------------------------ */
 /* Point kernels, batched (device does the work; host arrays in/out)   */
 /* ------------------------------------------------------------------ */

@@ -20,6 +20,7 @@ MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
 GEOM_PLANE, GEOM_SPHERE = 0, 3
+RIEMANN_HIST_BINS = 512
 OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY = \
     0, -1, -2, -3, -4, -5, -6, -7, -8
 

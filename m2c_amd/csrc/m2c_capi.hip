@@ -278,7 +278,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   HIPCHK(h, hipMemsetAsync(h->IDn, 0, n, h->stream));
   HIPCHK(h, hipMalloc(&h->staging, n * 5 * 8));
   HIPCHK(h, hipMalloc(&h->staging_i, n * 4));
-  h->dt_blocks = grid_blocks((long)NX * NY * NZ);
+  h->dt_blocks = std::min(grid_blocks((long)NX * NY * NZ), DT_BLOCKS);
   HIPCHK(h, hipMalloc(&h->dt_part, h->dt_blocks * 8));
   HIPCHK(h, hipMalloc(&h->cnt, sizeof(DevCounters)));
   DevCounters c0{};
@@ -739,8 +739,25 @@ int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out) {
   return 0;
 }
 
+int m2cx_get_riemann_histogram(m2cx_solver* h, uint64_t* iters, uint64_t* fevals) {
+  if (!h || !iters || !fevals) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  if (!h->multimat) {
+    std::fill(iters, iters + M2CX_RIEMANN_HIST_BINS, (uint64_t)0);
+    std::fill(fevals, fevals + M2CX_RIEMANN_HIST_BINS, (uint64_t)0);
+    return 0;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(iters, h->mp.L.hist, M2CX_RIEMANN_HIST_BINS * 8, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(fevals, h->mp.L.hist + M2CX_RIEMANN_HIST_BINS, M2CX_RIEMANN_HIST_BINS * 8,
+                      hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int m2cx_reset_counters(m2cx_solver* h) {
   DevCounters c;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->multimat) HIPCHK(h, hipMemset(h->mp.L.hist, 0, 2 * M2CX_RIEMANN_HIST_BINS * 8));
   HIPCHK(h, hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
   c.clipped = c.riemann_calls = c.riemann_trivial = c.riemann_errors = c.riemann_iters = c.riemann_fevals = 0;
   c.riemann_fallbacks = c.guess_fallbacks = c.id_changed = c.steps = 0;

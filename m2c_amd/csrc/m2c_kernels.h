@@ -34,6 +34,7 @@ struct IfaceList {
   double* vsp;    // [5][cap]
   int* err;       // Riemann error of the capture
   int* stats;     // [3][cap]: main-loop iterations, f(p) evaluations, flags (last solve)
+  unsigned long long* hist;  // [2][M2CX_RIEMANN_HIST_BINS]: counts of iterations, of f(p) evaluations
   int cap;
 };
 
@@ -389,12 +390,24 @@ __global__ void k_clip(DevGrid g, int mset, F5 V, const unsigned char* __restric
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double nmin(double a, double b) { return (b < a) ? b : a; }  // std::min(a,b)
 
+__device__ __forceinline__ double wave_min(double m) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = nmin(m, __shfl_xor(m, off, 64));
+  return m;
+}
+
+// Launched with DT_BLOCKS blocks of 256 (grid-stride over the cells): few
+// enough partial minima that the last block reduces them in one pass.
+constexpr int DT_BLOCKS = 2048;
 __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const unsigned char* __restrict__ ID,
                                             double* part, DevCounters* cnt, int advance) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   __shared__ bool last;
   double m = DBL_MAX;
-  long n = (long)g.NX * g.NY * g.NZ;
+  const long n = (long)g.NX * g.NY * g.NZ;
+  const double* dxs = g_axis(g, 3);
+  const double* dys = g_axis(g, 4);
+  const double* dzs = g_axis(g, 5);
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     int i, j, k;
     cell_ijk(g, q, i, j, k);
@@ -409,18 +422,16 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
     if (c2 < 0) atomicOr(&cnt->error, 1);
     double cc = sqrt(c2);
     double lf = fabs(v[1]) + cc, lg = fabs(v[2]) + cc, lh = fabs(v[3]) + cc;
-    if (g.NX > 1) m = nmin(m, g_axis(g, 3)[i + 1] / lf);
-    if (g.NY > 1) m = nmin(m, g_axis(g, 4)[j + 1] / lg);
-    if (g.NZ > 1) m = nmin(m, g_axis(g, 5)[k + 1] / lh);
+    if (g.NX > 1) m = nmin(m, dxs[i + 1] / lf);
+    if (g.NY > 1) m = nmin(m, dys[j + 1] / lg);
+    if (g.NZ > 1) m = nmin(m, dzs[k + 1] / lh);
   }
-  red[threadIdx.x] = m;
+  // min is exact and order-independent: any reduction order gives the same dt
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = nmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = red[0];
+    part[blockIdx.x] = nmin(nmin(red[0], red[1]), nmin(red[2], red[3]));
     __threadfence();
     unsigned int tk = atomicAdd(&cnt->dt_ticket, 1u);
     last = (tk == gridDim.x - 1);
@@ -430,12 +441,10 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
   __threadfence();
   double mm = DBL_MAX;
   for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) mm = nmin(mm, ((volatile double*)part)[b]);
-  red[threadIdx.x] = mm;
+  mm = wave_min(mm);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mm;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = nmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
+  if (threadIdx.x == 0) red[0] = nmin(nmin(red[0], red[1]), nmin(red[2], red[3]));
   if (threadIdx.x == 0) {
     cnt->dt_ticket = 0;
     double cfl = s.cfl;

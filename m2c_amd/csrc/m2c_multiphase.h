@@ -81,8 +81,10 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
   B.L.cap = (int)cap;
   if (hipMalloc(&B.L.cell, cap * 4) || hipMalloc(&B.L.dir, cap * 4) || hipMalloc(&B.L.f1, cap * 40) ||
       hipMalloc(&B.L.f2, cap * 40) || hipMalloc(&B.L.vsm, cap * 40) || hipMalloc(&B.L.vsp, cap * 40) ||
-      hipMalloc(&B.L.err, cap * 4) || hipMalloc(&B.L.stats, cap * 12))
+      hipMalloc(&B.L.err, cap * 4) || hipMalloc(&B.L.stats, cap * 12) ||
+      hipMalloc(&B.L.hist, 2 * M2CX_RIEMANN_HIST_BINS * 8))
     return M2CX_E_OOM;
+  if (hipMemsetAsync(B.L.hist, 0, 2 * M2CX_RIEMANN_HIST_BINS * 8, st) != hipSuccess) return M2CX_E_HIP;
   for (double** p : {&B.phi, &B.phi1, &B.phin, &B.rls, &B.phitmp}) {
     if (hipMalloc(p, n * 8) != hipSuccess) return M2CX_E_OOM;
     if (hipMemsetAsync(*p, 0, n * 8, st) != hipSuccess) return M2CX_E_HIP;
@@ -152,7 +154,7 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
 inline void mp_destroy(MPBuffers& B) {
   for (int d = 0; d < 3; d++) (void)hipFree(B.fidx[d]);
   for (void* p : {(void*)B.L.cell, (void*)B.L.dir, (void*)B.L.f1, (void*)B.L.f2, (void*)B.L.vsm, (void*)B.L.vsp,
-                  (void*)B.L.err, (void*)B.L.stats, (void*)B.phi, (void*)B.phi1, (void*)B.phin, (void*)B.rls,
+                  (void*)B.L.err, (void*)B.L.stats, (void*)B.L.hist, (void*)B.phi, (void*)B.phi1, (void*)B.phin, (void*)B.rls,
                   (void*)B.phitmp, (void*)B.fd3, (void*)B.slab, (void*)B.remaining, (void*)B.changed,
                   (void*)B.unresolved, (void*)B.tag, (void*)B.done, (void*)B.IDn})
     (void)hipFree(p);
@@ -257,6 +259,8 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   L.stats[q] = R.st.it_main;
   L.stats[L.cap + q] = R.st.fevals;
   L.stats[2L * L.cap + q] = flags;
+  atomicAdd(&L.hist[min(R.st.it_main, M2CX_RIEMANN_HIST_BINS - 1)], 1ull);
+  atomicAdd(&L.hist[M2CX_RIEMANN_HIST_BINS + min(R.st.fevals, M2CX_RIEMANN_HIST_BINS - 1)], 1ull);
   double f1[5], f2[5];
   int ferr = 0;
   if (s.mp_flux == M2CX_MPFLUX_EXACT) {

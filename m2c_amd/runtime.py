@@ -54,6 +54,7 @@ def lib():
         L.m2cx_run_steps.argtypes = [P, C.c_int]
         L.m2cx_get_counters.argtypes = [P, C.POINTER(abi.Counters)]
         L.m2cx_reset_counters.argtypes = [P]
+        L.m2cx_get_riemann_histogram.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.m2cx_numerical_flux_batch.argtypes = [P, C.c_int, C.c_int, D, D, I32, D]
         L.m2cx_riemann_batch.argtypes = [P, C.c_int, D, D, I32, D, I32, D, I32, D, D, I32, I32, I32]
         L.m2cx_profile.argtypes = [P, C.c_int]
@@ -190,6 +191,15 @@ class Solver:
         c = abi.Counters()
         self._check(self.L.m2cx_get_counters(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in abi.Counters._fields_}
+
+    def riemann_histogram(self):
+        """(iters, fevals): faces per count of secant iterations and of f(p)
+        evaluations since the last reset (last bin: that count or more)."""
+        it = np.zeros(abi.RIEMANN_HIST_BINS, dtype=np.uint64)
+        fe = np.zeros(abi.RIEMANN_HIST_BINS, dtype=np.uint64)
+        self._check(self.L.m2cx_get_riemann_histogram(self.h, it.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                      fe.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return it, fe
 
     def reset_counters(self):
         self._check(self.L.m2cx_reset_counters(self.h))

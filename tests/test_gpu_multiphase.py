@@ -168,3 +168,23 @@ def test_solvers_with_different_materials_side_by_side(fixtures):
             assert np.array_equal(IDg[1:-1, 1:-1, 1:-1], IDo[1:-1, 1:-1, 1:-1]), step
     A.close()
     B.close()
+
+
+def test_riemann_histogram_matches_counters():
+    """The per-face work histogram sums to the face count and, weighted by
+    the bin value, to the iteration and f(p)-evaluation totals."""
+    g, mats, nmat, s, V, ID, Phi = configs.c2_shock_bubble(24)
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(V, ID, Phi)
+    gpu.run_steps(3)
+    c = gpu.counters()
+    it, fe = gpu.riemann_histogram()
+    k = np.arange(len(it))
+    assert it.sum() == fe.sum() == c["riemann_calls"] > 0
+    assert it[-1] == 0 and fe[-1] == 0  # no face reached the last bin here
+    assert int((k * it).sum()) == c["riemann_main_iters"]
+    assert int((k * fe).sum()) == c["riemann_fevals"]
+    gpu.reset_counters()
+    it, fe = gpu.riemann_histogram()
+    assert it.sum() == 0 and fe.sum() == 0
+    gpu.close()
