@@ -63,6 +63,10 @@ enum { M2CX_MPREC_CONSTANT = 0, M2CX_MPREC_LINEAR = 1 };
 enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0 };
 enum { M2CX_RNORMAL_MESH = 1 };
 
+/* LevelSetReinitializationData::FirstLayerTreatment (IoData.h:754-756) */
+enum { M2CX_REINIT_FIXED = 0, M2CX_REINIT_CONSTRAINED1 = 1, M2CX_REINIT_CONSTRAINED2 = 2,
+       M2CX_REINIT_ITERATIVE_CONSTRAINED1 = 3, M2CX_REINIT_ITERATIVE_CONSTRAINED2 = 4 };
+
 /* Initial guess of the interface Riemann solve. */
 enum { M2CX_GUESS_ACOUSTIC = 0,   /* FindInitialFeasiblePointsByAcousticTheory */
        M2CX_GUESS_MLP = 1 };      /* learned p* guess, acoustic fallback       */
@@ -70,7 +74,7 @@ enum { M2CX_GUESS_ACOUSTIC = 0,   /* FindInitialFeasiblePointsByAcousticTheory *
 /* Error codes */
 enum { M2CX_OK = 0, M2CX_E_ARG = -1, M2CX_E_HIP = -2, M2CX_E_OOM = -3,
        M2CX_E_HYPERBOLICITY = -4, M2CX_E_RIEMANN_FATAL = -5, M2CX_E_UNRESOLVED = -6,
-       M2CX_E_UNSUPPORTED = -7, M2CX_E_CAPACITY = -8 };
+       M2CX_E_UNSUPPORTED = -7, M2CX_E_CAPACITY = -8, M2CX_E_REINIT = -9 };
 
 /* ------------------------------------------------------------------ */
 /* Problem description                                                 */
@@ -134,6 +138,16 @@ typedef struct {
   int32_t ls_bc[6];            /* LevelSet[0].BoundaryCondition* (M2CX_LSBC_*) */
   int32_t riem_guess;          /* M2CX_GUESS_*                                 */
   int32_t reserved;
+  /* LevelSet[0].Reinitialization (IoData.cpp:1421-1429); the level set is
+   * reinitialized when reinit_frequency > 0 or reinit_frequency_dt > 0      */
+  int32_t reinit_frequency;    /* Frequency (-1)                               */
+  int32_t reinit_maxits;       /* MaxIts (30)                                  */
+  double reinit_frequency_dt;  /* TimeInterval (-1)                            */
+  double reinit_cfl;           /* CFL (0.8)                                    */
+  double reinit_tol;           /* ConvergenceTolerance (2e-4)                  */
+  int32_t reinit_first_layer;  /* FirstLayerTreatment, M2CX_REINIT_* (Fixed)   */
+  int32_t reserved2;
+  double time_step;            /* Time.TimeStep: fixed dt when > 0 (0)         */
 } m2cx_scheme;
 
 /* Fields held on the device by a solver handle. */
@@ -157,6 +171,10 @@ typedef struct {
   int64_t clipped_cells;       /* ClipDensityAndPressure counts                 */
   int64_t id_changed_cells;    /* cells whose material ID changed               */
   int64_t steps;               /* completed time steps                          */
+  int64_t reinit_calls;        /* level-set reinitializations                   */
+  int64_t reinit_iterations;   /* their pseudo-time iterations, summed          */
+  int64_t unresolved_cells;    /* cells UpdateStateVariablesAfterInterfaceMotion
+                                  left unresolved                               */
 } m2cx_counters;
 
 typedef struct m2cx_solver m2cx_solver;
@@ -242,6 +260,19 @@ int m2cx_clip(m2cx_solver* h, int v_field);
  * buffer (download with m2cx_download_ls_residual). */
 int m2cx_levelset_residual(m2cx_solver* h, int v_field);
 int m2cx_download_ls_residual(m2cx_solver* h, double* host);
+
+/* LevelSetOperator::Reinitialize with must_do = true (LevelSetOperator.cpp:
+ * 1338-1363) -> LevelSetReinitializer::ReinitializeFullDomain
+ * (LevelSetReinitializer.cpp:76-176): restores level set 0 to a signed
+ * distance by pseudo-time RK3 iterations, first-layer nodes treated as
+ * scheme.reinit_first_layer says.  special_maxits > 0 replaces
+ * scheme.reinit_maxits (the reference's initializers pass 600 or 2000).
+ * Requires reinit_frequency > 0 or reinit_frequency_dt > 0, as the reference
+ * requires a reinitializer; returns M2CX_E_UNSUPPORTED otherwise, and
+ * M2CX_E_REINIT when the iterations fail to converge with CFL below 0.02
+ * (the reference exits there).  Time stepping calls it on its own at the
+ * configured frequency. */
+int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits);
 
 /* ------------------------------------------------------------------ */
 /* TimeIntegrator (TimeIntegrator.h:77-88, TimeIntegrator.cpp:167-843) */

@@ -18,11 +18,13 @@ TI_FE, TI_RK2, TI_RK3 = 0, 1, 2
 MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
+REINIT_FIXED, REINIT_CONSTRAINED1, REINIT_CONSTRAINED2, REINIT_ITERATIVE_CONSTRAINED1, \
+    REINIT_ITERATIVE_CONSTRAINED2 = range(5)
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
 GEOM_PLANE, GEOM_SPHERE = 0, 3
 RIEMANN_HIST_BINS = 512
-OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY = \
-    0, -1, -2, -3, -4, -5, -6, -7, -8
+OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY, \
+    E_REINIT = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9
 
 # Python name -> C name where they differ beyond the M2CX_ prefix
 C_NAMES = {"EOS_SG": "M2CX_EOS_SG", "LIM_GMM": "M2CX_LIM_GENERALIZED_MINMOD", "TI_FE": "M2CX_TI_FORWARD_EULER"}
@@ -61,13 +63,17 @@ class Scheme(C.Structure):
                 ("riem_tol_rarefaction", C.c_double), ("riem_min_pressure", C.c_double),
                 ("riem_failure_threshold", C.c_double), ("riem_pressure_at_failure", C.c_double),
                 ("n_levelsets", C.c_int32), ("ls_materialid", C.c_int32), ("ls_bc", C.c_int32 * 6),
-                ("riem_guess", C.c_int32), ("reserved", C.c_int32)]
+                ("riem_guess", C.c_int32), ("reserved", C.c_int32),
+                ("reinit_frequency", C.c_int32), ("reinit_maxits", C.c_int32),
+                ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
+                ("reinit_first_layer", C.c_int32), ("reserved2", C.c_int32), ("time_step", C.c_double)]
 
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "riemann_calls", "riemann_trivial", "riemann_errors", "riemann_main_iters", "riemann_fevals",
-        "riemann_guess_fallbacks", "clipped_cells", "id_changed_cells", "steps")]
+        "riemann_guess_fallbacks", "clipped_cells", "id_changed_cells", "steps", "reinit_calls",
+        "reinit_iterations", "unresolved_cells")]
 
 
 def material(eos="SG", **kw):
@@ -135,6 +141,13 @@ def default_scheme(**kw):
     for i in range(6):
         s.ls_bc[i] = LSBC_ZERO_NEUMANN
     s.riem_guess = GUESS_ACOUSTIC
+    s.reinit_frequency = -1
+    s.reinit_maxits = 30
+    s.reinit_frequency_dt = -1.0
+    s.reinit_cfl = 0.8
+    s.reinit_tol = 2.0e-4
+    s.reinit_first_layer = REINIT_FIXED
+    s.time_step = 0.0
     for k, v in kw.items():
         setattr(s, k, v)
     return s

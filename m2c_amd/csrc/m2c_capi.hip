@@ -13,6 +13,7 @@
 #include <vector>
 #include "m2c_kernels.h"
 #include "m2c_multiphase.h"
+#include "m2c_reinit.h"
 #include "m2c_mlp.h"
 #include <algorithm>
 #include <random>
@@ -55,6 +56,10 @@ struct m2cx_solver {
   // multi-material
   MPBuffers mp{};
   bool multimat = false;
+  // level-set reinitialization
+  RBuf rb{};
+  double reinit_cfl = 0.8;  // the reinitializer's CFL, lowered by its failsafe
+  long long reinit_calls = 0, reinit_iters = 0;
   hipStream_t stream_saved = nullptr;  // own stream while an ensemble borrows it
   int tile = -1;                        // stage tile variant (-1: not tuned yet)
   // learned p* guess
@@ -182,6 +187,8 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if ((long)grid->nx * grid->ny * grid->nz >= (1L << 31)) return M2CX_E_ARG;  // 32-bit cell numbering
   if (sch->var_type < M2CX_VAR_PRIMITIVE || sch->var_type > M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) return M2CX_E_ARG;
   if (sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
+  if (sch->reinit_first_layer < M2CX_REINIT_FIXED || sch->reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2)
+    return M2CX_E_ARG;
   auto* h = new m2cx_solver;
   *out = h;
   h->grid = *grid;
@@ -244,6 +251,8 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   s.integrator = sch->integrator; s.mp_flux = sch->mp_flux; s.mp_recon = sch->mp_recon;
   s.nmat = nmat; s.inactive = nmat;
   s.gmm = sch->gmm_coeff; s.del = sch->entropy_fix; s.cfl = sch->cfl;
+  s.time_step = sch->time_step;
+  h->reinit_cfl = sch->reinit_cfl;
   s.var_type = sch->var_type;
   DevMats& M = h->M;
   std::memset(&M, 0, sizeof M);
@@ -307,6 +316,7 @@ int m2cx_destroy(m2cx_solver* h) {
   hipFree(h->geo); hipFree(h->coefK);
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
+  ri_free(h->rb);
   mat_slot_release(h);
   for (void* p : {(void*)h->mlp.theta, (void*)h->mlp.m, (void*)h->mlp.v, (void*)h->mlp.wt1, (void*)h->mlp.wt2,
                   (void*)h->mlp.wt3, (void*)h->mlp.wt4, (void*)h->mlp.w2, (void*)h->mlp.w3, (void*)h->mlp.guess})
@@ -653,17 +663,141 @@ static void stage_after_riemann(m2cx_solver* h, const StageSpec& sp) {
   levelset_stage(h, sp.vin, sp.ls_which);
 }
 
-static void end_of_step(m2cx_solver* h) {
-  if (h->multimat && h->sch.n_levelsets > 0) {
-    hipEvent_t e0{};
+static bool reinit_enabled(const m2cx_solver* h) {
+  return h->sch.n_levelsets > 0 && (h->sch.reinit_frequency > 0 || h->sch.reinit_frequency_dt > 0);
+}
+
+// LevelSetReinitializer::ReinitializeFullDomain (LevelSetReinitializer.cpp:
+// 76-176) on level set 0.  Iterations go in chunks; between chunks the host
+// reads whether they converged (later kernels of a chunk return at once).
+constexpr int REINIT_CHUNK = 16;
+static int reinit_full(m2cx_solver* h, int special_maxits) {
+  MPBuffers& B = h->mp;
+  const int flt = h->sch.reinit_first_layer;
+  const bool cr = flt == M2CX_REINIT_CONSTRAINED1 || flt == M2CX_REINIT_CONSTRAINED2;
+  const bool hcr = flt == M2CX_REINIT_ITERATIVE_CONSTRAINED1 || flt == M2CX_REINIT_ITERATIVE_CONSTRAINED2;
+  const int fix = (flt == M2CX_REINIT_FIXED || cr) ? 1 : 0;
+  if (int rc = ri_alloc(h->rb, h->ncells, B.list_cap, cr, h->stream)) return fail(h, rc, "reinitialization buffers");
+  RBuf& b = h->rb;
+  const DevGrid& g = h->g;
+  hipStream_t st = h->stream;
+  const long nin = (long)g.NX * g.NY * g.NZ;
+  const long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+  const int* bc = B.lsbc;
+  double* phi = B.phi;
+  hipEvent_t e0{};
+  tic(h, "reinit", &e0);
+  k_rstate_reset<<<1, 1, 0, st>>>(b.st, 1);
+  k_rtag<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, b);
+  ReinitState hs;
+  HIPCHK(h, hipMemcpyAsync(&hs, b.st, sizeof hs, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  if (hs.nfl == 0) return 0;  // no interface (TagFirstLayerNodes returns false)
+  if (hs.nfl > b.cap) return fail(h, M2CX_E_CAPACITY, "first layer of the level set exceeds the list capacity");
+  h->reinit_calls++;
+  const int lb = blocks_for(hs.nfl);
+  k_rsign<<<blocks_for(nin), 256, 0, st>>>(g, phi, b.sign);
+  if (cr) {
+    k_raxpy_copy<<<blocks_for(g.ncells), 256, 0, st>>>(b.pa, phi, g.ncells);
+    k_rfirst<<<lb, 256, 0, st>>>(g, b, phi, b.pa);
+    k_rnormal<<<blocks_for(nin), 256, 0, st>>>(g, b, phi);
+    k_rnormal_fl<<<lb, 256, 0, st>>>(b);
+    k_rcurv<<<lb, 256, 0, st>>>(g, b, phi, b.pa, flt);
+    k_rapply<<<lb, 256, 0, st>>>(b, phi);
+    k_ls_bc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+  }
+  k_copy_f64<<<blocks_for(g.ncells), 256, 0, st>>>(b.phibk, phi, g.ncells);
+  const double tol = h->sch.reinit_tol;
+  for (;;) {  // RETRY_FullDomain
+    const int maxits = special_maxits > 0 ? special_maxits : h->sch.reinit_maxits;
+    const double cfl = h->reinit_cfl;
+    k_rstate_reset<<<1, 1, 0, st>>>(b.st, 0);
+    bool done = false;
+    hs.iters = 0;
+    hs.last_dphi = 0.0;
+    for (int launched = 0; launched < maxits;) {
+      const int chunk = std::min(REINIT_CHUNK, maxits - launched);
+      for (int q = 0; q < chunk; q++) {
+        k_rstage1<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, cfl);
+        k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, b.pa, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
+        k_rstage2<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, cfl, tol);
+        k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, b.pb, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
+        k_rstage3<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, cfl, tol);
+        k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
+        if (hcr) {
+          k_rcorr_f<<<lb, 256, 0, st>>>(g, b, phi, flt, tol);
+          k_rcorr_apply<<<lb, 256, 0, st>>>(g, b, phi, cfl, tol);
+          k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
+        }
+        k_rerr<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, tol);
+        k_rcheck<<<1, 1, 0, st>>>(b.st, tol);
+      }
+      launched += chunk;
+      HIPCHK(h, hipGetLastError());
+      HIPCHK(h, hipMemcpyAsync(&hs, b.st, sizeof hs, hipMemcpyDeviceToHost, st));
+      HIPCHK(h, hipStreamSynchronize(st));
+      if (hs.done) {
+        done = true;
+        break;
+      }
+    }
+    h->reinit_iters += hs.iters;
+    if (done || hs.last_dphi < 0.02) break;  // converged, or close enough (:157-160)
+    if (h->reinit_cfl < 0.02) {
+      toc(h, "reinit", e0);
+      return fail(h, M2CX_E_REINIT, "level-set reinitialization failed to converge");
+    }
+    h->reinit_cfl /= 1.5;
+    k_copy_f64<<<blocks_for(g.ncells), 256, 0, st>>>(phi, b.phibk, g.ncells);
+  }
+  toc(h, "reinit", e0);
+  return 0;
+}
+
+// isTimeToWrite (Utils.cpp:337-357) for the reinitialization frequency
+static bool reinit_due(const m2cx_solver* h, double time, double dt, int time_step) {
+  if (h->sch.reinit_frequency_dt > 0)
+    return std::floor(time / h->sch.reinit_frequency_dt) != std::floor((time - dt) / h->sch.reinit_frequency_dt);
+  return h->sch.reinit_frequency > 0 && time_step % h->sch.reinit_frequency == 0;
+}
+
+// TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
+// 720-800), one level set, no embedded surfaces
+static int end_of_step(m2cx_solver* h) {
+  if (!(h->multimat && h->sch.n_levelsets > 0)) return 0;
+  hipEvent_t e0{};
+  if (!reinit_enabled(h)) {
     tic(h, "endstep", &e0);
     mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
     toc(h, "endstep", e0);
+    return 0;
   }
+  MPBuffers& B = h->mp;
+  F5 V = f5(h, h->V[0]);
+  mp_end_ids(B, h->g, h->s, h->ID, h->cnt, true, h->stream);
+  DevCounters c;
+  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (reinit_due(h, c.t, c.dts, c.time_step))
+    if (int rc = reinit_full(h, 0)) return rc;
+  mp_end_states(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
+  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (c.n_unresolved > 0) {
+    // TimeIntegrator.cpp:771-793: Phi_tmp = Phi, the unresolved cells' level
+    // set, a forced reinitialization, then the IDs from the sign changes
+    const long n = h->ncells;
+    k_copy_f64<<<blocks_for(n), 256, 0, h->stream>>>(B.phin, B.phi, n);
+    k_ls_unresolved<<<blocks_for(c.n_unresolved), 256, 0, h->stream>>>(B.unresolved, h->cnt, B.list_cap, B.phi);
+    if (int rc = reinit_full(h, 0)) return rc;
+    mp_end_ids(B, h->g, h->s, h->ID, h->cnt, false, h->stream);
+  }
+  mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, 1, h->stream);
+  return 0;
 }
 
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
-static void step_kernels(m2cx_solver* h) {
+static int step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
   if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
   if (ti == M2CX_TI_FORWARD_EULER) {
@@ -681,7 +815,7 @@ static void step_kernels(m2cx_solver* h) {
       stage_after_riemann(h, sp[q]);
     }
   }
-  end_of_step(h);
+  return end_of_step(h);
 }
 
 int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_step) {
@@ -691,10 +825,11 @@ int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_
   DevCounters c;
   HIPCHK(h, hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
   c.dt = dt;
+  c.dts = dt;
   c.t = time;
   c.time_step = time_step;
   HIPCHK(h, hipMemcpy(h->cnt, &c, sizeof c, hipMemcpyHostToDevice));
-  step_kernels(h);
+  if (int rc = step_kernels(h)) return rc;
   HIPCHK(h, hipGetLastError());
   return check_error(h);
 }
@@ -717,7 +852,7 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
     tic(h, "dt", &e0);
     k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 1);
     toc(h, "dt", e0);
-    step_kernels(h);
+    if (int rc = step_kernels(h)) return rc;
     HIPCHK(h, hipGetLastError());
   }
   return check_error(h);
@@ -736,7 +871,20 @@ int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out) {
   out->clipped_cells = c.clipped;
   out->id_changed_cells = c.id_changed;
   out->steps = c.steps;
+  out->reinit_calls = h->reinit_calls;
+  out->reinit_iterations = h->reinit_iters;
+  out->unresolved_cells = (int64_t)c.unresolved_total;
   return 0;
+}
+
+int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits) {
+  if (!h) return M2CX_E_ARG;
+  if (!reinit_enabled(h) || !h->multimat)
+    return fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = reinit_full(h, special_maxits)) return rc;
+  HIPCHK(h, hipGetLastError());
+  return check_error(h);
 }
 
 int m2cx_get_riemann_histogram(m2cx_solver* h, uint64_t* iters, uint64_t* fevals) {
@@ -761,6 +909,8 @@ int m2cx_reset_counters(m2cx_solver* h) {
   HIPCHK(h, hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
   c.clipped = c.riemann_calls = c.riemann_trivial = c.riemann_errors = c.riemann_iters = c.riemann_fevals = 0;
   c.riemann_fallbacks = c.guess_fallbacks = c.id_changed = c.steps = 0;
+  c.unresolved_total = 0;
+  h->reinit_calls = h->reinit_iters = 0;
   HIPCHK(h, hipMemcpy(h->cnt, &c, sizeof c, hipMemcpyHostToDevice));
   return 0;
 }
@@ -1104,7 +1254,9 @@ static void ensemble_step(m2cx_ensemble* e) {
     }
     for (m2cx_solver* h : e->m) stage_after_riemann(h, sp[q]);
   }
-  for (m2cx_solver* h : e->m) end_of_step(h);
+  for (m2cx_solver* h : e->m)
+    if (h->multimat && h->sch.n_levelsets > 0)
+      mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
 }
 
 int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out) {
@@ -1117,6 +1269,9 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
   for (m2cx_solver* h : e->m) {
     if (h->device != e->device) return efail(e, M2CX_E_ARG, "ensemble members must share one device");
     if (h->sch.integrator != e->ti) return efail(e, M2CX_E_ARG, "ensemble members must share the integrator");
+    if (reinit_enabled(h))
+      return efail(e, M2CX_E_UNSUPPORTED, "level-set reinitialization needs host control between steps; "
+                                          "step such members alone");
   }
   // forward Euler exchanges the state buffers every step, which a replayed
   // graph cannot follow

@@ -17,6 +17,7 @@ struct DevCounters {
   int n_remaining, n_changed, n_unresolved, pad;
   double t, dt, cfl, dts, tmax;
   int time_step, pad2;
+  unsigned long long unresolved_total;  // cells left unresolved by the state update, summed
 };
 
 enum StageMode { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_RK2_FINAL = 2, MODE_RK3_STAGE2 = 3, MODE_RK3_FINAL = 4,
@@ -447,8 +448,14 @@ __global__ void __launch_bounds__(256) k_dt(DevGrid g, Scheme s, CF5 V, const un
   if (threadIdx.x == 0) red[0] = nmin(nmin(red[0], red[1]), nmin(red[2], red[3]));
   if (threadIdx.x == 0) {
     cnt->dt_ticket = 0;
-    double cfl = s.cfl;
-    double dt = cfl * red[0];
+    double cfl, dt;
+    if (s.time_step > 0) {  // Time.TimeStep (SpaceOperator.cpp:1315-1321)
+      dt = s.time_step;
+      cfl = dt / red[0];
+    } else {
+      cfl = s.cfl;
+      dt = cfl * red[0];
+    }
     if (advance) {
       double t = cnt->t, tmax = cnt->tmax;
       if (t + dt > tmax) {

@@ -502,9 +502,10 @@ __global__ void k_levelset(LSArgs a) {
   }
 }
 
-// LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:408-508)
-__global__ void k_ls_bc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, int b4, int b5) {
-  const int bcs[6] = {b0, b1, b2, b3, b4, b5};
+// LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:408-508);
+// LevelSetReinitializer::ApplyBoundaryConditions (LevelSetReinitializer.cpp:
+// 1249-1360) fills the same face ghosts by the same rules.
+__device__ void ls_bc_faces(const DevGrid& g, double* phi, const int* bcs) {
   long nfaces = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nfaces; q += (long)gridDim.x * blockDim.x) {
     long r = q;
@@ -547,6 +548,11 @@ __global__ void k_ls_bc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, 
     }
     phi[c] = ph;
   }
+}
+
+__global__ void k_ls_bc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, int b4, int b5) {
+  const int bcs[6] = {b0, b1, b2, b3, b4, b5};
+  ls_bc_faces(g, phi, bcs);
 }
 
 // ---------------------------------------------------------------------------
@@ -690,6 +696,7 @@ struct UpdArgs {
   int* unresolved;
   unsigned char* done;
   int cap;
+  int reinit;  // a reinitializer exists (the unresolved cells' level set was updated)
   DevCounters* cnt;
 };
 
@@ -807,6 +814,13 @@ __global__ void k_fix_unresolved(UpdArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nu = min(a.cnt->n_unresolved, a.cap);
   if (nu == 0) return;
+  a.cnt->unresolved_total += (unsigned long long)nu;
+  if (!a.reinit) {
+    // the unresolved cells' level set must be reinitialized, and without a
+    // reinitializer the reference exits (LevelSetOperator.cpp:1341-1345)
+    atomicOr(&a.cnt->error, 8);
+    return;
+  }
   const DevGrid& g = a.g;
   int* U = a.unresolved;
   for (int x = 1; x < nu; x++) {  // insertion sort by linear index = (k,j,i) order
@@ -1045,16 +1059,25 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
   }
 }
 
-inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
-                        hipStream_t st) {
+__global__ void k_remaining_reset(DevCounters* cnt) { cnt->n_remaining = 0; }
+
+// UpdateMaterialIDByLevelSet(phin, phi) with the ghost IDs; `first` resets
+// the end-of-step counters (a second call follows the level-set update of
+// unresolved cells, TimeIntegrator.cpp:791-792)
+inline void mp_end_ids(MPBuffers& B, const DevGrid& g, const Scheme& s, unsigned char* ID, DevCounters* cnt,
+                       bool first, hipStream_t st) {
   const long nin = (long)g.NX * g.NY * g.NZ;
-  k_end_reset<<<1, 1, 0, st>>>(cnt);
+  if (first) k_end_reset<<<1, 1, 0, st>>>(cnt);
+  else k_remaining_reset<<<1, 1, 0, st>>>(cnt);
   k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, s, B.phin, B.phi, ID, B.tag, B.remaining, B.list_cap, B.lsid,
                                               cnt);
   k_id_vote<<<1, 1024, 0, st>>>(g, s, ID, B.tag, B.remaining, B.list_cap, B.lsid, cnt);
   long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
   k_ghost_ids<<<blocks_for(nf), 256, 0, st>>>(g, ID);
-  k_changed<<<blocks_for(nin), 256, 0, st>>>(g, s, ID, B.IDn, B.changed, B.list_cap, cnt);
+}
+
+inline UpdArgs mp_upd_args(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID,
+                           DevCounters* cnt, int reinit) {
   UpdArgs a{};
   a.g = g;
   a.s = s;
@@ -1067,10 +1090,33 @@ inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, u
   a.unresolved = B.unresolved;
   a.done = B.done;
   a.cap = B.list_cap;
+  a.reinit = reinit;
   a.cnt = cnt;
-  k_state_update<<<1, 1024, 0, st>>>(a);
+  return a;
+}
+
+// UpdateStateVariablesAfterInterfaceMotion up to the list of unresolved cells
+inline void mp_end_states(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID,
+                          DevCounters* cnt, hipStream_t st) {
+  const long nin = (long)g.NX * g.NY * g.NZ;
+  k_changed<<<blocks_for(nin), 256, 0, st>>>(g, s, ID, B.IDn, B.changed, B.list_cap, cnt);
+  k_state_update<<<1, 1024, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt, 0));
+}
+
+// FixUnresolvedNodes, then clipping and the boundary fill of V
+inline void mp_end_fix(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
+                       int reinit, hipStream_t st) {
+  UpdArgs a = mp_upd_args(B, g, s, V, ID, cnt, reinit);
   k_fix_unresolved<<<1, 64, 0, st>>>(a);
   k_post_update<<<256, 256, 0, st>>>(a);
+}
+
+// the end-of-step update without a reinitializer
+inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
+                        hipStream_t st) {
+  mp_end_ids(B, g, s, ID, cnt, true, st);
+  mp_end_states(B, g, s, V, ID, cnt, st);
+  mp_end_fix(B, g, s, V, ID, cnt, 0, st);
 }
 
 }  // namespace m2c

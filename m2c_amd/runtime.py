@@ -50,6 +50,7 @@ def lib():
         L.m2cx_clip.argtypes = [P, C.c_int]
         L.m2cx_levelset_residual.argtypes = [P, C.c_int]
         L.m2cx_download_ls_residual.argtypes = [P, D]
+        L.m2cx_levelset_reinitialize.argtypes = [P, C.c_int]
         L.m2cx_advance_one_time_step.argtypes = [P, C.c_double, C.c_double, C.c_int]
         L.m2cx_run_steps.argtypes = [P, C.c_int]
         L.m2cx_get_counters.argtypes = [P, C.POINTER(abi.Counters)]
@@ -176,6 +177,11 @@ class Solver:
         out = np.zeros(self.shape)
         self._check(self.L.m2cx_download_ls_residual(self.h, _d(out)))
         return out
+
+    def reinitialize(self, special_maxits=0):
+        """LevelSetOperator::Reinitialize with must_do = true: level set 0
+        back to a signed distance (scheme.reinit_* settings)."""
+        return self._check(self.L.m2cx_levelset_reinitialize(self.h, special_maxits))
 
     # ---- TimeIntegrator / driver ----
     def advance_one_time_step(self, time, dt, step):

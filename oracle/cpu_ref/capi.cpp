@@ -118,12 +118,29 @@ int orc_set_time(orc_solver* h, double t, int step) {
   return 0;
 }
 
-// counters: calls, trivial, errors, iters, fevals, fallbacks, clipped, id_changed, steps
+// counters: calls, trivial, errors, iters, fevals, fallbacks, clipped, id_changed, steps,
+// reinit calls, reinit iterations, unresolved cells
 int orc_counters(orc_solver* h, long* out) {
   Counters& c = h->s->cnt;
-  long v[9] = {c.riemann_calls, c.riemann_trivial, c.riemann_errors, c.riemann_iters, c.riemann_fevals,
-               c.riemann_fallbacks, c.clipped, c.id_changed, c.steps};
+  long v[12] = {c.riemann_calls, c.riemann_trivial, c.riemann_errors, c.riemann_iters, c.riemann_fevals,
+                c.riemann_fallbacks, c.clipped, c.id_changed, c.steps, c.reinit_calls, c.reinit_iters,
+                c.unresolved};
   std::memcpy(out, v, sizeof v);
+  return 0;
+}
+
+// LevelSetOperator::Reinitialize with must_do = true
+int orc_reinitialize(orc_solver* h, int special_maxits) {
+  Solver& s = *h->s;
+  s.Reinitialize(0.0, 1.0, 0, special_maxits, true);
+  return s.fatal ? -1 : 0;
+}
+
+double orc_reinit_cfl(orc_solver* h) { return h->s->reinit_cfl; }
+
+// Prescribed-velocity level-set test 1, 2 or 3 (0: off)
+int orc_set_ls_test(orc_solver* h, int test) {
+  h->s->ls_test = test;
   return 0;
 }
 

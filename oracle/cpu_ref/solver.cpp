@@ -613,6 +613,10 @@ void Solver::ComputeAdvectionFluxes(const std::vector<double>& Vin, std::vector<
 
 // SpaceOperator::ComputeResidual (SpaceOperator.cpp:2573-2641)
 void Solver::ComputeResidual(const std::vector<double>& Vin, std::vector<double>& Rout, bool capture) {
+  if (ls_test) {  // SpaceOperator.cpp:2581-2583: no flow residual in the level-set tests
+    std::fill(Rout.begin(), Rout.end(), 0.0);
+    return;
+  }
   ComputeAdvectionFluxes(Vin, Rout, capture);
   if (fatal) return;
   for (int k = 0; k < m.NZ; k++)
@@ -680,8 +684,13 @@ void Solver::ComputeTimeStepSize(const std::vector<double>& Vin, double& dt, dou
         if (m.NY > 1) dxmin = std::min(dxmin, m.DY(j) / lg);
         if (m.NZ > 1) dxmin = std::min(dxmin, m.DZ(k) / lh);
       }
-  cfl = sch.cfl;
-  dt = cfl * dxmin;
+  if (sch.time_step > 0) {  // Time.TimeStep (SpaceOperator.cpp:1315-1321)
+    dt = sch.time_step;
+    cfl = dt / dxmin;
+  } else {
+    cfl = sch.cfl;
+    dt = cfl * dxmin;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -872,7 +881,18 @@ int Solver::UpdateStateVariablesByRiemannSolutions() {
         if (!ok) ok = LocalUpdate(i, j, k, ID[c], false);
         if (!ok) unresolved.push_back({k, j, i});
       }
-  if (!unresolved.empty()) FixUnresolvedNodes(unresolved);
+  if (!unresolved.empty()) {
+    // TimeIntegrator.cpp:771-797: the unresolved cells' level set is updated
+    // and reinitialized (a reinitializer is required: the reference exits
+    // without one), the IDs follow, then the nodes are fixed.
+    cnt.unresolved += (long)unresolved.size();
+    Phi_tmp = Phi;
+    UpdateLevelSetsInUnresolvedCells(unresolved);
+    Reinitialize(t, dts, time_step, 0, true);
+    if (fatal) return -1;
+    UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
+    FixUnresolvedNodes(unresolved);
+  }
   return 0;
 }
 
@@ -973,6 +993,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     if (fatal) return;
     ApplyBoundaryConditions(V);
     if (ls) {
+      if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
       for (int k = 0; k < m.NZ; k++)
         for (int j = 0; j < m.NY; j++)
@@ -1001,6 +1022,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     if (clipped) PrimitiveToConservative(V1, U1);
     ApplyBoundaryConditions(V1);
     if (ls) {
+      if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
       interior([&](long c) { Phi1[c] = 0.0 * Phi1[c] + 1.0 * Phi[c]; });
       interior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + dt * Rls[c]; });
@@ -1018,6 +1040,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (fatal) return;
       ApplyBoundaryConditions(V);
       if (ls) {
+        if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
         interior([&](long c) { Phi[c] = 0.5 * Phi[c] + 0.5 * Phi1[c]; });
         interior([&](long c) { Phi[c] = 1.0 * Phi[c] + hdt * Rls[c]; });
@@ -1036,6 +1059,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (clipped) PrimitiveToConservative(V2, U1);
       ApplyBoundaryConditions(V2);
       if (ls) {
+        if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
         interior([&](long c) { Phi1[c] = 0.25 * Phi1[c] + 0.75 * Phi[c]; });
         interior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + qdt * Rls[c]; });
@@ -1052,6 +1076,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (fatal) return;
       ApplyBoundaryConditions(V);
       if (ls) {
+        if (ls_test) PrescribeVelocityForTesting(V2, Phi1, time - 0.5 * dt);
         LSComputeResidual(V2, Phi1, Rls);
         interior([&](long c) { Phi[c] = 1.0 / 3.0 * Phi[c] + 2.0 / 3.0 * Phi1[c]; });
         interior([&](long c) { Phi[c] = 1.0 * Phi[c] + tdt * Rls[c]; });
@@ -1062,10 +1087,16 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
   if (ls) UpdateSolutionAfterTimeStepping();
 }
 
+// TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
+// 720-795) for one level set and no embedded surfaces: ID update, the
+// level-set reinitialization at its frequency, then the state update.
 void Solver::UpdateSolutionAfterTimeStepping() {
   IDn = ID;
-  UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
+  if (!ls_test) UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
+  Reinitialize(t, dts, time_step, 0, false);
+  if (fatal || ls_test) return;
   UpdateStateVariablesByRiemannSolutions();
+  if (fatal) return;
   ClipDensityAndPressure(V);
   if (fatal) return;
   ApplyBoundaryConditions(V);

@@ -47,6 +47,17 @@ struct RiemannStore {  // RiemannSolutions.h:12-30, keyed by cell linear index
 struct Counters {
   long riemann_calls = 0, riemann_trivial = 0, riemann_errors = 0, riemann_iters = 0, riemann_fevals = 0;
   long riemann_fallbacks = 0, clipped = 0, id_changed = 0, steps = 0;
+  long reinit_calls = 0, reinit_iters = 0, unresolved = 0;
+};
+
+// A first-layer node of the level-set reinitialization with its neighbours
+// across the interface (LevelSetReinitializer.h:55-81).
+struct FirstLayerNode {
+  int i, j, k;
+  bool s[6];
+  int ns;
+  double r[6], r0, f;
+  double nphi0[3];
 };
 
 class Solver {
@@ -96,7 +107,15 @@ class Solver {
   // LevelSetOperator
   void LSApplyBoundaryConditions(std::vector<double>& phi);
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
+  // LevelSetReinitializer, full domain (LevelSetReinitializer.cpp:76-176)
+  bool ReinitEnabled() const { return sch.reinit_frequency > 0 || sch.reinit_frequency_dt > 0; }
+  // LevelSetOperator::Reinitialize (LevelSetOperator.cpp:1338-1363); returns
+  // whether it ran
+  bool Reinitialize(double time, double dt, int time_step, int special_maxIts, bool must_do);
+  void ReinitializeFullDomain(int special_maxIts);
+  double reinit_cfl = 0.8;  // the reinitializer's CFL, reduced by its failsafe
   // MultiPhaseOperator
+  void UpdateLevelSetsInUnresolvedCells(const std::vector<std::array<int, 3>>& unresolved);
   void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
   void UpdateMaterialIDAtGhostNodes();
   int UpdateStateVariablesByRiemannSolutions();
@@ -114,6 +133,24 @@ class Solver {
               double* Vsp);
   bool LocalUpdate(int i, int j, int k, int id, bool upwind);
   void FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved);
+  // reinitialization helpers and work fields (ghost width 1; rNormal 3/cell)
+  std::vector<double> rTag, rSign, rPhibk, rPhi0, rPhi1, rR, rNormal;
+  double PhiOrZero(const std::vector<double>& phi, int i, int j, int k) const;
+  bool TagFirstLayerNodes(std::vector<FirstLayerNode>& fl);
+  void EvaluateSignFunction(double eps);
+  void ReinitializeFirstLayerNodes(std::vector<FirstLayerNode>& fl);
+  void ComputeNormalDirectionBeyondFirstLayer(std::vector<FirstLayerNode>& fl);
+  double ComputeReinitResidual(const std::vector<double>& phi, double cfl);
+  void ApplyCorrectionToFirstLayerNodes(std::vector<FirstLayerNode>& fl, double cfl);
+  double MaxRelativeError();
+
+ public:
+  // Prescribed-velocity level-set tests (LevelSetOperator.cpp:1398-1490,
+  // compiled in the reference with LEVELSET_TEST = 1, 2, 3): the flow
+  // residual is zero, the velocity is prescribed before every level-set
+  // residual, and the material IDs and states are not updated.
+  int ls_test = 0;
+  void PrescribeVelocityForTesting(std::vector<double>& Vio, const std::vector<double>& phi, double time);
 };
 
 }  // namespace orc

@@ -1,0 +1,138 @@
+"""The reference's full-domain level-set tests (Tests/LevelSetTests/*/
+FullDomain/input.st): prescribed velocity, FD3 advection with RK2, full-domain
+reinitialization at a fixed step frequency.  Initial conditions restate
+SpaceInitializer.cpp:1606-1700 (compiled there with LEVELSET_TEST = 1, 2, 3);
+the velocity fields are LevelSetOperator.cpp:1398-1490, restated in the
+oracle."""
+import math
+
+import numpy as np
+
+from m2c_amd import abi, configs
+
+CASES = {
+    "MergingCircles": dict(test=3, n=(128, 128, 1), lo=(-5.0, -5.0, -0.04), hi=(5.0, 5.0, 0.04), dt=0.005,
+                           tmax=3.2, freq=60, maxits=500, tol=1.0e-5, cfl=0.8, flt=abi.REINIT_FIXED),
+    "SlottedDisk": dict(test=1, n=(256, 256, 1), lo=(0.0, 0.0, -0.2), hi=(100.0, 100.0, 0.2), dt=0.3, tmax=628.0,
+                        freq=104, maxits=1000, tol=1.0e-5, cfl=0.8, flt=abi.REINIT_FIXED),
+    "VortexDeformation": dict(test=2, n=(256, 256, 1), lo=(0.0, 0.0, -2.0e-3), hi=(1.0, 1.0, 2.0e-3), dt=0.003,
+                              tmax=8.0, freq=10, maxits=500, tol=1.0e-5, cfl=0.8,
+                              flt=abi.REINIT_ITERATIVE_CONSTRAINED2),
+}
+
+
+def _centres(n, a0, a1):
+    h = (a1 - a0) / n
+    return [a0 + 0.5 * h + i * h for i in range(n)]
+
+
+def _slotted_disk(X, Y):
+    """Zones 1-8 of the slotted disk (Hartmann et al. 2008, problem 4.2.1)."""
+    x0, r, wid, ln = (50.0, 75.0), 15.0, 5.0, 25.0
+    x1 = (x0[0] - 0.5 * wid, x0[1] - math.sqrt(r * r - (0.5 * wid) ** 2))
+    x2 = (x0[0] - 0.5 * wid, x0[1] + (ln - r))
+    x3 = (x0[0] + 0.5 * wid, x0[1] + (ln - r))
+    x4 = (x0[0] + 0.5 * wid, x0[1] - math.sqrt(r * r - (0.5 * wid) ** 2))
+    alpha = math.atan((x4[0] - x0[0]) / (x0[1] - x4[1]))
+
+    def dist(a, b):
+        return math.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2)
+
+    p = (X, Y)
+    if Y <= x1[1] and math.atan(abs((X - x0[0]) / (Y - x0[1]))) <= alpha:
+        return dist(p, x1) if X <= x0[0] else dist(p, x4)
+    if dist(p, x0) >= r:
+        return dist(p, x0) - r
+    if x1[0] <= X <= x4[0] and Y <= x2[1]:
+        return min(x4[0] - X, min(X - x1[0], x2[1] - Y))
+    if X <= x2[0] and Y <= x2[1]:
+        return -min(r - dist(p, x0), x2[0] - X)
+    if X >= x3[0] and Y <= x3[1]:
+        return -min(r - dist(p, x0), X - x3[0])
+    if X <= x2[0]:
+        return -min(r - dist(p, x0), dist(p, x2))
+    if X <= x3[0]:
+        return -min(r - dist(p, x0), Y - x2[1])
+    return -min(r - dist(p, x0), dist(p, x3))
+
+
+def initial_phi(name):
+    """Interior phi [k][j][i] at t = 0."""
+    c = CASES[name]
+    xs = _centres(c["n"][0], c["lo"][0], c["hi"][0])
+    ys = _centres(c["n"][1], c["lo"][1], c["hi"][1])
+    out = np.zeros((1, len(ys), len(xs)))
+    for j, Y in enumerate(ys):
+        for i, X in enumerate(xs):
+            if c["test"] == 3:  # two circles, a = 1.75, r = 1.5
+                p1 = math.sqrt((X - 1.75) * (X - 1.75) + Y * Y) - 1.5
+                p2 = math.sqrt((X + 1.75) * (X + 1.75) + Y * Y) - 1.5
+                v = p1 if p1 <= 0 else (p2 if p2 <= 0 else min(p1, p2))
+            elif c["test"] == 2:  # circle r = 0.15 at (0.5, 0.75)
+                v = math.sqrt((X - 0.5) ** 2 + (Y - 0.75) ** 2) - 0.15
+            else:
+                v = _slotted_disk(X, Y)
+            out[0, j, i] = v
+    return out
+
+
+def setup(name):
+    """grid, materials, scheme, V, ID, Phi (ghosted) of a test deck; the two
+    materials and the flow state are placeholders (no flow is solved)."""
+    c = CASES[name]
+    g = abi.grid(c["n"], c["lo"], c["hi"])
+    mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.4))
+    s = abi.default_scheme(integrator=abi.TI_RK2, n_levelsets=1, ls_materialid=1, max_time=c["tmax"],
+                           time_step=c["dt"], reinit_frequency=c["freq"], reinit_maxits=c["maxits"],
+                           reinit_tol=c["tol"], reinit_cfl=c["cfl"], reinit_first_layer=c["flt"])
+    for q, b in enumerate([abi.LSBC_LINEAR_EXTRAPOLATION] * 4 + [abi.LSBC_ZERO_NEUMANN] * 2):
+        s.ls_bc[q] = b
+    nx, ny, nz = c["n"]
+    V = np.zeros((nz + 2, ny + 2, nx + 2, 5))
+    V[..., 0] = 1.0
+    V[..., 4] = 1.0e5
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    Phi = np.zeros(V.shape[:3])
+    Phi[1:-1, 1:-1, 1:-1] = initial_phi(name)
+    return g, mats, 2, s, V, ID, Phi
+
+
+def stretched_sphere(n=(28, 24, 20), flt=abi.REINIT_FIXED, maxits=400, bc=abi.LSBC_LINEAR_EXTRAPOLATION):
+    """A sphere's distance function stretched by a smooth positive factor:
+    same zero level set, |grad phi| far from 1."""
+    g = abi.grid(n, (0, 0, 0), (1.4, 1.2, 1.0))
+    mats = abi.materials(abi.material("SG"), abi.material("SG"))
+    s = abi.default_scheme(n_levelsets=1, reinit_frequency=1, reinit_maxits=maxits, reinit_tol=1e-6,
+                           reinit_first_layer=flt)
+    for q in range(6):
+        s.ls_bc[q] = bc
+    x = np.array(configs._axis(n[0], 0, 1.4))[None, None, :]
+    y = np.array(configs._axis(n[1], 0, 1.2))[None, :, None]
+    z = np.array(configs._axis(n[2], 0, 1.0))[:, None, None]
+    r = np.sqrt((x - 0.7) ** 2 + (y - 0.6) ** 2 + (z - 0.5) ** 2)
+    d = r - 0.3
+    phi = d * (1.6 + 0.5 * np.sin(3 * x + 2 * y) * np.cos(2 * z))
+    V = np.zeros(phi.shape + (5,))
+    V[..., 0] = 1.0
+    V[..., 4] = 1.0
+    ID = (phi < 0).astype(np.int32)
+    return g, mats, 2, s, V, ID, phi, d
+
+
+def unresolved_case(reinit):
+    """A plane level set whose material IDs were never set: the moving
+    interface changes cells to material 1 with no interface Riemann solution
+    next to them, so the state update leaves them unresolved."""
+    n = (24, 8, 8)
+    g = abi.grid(n, (0, 0, 0), (1.0, 1 / 3, 1 / 3))
+    mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.6))
+    s = abi.default_scheme(n_levelsets=1, integrator=abi.TI_RK2,
+                           reinit_frequency=1000 if reinit else -1, reinit_maxits=40)
+    x = np.array(configs._axis(n[0], 0, 1.0))[None, None, :]
+    V = np.zeros((n[2] + 2, n[1] + 2, n[0] + 2, 5))
+    V[..., 0] = 1.0
+    V[..., 1] = 1.0
+    V[..., 4] = 1.0
+    phi = np.broadcast_to(x - 0.31, V.shape[:3]).copy()
+    ID = np.zeros(V.shape[:3], dtype=np.int32)
+    return g, mats, 2, s, V, ID, phi

@@ -57,6 +57,10 @@ def lib():
         L.orc_cr_root.restype = C.c_double
         L.orc_cr_root.argtypes = [C.c_double, C.c_int]
         L.orc_set_ck_textbook.argtypes = [C.c_int]
+        L.orc_reinitialize.argtypes = [P, C.c_int]
+        L.orc_reinit_cfl.restype = C.c_double
+        L.orc_reinit_cfl.argtypes = [P]
+        L.orc_set_ls_test.argtypes = [P, C.c_int]
         _lib = L
     return _lib
 
@@ -147,11 +151,25 @@ class Oracle:
         return t.value, dt.value, st.value
 
     def counters(self):
-        out = (C.c_long * 9)()
+        out = (C.c_long * 12)()
         self.L.orc_counters(self.h, out)
         keys = ["riemann_calls", "riemann_trivial", "riemann_errors", "riemann_iters", "riemann_fevals",
-                "riemann_fallbacks", "clipped", "id_changed", "steps"]
+                "riemann_fallbacks", "clipped", "id_changed", "steps", "reinit_calls", "reinit_iterations",
+                "unresolved_cells"]
         return dict(zip(keys, list(out)))
+
+    def reinitialize(self, special_maxits=0):
+        """LevelSetOperator::Reinitialize with must_do = true."""
+        if self.L.orc_reinitialize(self.h, special_maxits) < 0:
+            raise RuntimeError(self.error())
+
+    def reinit_cfl(self):
+        return self.L.orc_reinit_cfl(self.h)
+
+    def set_ls_test(self, test):
+        """Prescribed-velocity level-set test 1 (slotted disk), 2 (vortex
+        deformation) or 3 (merging circles); 0 switches it off."""
+        self.L.orc_set_ls_test(self.h, test)
 
 
     def riemann_log_enable(self, on=True):
