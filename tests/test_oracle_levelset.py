@@ -29,13 +29,22 @@ import levelset_cases as lc
 import oracle_lib as ol
 from m2c_amd import abi
 
-FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "levelset_tests.npz"))
+_FIX = {}
+
+
+def _fix(key):
+    """The converted reference results (loaded on first use: the file does not
+    travel with the GPU tests)."""
+    if not _FIX:
+        with np.load(os.path.join(os.path.dirname(__file__), "golden", "levelset_tests.npz")) as f:
+            _FIX.update({k: f[k] for k in f.files})
+    return _FIX[key]
 
 
 @pytest.mark.parametrize("name", sorted(lc.CASES))
 def test_initial_conditions_match_reference_snapshot(name):
     phi0 = lc.initial_phi(name)
-    assert hashlib.sha256(phi0[0].tobytes()).digest() == FIX[name + "_initial_sha256"].tobytes()
+    assert hashlib.sha256(phi0[0].tobytes()).digest() == _fix(name + "_initial_sha256").tobytes()
 
 
 def _replay(name):
@@ -54,11 +63,11 @@ def _replay(name):
         os.environ.get("M2C_LONG_FIXTURES") != "1", reason="about 5 minutes; set M2C_LONG_FIXTURES=1"))])
 def test_levelset_test_deck_final_phi(name):
     o = _replay(name)
-    assert o.time()[2] == int(FIX[name + "_steps"])
+    assert o.time()[2] == int(_fix(name + "_steps"))
     c = o.counters()
-    assert c["reinit_calls"] == int(FIX[name + "_steps"]) // lc.CASES[name]["freq"]
+    assert c["reinit_calls"] == int(_fix(name + "_steps")) // lc.CASES[name]["freq"]
     _, _, P = o.get_state()
-    ref = FIX[name + "_final"]
+    ref = _fix(name + "_final")
     err = np.abs(P[1:-1, 1:-1, 1:-1][0] - ref).max()
     assert err <= 1e-12 * np.abs(ref).max(), err
 
