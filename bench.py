@@ -74,7 +74,8 @@ def build_config(name):
         return configs.c5_bubble_cloud(), ("bubble cloud 512x256x256: NASG water (1.19, 6.2178e8, 6.7212e-4, "
                                            "-1.177788e6) with 200 air bubbles r 0.005-0.03 L, Mach-2 shock, exact "
                                            "interface Riemann tol 1e-10, HLLC, van Albada, RK3, CFL 0.4, level set "
-                                           "FD3 without reinitialisation")
+                                           "FD3 reinitialised every 10 steps (full domain, Fixed first layer, "
+                                           "30 iterations at most)")
     raise SystemExit("unknown config " + name)
 
 
@@ -192,10 +193,14 @@ def run_single(cfg, steps, warmup, learned_params=None):
     stage_ms, stage_n = solver.kernel_time("stage")
     riem_ms, _ = solver.kernel_time("riemann")
     mlp_ms, _ = solver.kernel_time("mlp")
+    reinit_ms, _ = solver.kernel_time("reinit")
     res = {"value": cells * stages * steps / el, "ms_per_step": el * 1e3 / steps, "cells": cells,
            "stages": stages, "stage_ms": stage_ms, "stage_launches": stage_n,
            "riemann": dict(riemann_stats(cnt, hist), riemann_ms_per_step=riem_ms / steps,
                            mlp_ms_per_step=mlp_ms / steps)}
+    if cnt["reinit_calls"]:
+        res["reinit"] = {"calls": cnt["reinit_calls"], "iterations": cnt["reinit_iterations"],
+                         "ms_per_call": reinit_ms / cnt["reinit_calls"], "ms_per_step": reinit_ms / steps}
     solver.close()
     return res
 
@@ -249,6 +254,7 @@ def main():
         "config": {"workload": args.config, "description": desc, "cells": cells, "rk_stages": plain["stages"],
                    "riemann_guess": "acoustic (plain)"},
         "riemann": plain["riemann"],
+        "reinit": plain.get("reinit"),
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused residual + RK stage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(args.config),

@@ -178,19 +178,21 @@ def shock_bubble_geometries(L, radius, mach, shock_x, water, rho_w=1000.0, p_w=1
 
 
 def c5_bubble_cloud(n=(512, 256, 256), nbubbles=200, seed=5, mach=2.0, shock_x=0.1, r_range=(0.005, 0.03),
-                    gap=0.002):
+                    gap=0.002, reinit_every=10):
     """Bubble cloud: NASG water (gamma 1.19, p_inf 6.2178e8, b 6.7212e-4,
     q -1.177788e6) on [0, 2L] x [0, L] x [0, L] with `nbubbles` non-overlapping
     ideal-gas bubbles (radius log-uniform in r_range x L, seeded centres by
     rejection, all downstream of the shock), hit by a planar Mach-2 shock.
-    HLLC bulk, exact interface Riemann, RK3.  Returns (grid, mats, nmat,
-    scheme, default_state, default_id, geometries) for Solver.initialize."""
+    HLLC bulk, exact interface Riemann, RK3, the level set reinitialized every
+    `reinit_every` steps (full domain, Fixed first layer, the reference's
+    defaults otherwise).  Returns (grid, mats, nmat, scheme, default_state,
+    default_id, geometries) for Solver.initialize."""
     L = 1.0
     lx = L * n[0] / n[2]
     g = abi.grid(n, (0.0, 0.0, 0.0), (lx, L, L), (abi.BC_OUTLET,) * 6)
     mats = abi.materials(abi.material("NASG", **WATER_NASG), abi.material("SG", **AIR_SG))
     s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK3, cfl=0.4,
-                           n_levelsets=1, ls_materialid=1, riem_tol_main=1e-10)
+                           n_levelsets=1, ls_materialid=1, riem_tol_main=1e-10, reinit_frequency=reinit_every)
     rng = np.random.default_rng(seed)
     placed = []
     tries = 0

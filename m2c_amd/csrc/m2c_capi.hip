@@ -722,14 +722,13 @@ static int reinit_full(m2cx_solver* h, int special_maxits) {
         k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, b.pa, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
         k_rstage2<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, cfl, tol);
         k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, b.pb, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
-        k_rstage3<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, cfl, tol);
+        k_rstage3<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, fix, hcr ? 1 : 0, cfl, tol);
         k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
         if (hcr) {
           k_rcorr_f<<<lb, 256, 0, st>>>(g, b, phi, flt, tol);
           k_rcorr_apply<<<lb, 256, 0, st>>>(g, b, phi, cfl, tol);
           k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
         }
-        k_rerr<<<blocks_for(nin), 256, 0, st>>>(g, b, phi, tol);
         k_rcheck<<<1, 1, 0, st>>>(b.st, tol);
       }
       launched += chunk;

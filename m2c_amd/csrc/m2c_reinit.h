@@ -47,10 +47,19 @@ __device__ __forceinline__ double wave_max(double m) {
   return m;
 }
 
-// one atomic per wave; v >= 0
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v) {
+// maximum of v >= 0 over the work-group (256 threads), then one atomic per
+// work-group: atomics on one address serialize, so one per wave would cost
+// more than the pass itself
+__device__ __forceinline__ void block_max_atomic(unsigned long long* dst, double v) {
+  __shared__ double red[4];
   v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) atomicMax(dst, (unsigned long long)__double_as_longlong(v));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (m < red[w]) ? red[w] : m;
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // Phi of the reference never holds edge or corner ghosts (cells outside the
@@ -384,7 +393,7 @@ __global__ void k_rstage1(DevGrid g, RBuf b, const double* __restrict__ phi, int
     double o = 0.0 * b.pa[c] + 1.0 * phi[c];
     b.pa[c] = 1.0 * o + 1.0 * r;
   }
-  atomic_max_nonneg(&b.st->res_bits, m);
+  block_max_atomic(&b.st->res_bits, m);
 }
 
 // stage 2 (:136-142): Phi2 = 0.25 Phi1 + 0.75 Phi + 0.25 R(Phi1)
@@ -402,10 +411,22 @@ __global__ void k_rstage2(DevGrid g, RBuf b, const double* __restrict__ phi, int
   }
 }
 
-// stage 3 (:145-150): Phi = 1/3 Phi + 2/3 Phi2 + 2/3 R(Phi2), in place
-__global__ void k_rstage3(DevGrid g, RBuf b, double* __restrict__ phi, int fix, double cfl, double tol) {
+// relative change of one node (CalculateMaximumRelativeErrorFullDomain,
+// :1966-1994)
+__device__ __forceinline__ double rel_change(double phi, double phi0) {
+  double e = fabs(phi - phi0);
+  if (phi != 0) e /= fabs(phi);
+  return e;
+}
+
+// stage 3 (:145-150): Phi = 1/3 Phi + 2/3 Phi2 + 2/3 R(Phi2), in place, with
+// the relative change of the nodes that keep this value: all of them, or
+// with HCR forcing the nodes off the first layer (k_rcorr_apply takes the
+// first layer after its correction)
+__global__ void k_rstage3(DevGrid g, RBuf b, double* __restrict__ phi, int fix, int hcr, double cfl, double tol) {
   if (reinit_skip(b.st, tol, true)) return;
   const long n = (long)g.NX * g.NY * g.NZ;
+  double m = 0.0;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     int i, j, k;
     cell_ijk(g, q, i, j, k);
@@ -413,8 +434,14 @@ __global__ void k_rstage3(DevGrid g, RBuf b, double* __restrict__ phi, int fix, 
     double ar;
     const double r = reinit_residual(g, b, b.pb, c, i, j, k, fix != 0, cfl, &ar);
     double o = 1.0 / 3.0 * phi[c] + 2.0 / 3.0 * b.pb[c];
-    phi[c] = 1.0 * o + 2.0 / 3.0 * r;
+    o = 1.0 * o + 2.0 / 3.0 * r;
+    phi[c] = o;
+    if (!hcr || b.tag[c] == 0) {
+      const double e = rel_change(o, b.phi0[c]);
+      m = (m < e) ? e : m;
+    }
   }
+  block_max_atomic(&b.st->dphi_bits, m);
 }
 
 __global__ void k_rbc(DevGrid g, double* phi, int b0, int b1, int b2, int b3, int b4, int b5, const ReinitState* st,
@@ -466,6 +493,7 @@ __global__ void k_rcorr_f(DevGrid g, RBuf b, const double* __restrict__ phi, int
 __global__ void k_rcorr_apply(DevGrid g, RBuf b, double* __restrict__ phi, double cfl, double tol) {
   if (reinit_skip(b.st, tol, true)) return;
   const int n = min(b.st->nfl, b.cap);
+  double m = 0.0;
   const double* dxs = g_axis(g, 3);
   const double* dys = g_axis(g, 4);
   const double* dzs = g_axis(g, 5);
@@ -479,23 +507,10 @@ __global__ void k_rcorr_apply(DevGrid g, RBuf b, double* __restrict__ phi, doubl
     const double dmin = (dyz < dxs[i + 1]) ? dyz : dxs[i + 1];
     const double dt = cfl * dmin;
     phi[c] += dt * 0.5 * b.fl_f[q];
-  }
-}
-
-// CalculateMaximumRelativeErrorFullDomain (:1966-1994)
-__global__ void k_rerr(DevGrid g, RBuf b, const double* __restrict__ phi, double tol) {
-  if (reinit_skip(b.st, tol, true)) return;
-  const long n = (long)g.NX * g.NY * g.NZ;
-  double m = 0.0;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    cell_ijk(g, q, i, j, k);
-    const long c = gidx(g, i, j, k);
-    double e = fabs(phi[c] - b.phi0[c]);
-    if (phi[c] != 0) e /= fabs(phi[c]);
+    const double e = rel_change(phi[c], b.phi0[c]);
     m = (m < e) ? e : m;
   }
-  atomic_max_nonneg(&b.st->dphi_bits, m);
+  block_max_atomic(&b.st->dphi_bits, m);
 }
 
 // end of one iteration (:118-128, :157-164): stop on a small residual before
