@@ -61,7 +61,7 @@ enum { M2CX_TI_FORWARD_EULER = 0, M2CX_TI_RK2 = 1, M2CX_TI_RK3 = 2 };
 enum { M2CX_MPFLUX_EXACT = 0, M2CX_MPFLUX_NUMERICAL = 1, M2CX_MPFLUX_LLF = 2 };
 enum { M2CX_MPREC_CONSTANT = 0, M2CX_MPREC_LINEAR = 1 };
 enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0 };
-enum { M2CX_RNORMAL_MESH = 1 };
+enum { M2CX_RNORMAL_LEVEL_SET = 0, M2CX_RNORMAL_MESH = 1, M2CX_RNORMAL_AVERAGE = 2 };
 
 /* LevelSetReinitializationData::FirstLayerTreatment (IoData.h:754-756) */
 enum { M2CX_REINIT_FIXED = 0, M2CX_REINIT_CONSTRAINED1 = 1, M2CX_REINIT_CONSTRAINED2 = 2,
@@ -122,7 +122,7 @@ typedef struct {
   int32_t mp_flux;             /* M2CX_MPFLUX_*                                */
   int32_t mp_recon;            /* M2CX_MPREC_*                                 */
   int32_t mp_phase_change;     /* M2CX_PHASECHANGE_*                           */
-  int32_t riemann_normal;      /* M2CX_RNORMAL_MESH                            */
+  int32_t riemann_normal;      /* RiemannNormal, M2CX_RNORMAL_* (Mesh)          */
   int32_t riem_maxits_main;    /* MaxIts (200)                                 */
   int32_t riem_maxits_bracket; /* MaxItsBracketing (100)                       */
   int32_t riem_maxits_shock;   /* MaxItsShock (200)                            */

@@ -18,6 +18,7 @@ TI_FE, TI_RK2, TI_RK3 = 0, 1, 2
 MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
+RNORMAL_LEVEL_SET, RNORMAL_MESH, RNORMAL_AVERAGE = 0, 1, 2
 REINIT_FIXED, REINIT_CONSTRAINED1, REINIT_CONSTRAINED2, REINIT_ITERATIVE_CONSTRAINED1, \
     REINIT_ITERATIVE_CONSTRAINED2 = range(5)
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
@@ -125,7 +126,7 @@ def default_scheme(**kw):
     s.mp_flux = MPFLUX_NUMERICAL
     s.mp_recon = MPREC_CONSTANT
     s.mp_phase_change = 0
-    s.riemann_normal = 1
+    s.riemann_normal = RNORMAL_MESH
     s.riem_maxits_main = 200
     s.riem_maxits_bracket = 100
     s.riem_maxits_shock = 200

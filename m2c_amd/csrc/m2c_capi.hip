@@ -189,6 +189,11 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
   if (sch->reinit_first_layer < M2CX_REINIT_FIXED || sch->reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2)
     return M2CX_E_ARG;
+  if (sch->riemann_normal < M2CX_RNORMAL_LEVEL_SET || sch->riemann_normal > M2CX_RNORMAL_AVERAGE) return M2CX_E_ARG;
+  // the level-set normal is defined for faces between material 0 and the
+  // level set's material (CalculateGradPhiAtCellInterface asserts it)
+  if (sch->riemann_normal != M2CX_RNORMAL_MESH && (sch->n_levelsets != 1 || nmat != 2 || sch->ls_materialid != 1))
+    return M2CX_E_UNSUPPORTED;
   auto* h = new m2cx_solver;
   *out = h;
   h->grid = *grid;
@@ -252,6 +257,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   s.nmat = nmat; s.inactive = nmat;
   s.gmm = sch->gmm_coeff; s.del = sch->entropy_fix; s.cfl = sch->cfl;
   s.time_step = sch->time_step;
+  s.rnormal = sch->riemann_normal;
   h->reinit_cfl = sch->reinit_cfl;
   s.var_type = sch->var_type;
   DevMats& M = h->M;

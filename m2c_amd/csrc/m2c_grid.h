@@ -61,6 +61,7 @@ struct Scheme {
   int mset;                     // slot of the solver's material table in c_mats
   double gmm, del, cfl;
   double time_step;             // Time.TimeStep: fixed dt when > 0
+  int rnormal;                  // MultiPhase.RiemannNormal, M2CX_RNORMAL_*
   // per cell (ghosted index): 1 where a reconstruction in var_type fails
   // CheckState on some face and the cell falls back to primitive variables
   // (Reconstructor.cpp:722-778); written by k_recon_flags each stage

@@ -36,6 +36,7 @@ struct IfaceList {
   int* err;       // Riemann error of the capture
   int* stats;     // [3][cap]: main-loop iterations, f(p) evaluations, flags (last solve)
   unsigned long long* hist;  // [2][M2CX_RIEMANN_HIST_BINS]: counts of iterations, of f(p) evaluations
+  const double* phi;         // level set at time n (the Riemann normal of LevelSet / Average)
   int cap;
 };
 

@@ -89,6 +89,7 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
     if (hipMalloc(p, n * 8) != hipSuccess) return M2CX_E_OOM;
     if (hipMemsetAsync(*p, 0, n * 8, st) != hipSuccess) return M2CX_E_HIP;
   }
+  B.L.phi = B.phi;
   // FD3 tables (GradientCalculatorFD3.cpp:300-453), per axis because the mesh
   // is a tensor product; cyR1 holds the first-order right-biased y stencil the
   // reference uses where its `i<NY-1` test (:400) fails.
@@ -215,6 +216,71 @@ __device__ __forceinline__ void if_states(const DevGrid& g, const Scheme& s, con
   face_state(g, s, M, V, ID, D, i, j, k, 0, Vp, f, u);
 }
 
+// CentralDifferenceLocal (SpaceOperator.h:223-228)
+__device__ __forceinline__ double cdiff3(double phi0, double phi1, double phi2, double x0, double x1, double x2) {
+  double c0 = -(x2 - x1) / ((x1 - x0) * (x2 - x0));
+  double c1 = 1.0 / (x1 - x0) - 1.0 / (x2 - x1);
+  double c2 = (x1 - x0) / ((x2 - x0) * (x2 - x1));
+  return c0 * phi0 + c1 * phi1 + c2 * phi2;
+}
+
+// GetNormalForBimaterialRiemann / CalculateGradPhiAtCellInterface /
+// CalculateGradientAtCellInterface (SpaceOperator.cpp:2186-2415) for one
+// level set at the face of cell c on axis d (minus side c - stride): grad phi
+// with its normal part from the two cells and tangential parts from central
+// differences in both columns weighted to the face, normalized; pointing from
+// the level set's material into material 0; its axis component made
+// non-negative; Average adds the axis and normalizes again.  Interface faces
+// lie between interior cells, so the columns' neighbours are interior cells
+// or face ghosts (the reference's one-sided branches at ghost rows are not
+// reached).
+__device__ __forceinline__ void bimat_normal(const DevGrid& g, int rnormal, const double* __restrict__ phi, int d,
+                                             long c, int idl, double* dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  if (rnormal == M2CX_RNORMAL_MESH) {
+    dir[d] = 1.0;
+    return;
+  }
+  long r = c;
+  const int i = (int)(r % g.SX) - 1; r /= g.SX;
+  const int j = (int)(r % g.SY) - 1;
+  const int k = (int)(r / g.SY) - 1;
+  const int ijk[3] = {i, j, k};
+  const long st3[3] = {1, g.SX, g.S};
+  const double* X = g_axis(g, d);
+  const double* DX = g_axis(g, 3 + d);
+  const long sd = st3[d];
+  const double xc = X[ijk[d] + 1], xm = X[ijk[d]];
+  dir[d] = (phi[c] - phi[c - sd]) / (xc - xm);
+  const double xh = xc - 0.5 * DX[ijk[d] + 1];
+  const double dx = xc - xm;
+  const double c1 = (xc - xh) / dx;
+  const double c2 = (xh - xm) / dx;
+  for (int e = 0; e < 3; e++) {
+    if (e == d) continue;
+    const double* Y = g_axis(g, e);
+    const long se = st3[e];
+    const int q = ijk[e];
+    const double t1 = cdiff3(phi[c - sd - se], phi[c - sd], phi[c - sd + se], Y[q], Y[q + 1], Y[q + 2]);
+    const double t2 = cdiff3(phi[c - se], phi[c], phi[c + se], Y[q], Y[q + 1], Y[q + 2]);
+    dir[e] = c1 * t1 + c2 * t2;
+  }
+  double nrm = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (nrm != 0.0) {
+    const double inv = 1.0 / nrm;  // Vec3D::operator/= multiplies by the reciprocal
+    dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+  }
+  if (idl == 0)
+    for (int e = 0; e < 3; e++) dir[e] *= -1.0;
+  if (sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  if (dir[d] < 0.0) dir[d] *= -1.0;
+  if (rnormal == M2CX_RNORMAL_AVERAGE) {
+    dir[d] += 1.0;
+    const double inv = 1.0 / sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+  }
+}
+
 // One interface face: the face states, the exact solve and the two fluxes
 // into the list (SpaceOperator.cpp:1613-1690), shared by the single-solver
 // and the ensemble kernels.  Returns flag bits: 1 trivial, 2 error, 4
@@ -229,7 +295,8 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   const int idl = ID[c - st], idr = ID[c];
   double Vm[5], Vp[5];
   if_states(g, s, M, V, ID, d, c, Vm, Vp);
-  const double dir[3] = {(d == 0) ? 1.0 : 0.0, (d == 1) ? 1.0 : 0.0, (d == 2) ? 1.0 : 0.0};
+  double dir[3];
+  bimat_normal(g, s.rnormal, L.phi, d, c, idl, dir);
   // learned p* guess for this face (m2c_mlp.h), rejected when not a finite
   // positive pressure: the acoustic guess is then used
   double p0g = -1.0;
@@ -1049,7 +1116,10 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
     case 4: a.phi_st = B.phi1; a.phi_a = B.phi; a.phi_b = B.phi1; a.out = B.phi; written = B.phi; break;
   }
   k_levelset<<<blocks_for((long)g.NX * g.NY * g.NZ), 256, 0, st>>>(a);
-  if (which == 0) std::swap(B.phi, B.phitmp);
+  if (which == 0) {
+    std::swap(B.phi, B.phitmp);
+    B.L.phi = B.phi;
+  }
   if (which == 3) std::swap(B.phi1, B.phitmp);
   if (written) {
     long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);

@@ -129,6 +129,12 @@ int orc_counters(orc_solver* h, long* out) {
   return 0;
 }
 
+// The Riemann direction of face d of cell (i, j, k) from the current level set
+int orc_bimaterial_normal(orc_solver* h, int d, int i, int j, int k, int myid, int neighborid, double* dir) {
+  h->s->BimaterialNormal(d, i, j, k, myid, neighborid, dir);
+  return 0;
+}
+
 // LevelSetOperator::Reinitialize with must_do = true
 int orc_reinitialize(orc_solver* h, int special_maxits) {
   Solver& s = *h->s;

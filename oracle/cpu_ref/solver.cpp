@@ -516,10 +516,81 @@ void Solver::CheckReconstructedStates(const std::vector<double>& Vin) {
       }
 }
 
-int Solver::Riemann(int d, const double* Vm, int idm, const double* Vp, int idp, double* Vs, int& mid, double* Vsm,
-                    double* Vsp) {
-  double dir[3] = {0.0, 0.0, 0.0};
-  dir[d] = 1.0;  // GetNormalForBimaterialRiemann, MESH (SpaceOperator.cpp:2186-2203)
+// CentralDifferenceLocal (SpaceOperator.h:223-228)
+static inline double central_diff3(double phi0, double phi1, double phi2, double x0, double x1, double x2) {
+  double c0 = -(x2 - x1) / ((x1 - x0) * (x2 - x0));
+  double c1 = 1.0 / (x1 - x0) - 1.0 / (x2 - x1);
+  double c2 = (x1 - x0) / ((x2 - x0) * (x2 - x1));
+  return c0 * phi0 + c1 * phi1 + c2 * phi2;
+}
+
+// SpaceOperator::CalculateGradientAtCellInterface (SpaceOperator.cpp:
+// 2272-2415): grad phi at face d of cell (i, j, k) (between i-1 and i for
+// d = 0), normal part by the two cells, tangential parts from central
+// differences in the two columns weighted to the face, normalized.
+void Solver::GradPhiAtFace(int d, int i, int j, int k, double* dir) const {
+  auto P = [&](int a, int b, int c) { return Phi[m.idx(a, b, c)]; };
+  auto X = [&](int a) { return m.X(a); };
+  auto Y = [&](int a) { return m.Y(a); };
+  auto Z = [&](int a) { return m.Z(a); };
+  // derivative along axis e in the column through (a, b, c), one-sided at the
+  // domain's ghost rows as the reference does
+  auto tang = [&](int e, int a, int b, int c, int Nl) -> double {
+    int q = e == 0 ? a : (e == 1 ? b : c);
+    auto at = [&](int o) { return e == 0 ? P(a + o, b, c) : (e == 1 ? P(a, b + o, c) : P(a, b, c + o)); };
+    auto co = [&](int o) { return e == 0 ? X(a + o) : (e == 1 ? Y(b + o) : Z(c + o)); };
+    if (q < 0) return (at(1) - at(0)) / (co(1) - co(0));
+    if (q >= Nl) return (at(0) - at(-1)) / (co(0) - co(-1));
+    return central_diff3(at(-1), at(0), at(1), co(-1), co(0), co(1));
+  };
+  dir[0] = dir[1] = dir[2] = 0.0;
+  int im = i, jm = j, km = k;
+  double xc, xm, dxc;
+  if (d == 0) { im = i - 1; xc = X(i); xm = X(i - 1); dxc = m.DX(i); }
+  else if (d == 1) { jm = j - 1; xc = Y(j); xm = Y(j - 1); dxc = m.DY(j); }
+  else { km = k - 1; xc = Z(k); xm = Z(k - 1); dxc = m.DZ(k); }
+  dir[d] = (P(i, j, k) - P(im, jm, km)) / (xc - xm);
+  const double xh = xc - 0.5 * dxc;
+  const double dx = xc - xm;
+  const double c1 = (xc - xh) / dx;
+  const double c2 = (xh - xm) / dx;
+  const int Ns[3] = {m.NX, m.NY, m.NZ};
+  for (int e = 0; e < 3; e++) {
+    if (e == d) continue;
+    const double t1 = tang(e, im, jm, km, Ns[e]);
+    const double t2 = tang(e, i, j, k, Ns[e]);
+    dir[e] = c1 * t1 + c2 * t2;
+  }
+  double nrm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (nrm != 0.0) {
+    double inv = 1.0 / nrm;  // Vec3D::operator/= multiplies by the reciprocal
+    dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+  }
+}
+
+// SpaceOperator::GetNormalForBimaterialRiemann and CalculateGradPhiAtCellInterface
+// (SpaceOperator.cpp:2186-2270) with one level set: the face's direction of the
+// 1D Riemann problem (myid: this cell, neighborid: the cell on the minus side)
+void Solver::BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  if (sch.riemann_normal == M2CX_RNORMAL_MESH) {
+    dir[d] = 1.0;
+    return;
+  }
+  GradPhiAtFace(d, i, j, k, dir);
+  if (neighborid == 0)
+    for (int q = 0; q < 3; q++) dir[q] *= -1.0;
+  if (std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  if (dir[d] < 0.0) dir[d] *= -1.0;  // aligned with the positive grid axis
+  if (sch.riemann_normal == M2CX_RNORMAL_AVERAGE) {
+    dir[d] += 1.0;
+    double inv = 1.0 / std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+  }
+}
+
+int Solver::Riemann(const double* dir, int d, const double* Vm, int idm, const double* Vp, int idp, double* Vs,
+                    int& mid, double* Vsm, double* Vsp) {
   riem.st = RiemannStats();
   int err = riem.Solve(dir, Vm, idm, Vp, idp, Vs, mid, Vsm, Vsp);
   cnt.riemann_calls++;
@@ -578,7 +649,9 @@ void Solver::ComputeAdvectionFluxes(const std::vector<double>& Vin, std::vector<
               } else {
                 double Vmid[5], Vsm[5], Vsp[5];
                 int midid = -1;
-                int err = Riemann(d, Vm, nid, Vp, myid, Vmid, midid, Vsm, Vsp);
+                double dir[3];
+                BimaterialNormal(d, i, j, k, myid, nid, dir);
+                int err = Riemann(dir, d, Vm, nid, Vp, myid, Vmid, midid, Vsm, Vsp);
                 if (err) cnt.riemann_errors++;
                 vf[nid].ClipDensityAndPressure(Vsm);
                 vf[myid].ClipDensityAndPressure(Vsp);

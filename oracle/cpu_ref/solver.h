@@ -107,6 +107,9 @@ class Solver {
   // LevelSetOperator
   void LSApplyBoundaryConditions(std::vector<double>& phi);
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
+  // direction of the 1D Riemann problem at material interfaces
+  void GradPhiAtFace(int d, int i, int j, int k, double* dir) const;
+  void BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const;
   // LevelSetReinitializer, full domain (LevelSetReinitializer.cpp:76-176)
   bool ReinitEnabled() const { return sch.reinit_frequency > 0 || sch.reinit_frequency_dt > 0; }
   // LevelSetOperator::Reinitialize (LevelSetOperator.cpp:1338-1363); returns
@@ -129,8 +132,9 @@ class Solver {
   void SetupMesh();
   void SetupGhosts();
   void SetupFD3();
-  int Riemann(int dir, const double* Vm, int idm, const double* Vp, int idp, double* Vs, int& mid, double* Vsm,
-              double* Vsp);
+  int Riemann(const double* dir, int d, const double* Vm, int idm, const double* Vp, int idp, double* Vs, int& mid,
+              double* Vsm, double* Vsp);
+
   bool LocalUpdate(int i, int j, int k, int id, bool upwind);
   void FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved);
   // reinitialization helpers and work fields (ghost width 1; rNormal 3/cell)

@@ -61,6 +61,7 @@ def lib():
         L.orc_reinit_cfl.restype = C.c_double
         L.orc_reinit_cfl.argtypes = [P]
         L.orc_set_ls_test.argtypes = [P, C.c_int]
+        L.orc_bimaterial_normal.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D]
         _lib = L
     return _lib
 
@@ -165,6 +166,13 @@ class Oracle:
 
     def reinit_cfl(self):
         return self.L.orc_reinit_cfl(self.h)
+
+    def bimaterial_normal(self, d, i, j, k, myid, neighborid):
+        """Direction of the 1D Riemann problem at face d of cell (i, j, k)
+        (GetNormalForBimaterialRiemann) from the current level set."""
+        out = np.zeros(3)
+        self.L.orc_bimaterial_normal(self.h, d, i, j, k, myid, neighborid, dp(out))
+        return out
 
     def set_ls_test(self, test):
         """Prescribed-velocity level-set test 1 (slotted disk), 2 (vortex
