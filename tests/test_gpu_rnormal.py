@@ -55,6 +55,7 @@ def test_shock_bubble_with_levelset_normals_bitwise(rn):
     om = ol.Oracle(g, mats, nmat, s2)
     om.set_state(V, ID, Phi)
     assert not np.array_equal(_interior(om.residual()), _interior(o.residual()))
+    c0g, c0o = gpu.counters()["riemann_calls"], o.counters()["riemann_calls"]
     for step in range(4):
         gpu.run_steps(1)
         o.run_steps(1)
@@ -64,7 +65,7 @@ def test_shock_bubble_with_levelset_normals_bitwise(rn):
         assert np.array_equal(_interior(Vg), _interior(Vo)), step
         assert np.array_equal(_interior(IDg), _interior(IDo)), step
         assert np.array_equal(_interior(Pg), _interior(Po)), step
-    assert gpu.counters()["riemann_calls"] == o.counters()["riemann_calls"] > 0
+    assert gpu.counters()["riemann_calls"] - c0g == o.counters()["riemann_calls"] - c0o > 0
 
 
 def test_levelset_normals_need_one_level_set_and_two_materials():
