@@ -767,7 +767,9 @@ static bool reinit_due(const m2cx_solver* h, double time, double dt, int time_st
 }
 
 // TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
-// 720-800), one level set, no embedded surfaces
+// 720-800), one level set, no embedded surfaces.  FixUnresolvedNodes with the
+// failsafe density leaves no cell unresolved, so the level-set correction of
+// :771-793 never runs.
 static int end_of_step(m2cx_solver* h) {
   if (!(h->multimat && h->sch.n_levelsets > 0)) return 0;
   hipEvent_t e0{};
@@ -779,25 +781,14 @@ static int end_of_step(m2cx_solver* h) {
   }
   MPBuffers& B = h->mp;
   F5 V = f5(h, h->V[0]);
-  mp_end_ids(B, h->g, h->s, h->ID, h->cnt, true, h->stream);
+  mp_end_ids(B, h->g, h->s, h->ID, h->cnt, h->stream);
   DevCounters c;
   HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (reinit_due(h, c.t, c.dts, c.time_step))
     if (int rc = reinit_full(h, 0)) return rc;
   mp_end_states(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
-  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (c.n_unresolved > 0) {
-    // TimeIntegrator.cpp:771-793: Phi_tmp = Phi, the unresolved cells' level
-    // set, a forced reinitialization, then the IDs from the sign changes
-    const long n = h->ncells;
-    k_copy_f64<<<blocks_for(n), 256, 0, h->stream>>>(B.phin, B.phi, n);
-    k_ls_unresolved<<<blocks_for(c.n_unresolved), 256, 0, h->stream>>>(B.unresolved, h->cnt, B.list_cap, B.phi);
-    if (int rc = reinit_full(h, 0)) return rc;
-    mp_end_ids(B, h->g, h->s, h->ID, h->cnt, false, h->stream);
-  }
-  mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, 1, h->stream);
+  mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   return 0;
 }
 

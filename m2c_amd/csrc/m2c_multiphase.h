@@ -763,7 +763,6 @@ struct UpdArgs {
   int* unresolved;
   unsigned char* done;
   int cap;
-  int reinit;  // a reinitializer exists (the unresolved cells' level set was updated)
   DevCounters* cnt;
 };
 
@@ -882,12 +881,6 @@ __global__ void k_fix_unresolved(UpdArgs a) {
   const int nu = min(a.cnt->n_unresolved, a.cap);
   if (nu == 0) return;
   a.cnt->unresolved_total += (unsigned long long)nu;
-  if (!a.reinit) {
-    // the unresolved cells' level set must be reinitialized, and without a
-    // reinitializer the reference exits (LevelSetOperator.cpp:1341-1345)
-    atomicOr(&a.cnt->error, 8);
-    return;
-  }
   const DevGrid& g = a.g;
   int* U = a.unresolved;
   for (int x = 1; x < nu; x++) {  // insertion sort by linear index = (k,j,i) order
@@ -1129,16 +1122,11 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
   }
 }
 
-__global__ void k_remaining_reset(DevCounters* cnt) { cnt->n_remaining = 0; }
-
-// UpdateMaterialIDByLevelSet(phin, phi) with the ghost IDs; `first` resets
-// the end-of-step counters (a second call follows the level-set update of
-// unresolved cells, TimeIntegrator.cpp:791-792)
+// UpdateMaterialIDByLevelSet(phin, phi) with the ghost IDs
 inline void mp_end_ids(MPBuffers& B, const DevGrid& g, const Scheme& s, unsigned char* ID, DevCounters* cnt,
-                       bool first, hipStream_t st) {
+                       hipStream_t st) {
   const long nin = (long)g.NX * g.NY * g.NZ;
-  if (first) k_end_reset<<<1, 1, 0, st>>>(cnt);
-  else k_remaining_reset<<<1, 1, 0, st>>>(cnt);
+  k_end_reset<<<1, 1, 0, st>>>(cnt);
   k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, s, B.phin, B.phi, ID, B.tag, B.remaining, B.list_cap, B.lsid,
                                               cnt);
   k_id_vote<<<1, 1024, 0, st>>>(g, s, ID, B.tag, B.remaining, B.list_cap, B.lsid, cnt);
@@ -1147,7 +1135,7 @@ inline void mp_end_ids(MPBuffers& B, const DevGrid& g, const Scheme& s, unsigned
 }
 
 inline UpdArgs mp_upd_args(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID,
-                           DevCounters* cnt, int reinit) {
+                           DevCounters* cnt) {
   UpdArgs a{};
   a.g = g;
   a.s = s;
@@ -1160,7 +1148,6 @@ inline UpdArgs mp_upd_args(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V
   a.unresolved = B.unresolved;
   a.done = B.done;
   a.cap = B.list_cap;
-  a.reinit = reinit;
   a.cnt = cnt;
   return a;
 }
@@ -1170,13 +1157,13 @@ inline void mp_end_states(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V,
                           DevCounters* cnt, hipStream_t st) {
   const long nin = (long)g.NX * g.NY * g.NZ;
   k_changed<<<blocks_for(nin), 256, 0, st>>>(g, s, ID, B.IDn, B.changed, B.list_cap, cnt);
-  k_state_update<<<1, 1024, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt, 0));
+  k_state_update<<<1, 1024, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt));
 }
 
 // FixUnresolvedNodes, then clipping and the boundary fill of V
 inline void mp_end_fix(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
-                       int reinit, hipStream_t st) {
-  UpdArgs a = mp_upd_args(B, g, s, V, ID, cnt, reinit);
+                       hipStream_t st) {
+  UpdArgs a = mp_upd_args(B, g, s, V, ID, cnt);
   k_fix_unresolved<<<1, 64, 0, st>>>(a);
   k_post_update<<<256, 256, 0, st>>>(a);
 }
@@ -1184,9 +1171,9 @@ inline void mp_end_fix(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, un
 // the end-of-step update without a reinitializer
 inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
                         hipStream_t st) {
-  mp_end_ids(B, g, s, ID, cnt, true, st);
+  mp_end_ids(B, g, s, ID, cnt, st);
   mp_end_states(B, g, s, V, ID, cnt, st);
-  mp_end_fix(B, g, s, V, ID, cnt, 0, st);
+  mp_end_fix(B, g, s, V, ID, cnt, st);
 }
 
 }  // namespace m2c

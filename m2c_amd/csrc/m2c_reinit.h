@@ -529,18 +529,6 @@ __global__ void k_rcheck(ReinitState* st, double tol) {
   st->dphi_bits = 0;
 }
 
-// UpdateLevelSetsInUnresolvedCells (MultiPhaseOperator.cpp:1997-2051) for one
-// level set: an unresolved cell with phi >= 0 goes to the level set's side
-__global__ void k_ls_unresolved(const int* __restrict__ unresolved, const DevCounters* cnt, int cap,
-                                double* __restrict__ phi) {
-  const int n = min(cnt->n_unresolved, cap);
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const long c = unresolved[q];
-    if (phi[c] < 0) continue;
-    phi[c] = -phi[c];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------

@@ -401,17 +401,6 @@ void Solver::ReinitializeFullDomain(int special_maxIts) {
   }
 }
 
-// MultiPhaseOperator::UpdateLevelSetsInUnresolvedCells (MultiPhaseOperator.cpp:
-// 1997-2051) for one level set: an unresolved background cell (phi >= 0) goes
-// to the level set's material.
-void Solver::UpdateLevelSetsInUnresolvedCells(const std::vector<std::array<int, 3>>& unresolved) {
-  for (auto& u : unresolved) {
-    long c = m.idx(u[2], u[1], u[0]);
-    if (Phi[c] < 0) continue;
-    Phi[c] = -Phi[c];
-  }
-}
-
 // LevelSetOperator::PrescribeVelocityFieldForTesting (LevelSetOperator.cpp:
 // 1398-1490) with the normals of ComputeNormalDirectionCentralDifferencing_
 // FullDomain (:1509-1540) for test 3.

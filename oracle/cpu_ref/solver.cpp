@@ -954,16 +954,12 @@ int Solver::UpdateStateVariablesByRiemannSolutions() {
         if (!ok) ok = LocalUpdate(i, j, k, ID[c], false);
         if (!ok) unresolved.push_back({k, j, i});
       }
+  // FixUnresolvedNodes with apply_failsafe_density = On resolves every cell
+  // (a failsafe density where no neighbour helps) and reports none still
+  // unresolved (MultiPhaseOperator.cpp:1486-1525), so the level-set update
+  // and forced reinitialization of TimeIntegrator.cpp:771-797 do not run.
   if (!unresolved.empty()) {
-    // TimeIntegrator.cpp:771-797: the unresolved cells' level set is updated
-    // and reinitialized (a reinitializer is required: the reference exits
-    // without one), the IDs follow, then the nodes are fixed.
     cnt.unresolved += (long)unresolved.size();
-    Phi_tmp = Phi;
-    UpdateLevelSetsInUnresolvedCells(unresolved);
-    Reinitialize(t, dts, time_step, 0, true);
-    if (fatal) return -1;
-    UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
     FixUnresolvedNodes(unresolved);
   }
   return 0;

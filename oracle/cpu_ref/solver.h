@@ -118,7 +118,6 @@ class Solver {
   void ReinitializeFullDomain(int special_maxIts);
   double reinit_cfl = 0.8;  // the reinitializer's CFL, reduced by its failsafe
   // MultiPhaseOperator
-  void UpdateLevelSetsInUnresolvedCells(const std::vector<std::array<int, 3>>& unresolved);
   void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
   void UpdateMaterialIDAtGhostNodes();
   int UpdateStateVariablesByRiemannSolutions();

@@ -122,7 +122,7 @@ def stretched_sphere(n=(28, 24, 20), flt=abi.REINIT_FIXED, maxits=400, bc=abi.LS
 def unresolved_case(reinit):
     """A plane level set whose material IDs were never set: the moving
     interface changes cells to material 1 with no interface Riemann solution
-    next to them, so the state update leaves them unresolved."""
+    next to them, so the state update leaves them to FixUnresolvedNodes."""
     n = (24, 8, 8)
     g = abi.grid(n, (0, 0, 0), (1.0, 1 / 3, 1 / 3))
     mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.6))

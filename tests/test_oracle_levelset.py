@@ -13,8 +13,8 @@ Pins:
   its zero level set in place (parity of their exact arithmetic is otherwise
   unpinned);
 - the failsafe (CFL / 1.5 and restart until the last relative change is
-  below 0.02) and the unresolved cells' forced reinitialization
-  (TimeIntegrator.cpp:771-797).  The failsafe's error branch (CFL below 0.02
+  below 0.02), and that unresolved cells force no reinitialization with the
+  default failsafe density.  The failsafe's error branch (CFL below 0.02
   and still no convergence) is not exercised: every pseudo-time update
   changes a node by at most about CFL times its own value, so with a CFL
   below 0.02 the last change is below 0.02 already.
@@ -151,23 +151,17 @@ def test_failsafe_lowers_cfl_and_accepts_small_changes():
     assert o.counters()["reinit_iterations"] == 2 * 9
 
 
-def test_unresolved_cells_need_a_reinitializer():
-    g, mats, nmat, s, V, ID, phi = lc.unresolved_case(False)
-    o = ol.Oracle(g, mats, nmat, s)
-    o.set_state(V, ID, phi)
-    with pytest.raises(RuntimeError, match="reinitialization"):
+def test_unresolved_cells_are_fixed_without_reinitialization():
+    """FixUnresolvedNodes with the failsafe density (the reference's default)
+    resolves every cell and reports none still unresolved
+    (MultiPhaseOperator.cpp:1486-1525), so no forced reinitialization
+    follows, with or without a reinitializer."""
+    for reinit in (False, True):
+        g, mats, nmat, s, V, ID, phi = lc.unresolved_case(reinit)
+        o = ol.Oracle(g, mats, nmat, s)
+        o.set_state(V, ID, phi)
         o.run_steps(8)
-
-
-def test_unresolved_cells_force_a_reinitialization():
-    g, mats, nmat, s, V, ID, phi = lc.unresolved_case(True)
-    o = ol.Oracle(g, mats, nmat, s)
-    o.set_state(V, ID, phi)
-    o.run_steps(8)
-    c = o.counters()
-    assert c["unresolved_cells"] > 0 and c["reinit_calls"] > 0
-    _, IDo, P = o.get_state()
-    inner = (slice(1, -1),) * 3
-    # IDs follow the sign of phi where it changed; the unresolved cells got
-    # material 1 and a state from the failsafe
-    assert (IDo[inner] == 1).sum() > 0
+        c = o.counters()
+        assert c["unresolved_cells"] > 0 and c["reinit_calls"] == 0
+        _, IDo, _ = o.get_state()
+        assert (IDo[1:-1, 1:-1, 1:-1] == 1).sum() > 0
