@@ -60,8 +60,12 @@ enum { M2CX_TI_FORWARD_EULER = 0, M2CX_TI_RK2 = 1, M2CX_TI_RK3 = 2 };
 /* MultiPhaseData (IoData.h:893-903) */
 enum { M2CX_MPFLUX_EXACT = 0, M2CX_MPFLUX_NUMERICAL = 1, M2CX_MPFLUX_LLF = 2 };
 enum { M2CX_MPREC_CONSTANT = 0, M2CX_MPREC_LINEAR = 1 };
-enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0 };
+enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0, M2CX_PHASECHANGE_EXTRAPOLATION = 1 };
 enum { M2CX_RNORMAL_LEVEL_SET = 0, M2CX_RNORMAL_MESH = 1, M2CX_RNORMAL_AVERAGE = 2 };
+
+/* LevelSetSchemeData::Solver and ::Flux (IoData.h:806-810) */
+enum { M2CX_LSSOLVER_FINITE_VOLUME = 0, M2CX_LSSOLVER_FINITE_DIFFERENCE = 1 };
+enum { M2CX_LSFLUX_ROE = 0, M2CX_LSFLUX_LLF = 1, M2CX_LSFLUX_UPWIND = 2 };
 
 /* LevelSetReinitializationData::FirstLayerTreatment (IoData.h:754-756) */
 enum { M2CX_REINIT_FIXED = 0, M2CX_REINIT_CONSTRAINED1 = 1, M2CX_REINIT_CONSTRAINED2 = 2,
@@ -121,7 +125,8 @@ typedef struct {
   double max_time;             /* MaxTime; <= 0 means no clamp                 */
   int32_t mp_flux;             /* M2CX_MPFLUX_*                                */
   int32_t mp_recon;            /* M2CX_MPREC_*                                 */
-  int32_t mp_phase_change;     /* M2CX_PHASECHANGE_*                           */
+  int32_t mp_phase_change;     /* M2CX_PHASECHANGE_* (RiemannSolution; the
+                                  direction is Upwind, the default)             */
   int32_t riemann_normal;      /* RiemannNormal, M2CX_RNORMAL_* (Mesh)          */
   int32_t riem_maxits_main;    /* MaxIts (200)                                 */
   int32_t riem_maxits_bracket; /* MaxItsBracketing (100)                       */
@@ -148,6 +153,13 @@ typedef struct {
   int32_t reinit_first_layer;  /* FirstLayerTreatment, M2CX_REINIT_* (Fixed)   */
   int32_t reserved2;
   double time_step;            /* Time.TimeStep: fixed dt when > 0 (0)         */
+  /* LevelSet[0] Solver, Flux, Reconstruction (IoData.cpp:1513-1535, 1265-1274) */
+  int32_t ls_solver;           /* M2CX_LSSOLVER_* (FiniteDifference)           */
+  int32_t ls_flux;             /* finite volume: M2CX_LSFLUX_* (Roe)           */
+  double ls_delta;             /* Roe entropy-fix coefficient (0.2)            */
+  int32_t ls_rec_type;         /* M2CX_REC_* (Linear)                          */
+  int32_t ls_limiter;          /* M2CX_LIM_* (GeneralizedMinMod)               */
+  double ls_gmm_coeff;         /* GeneralizedMinModCoefficient (1.2)           */
 } m2cx_scheme;
 
 /* Fields held on the device by a solver handle. */

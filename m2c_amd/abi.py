@@ -19,6 +19,9 @@ MPFLUX_EXACT, MPFLUX_NUMERICAL, MPFLUX_LLF = 0, 1, 2
 MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
 RNORMAL_LEVEL_SET, RNORMAL_MESH, RNORMAL_AVERAGE = 0, 1, 2
+PHASECHANGE_RIEMANN_SOLUTION, PHASECHANGE_EXTRAPOLATION = 0, 1
+LSSOLVER_FINITE_VOLUME, LSSOLVER_FINITE_DIFFERENCE = 0, 1
+LSFLUX_ROE, LSFLUX_LLF, LSFLUX_UPWIND = 0, 1, 2
 REINIT_FIXED, REINIT_CONSTRAINED1, REINIT_CONSTRAINED2, REINIT_ITERATIVE_CONSTRAINED1, \
     REINIT_ITERATIVE_CONSTRAINED2 = range(5)
 FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
@@ -67,7 +70,9 @@ class Scheme(C.Structure):
                 ("riem_guess", C.c_int32), ("reserved", C.c_int32),
                 ("reinit_frequency", C.c_int32), ("reinit_maxits", C.c_int32),
                 ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
-                ("reinit_first_layer", C.c_int32), ("reserved2", C.c_int32), ("time_step", C.c_double)]
+                ("reinit_first_layer", C.c_int32), ("reserved2", C.c_int32), ("time_step", C.c_double),
+                ("ls_solver", C.c_int32), ("ls_flux", C.c_int32), ("ls_delta", C.c_double),
+                ("ls_rec_type", C.c_int32), ("ls_limiter", C.c_int32), ("ls_gmm_coeff", C.c_double)]
 
 
 class Counters(C.Structure):
@@ -125,7 +130,7 @@ def default_scheme(**kw):
     s.max_time = -1.0
     s.mp_flux = MPFLUX_NUMERICAL
     s.mp_recon = MPREC_CONSTANT
-    s.mp_phase_change = 0
+    s.mp_phase_change = PHASECHANGE_RIEMANN_SOLUTION
     s.riemann_normal = RNORMAL_MESH
     s.riem_maxits_main = 200
     s.riem_maxits_bracket = 100
@@ -149,6 +154,12 @@ def default_scheme(**kw):
     s.reinit_tol = 2.0e-4
     s.reinit_first_layer = REINIT_FIXED
     s.time_step = 0.0
+    s.ls_solver = LSSOLVER_FINITE_DIFFERENCE
+    s.ls_flux = LSFLUX_ROE
+    s.ls_delta = 0.2
+    s.ls_rec_type = REC_LINEAR
+    s.ls_limiter = LIM_GMM
+    s.ls_gmm_coeff = 1.2
     for k, v in kw.items():
         setattr(s, k, v)
     return s

@@ -453,3 +453,180 @@ void Solver::PrescribeVelocityForTesting(std::vector<double>& Vio, const std::ve
 }
 
 }  // namespace orc
+
+// ---------------------------------------------------------------------------
+// Level-set finite-volume scheme (LevelSetOperator::ComputeResidualFVM,
+// LevelSetOperator.cpp:966-1300) with its own reconstruction settings
+// (LevelSet[0].Reconstruction) and flux (LevelSet[0].Flux).
+// ---------------------------------------------------------------------------
+namespace orc {
+
+static inline double ls_sigma(const m2cx_scheme& s, double A, double B, int K, double dq0, double dq1) {
+  if (s.ls_rec_type != M2CX_REC_LINEAR) return 0.0;
+  if (s.ls_limiter == M2CX_LIM_GENERALIZED_MINMOD) return GeneralizedMinMod(A, B, s.ls_gmm_coeff, dq0, dq1);
+  if (s.ls_limiter == M2CX_LIM_VANALBADA) return VanAlbada(A, B, K, dq0, dq1);
+  return 0.5 * (dq0 + dq1);
+}
+
+// ComputeLocalAdvectionFlux (:1229-1262)
+static double ls_face_flux(const m2cx_scheme& s, double phim, double phip, double um, double up) {
+  double flux = 0.0;
+  if (s.ls_flux == M2CX_LSFLUX_UPWIND) {
+    double vmid = 0.5 * (um + up);
+    if (vmid > 0) flux = phim * um;
+    else if (vmid < 0) flux = phip * up;
+    else flux = 0.5 * (phim * um + phip * up);
+  } else if (s.ls_flux == M2CX_LSFLUX_LLF) {
+    flux = 0.5 * (phim * um + phip * up) - 0.5 * std::max(std::fabs(um), std::fabs(up)) * (phip - phim);
+  } else {  // Roe with Harten's entropy fix
+    double lam = std::fabs(0.5 * (um + up));
+    double tol = std::max(std::fabs(um), std::fabs(up)) * s.ls_delta;
+    if (lam < tol) lam = (lam * lam + tol * tol) / (2 * tol);
+    flux = 0.5 * (phim * um + phip * up) - 0.5 * lam * (phip - phim);
+  }
+  return flux;
+}
+
+void Solver::LSComputeResidualFVM(const std::vector<double>& Vin, const std::vector<double>& phi,
+                                  std::vector<double>& res) {
+  const long n = m.ncells();
+  // face values: [6][n] for phi (l r b t k f), [6][n] for the velocity
+  // component of each axis (minus, plus per axis), slopes [3][n]
+  std::vector<double> pf(6 * n, 0.0), uf(6 * n, 0.0), sl(3 * n, 0.0);
+  // Reconstructor::ReconstructIn1D (Reconstructor.cpp:922-1054) for u, v, w
+  // and Reconstruct (:335-790) for phi, interior cells
+  for (int k = 0; k < m.NZ; k++)
+    for (int j = 0; j < m.NY; j++)
+      for (int i = 0; i < m.NX; i++) {
+        const long c = m.idx(i, j, k);
+        const long st[3] = {1, m.PX, (long)m.PX * m.PY};
+        const double dd[3] = {m.DX(i), m.DY(j), m.DZ(k)};
+        for (int a = 0; a < 3; a++) {
+          const double A = A3[c * 3 + a], B = B3[c * 3 + a];
+          const int K = LSK3[c * 3 + a];
+          double u0 = Vin[(c - st[a]) * 5 + 1 + a], u1 = Vin[c * 5 + 1 + a], u2 = Vin[(c + st[a]) * 5 + 1 + a];
+          double sig = ls_sigma(sch, A, B, K, u1 - u0, u2 - u1);
+          uf[(2 * a) * n + c] = u1 - 0.5 * sig;
+          uf[(2 * a + 1) * n + c] = u1 + 0.5 * sig;
+          sl[a * n + c] = sig / dd[a];
+          double p0 = phi[c - st[a]], p1 = phi[c], p2 = phi[c + st[a]];
+          double sp = ls_sigma(sch, A, B, K, p1 - p0, p2 - p1);
+          pf[(2 * a) * n + c] = p1 - 0.5 * sp;
+          pf[(2 * a + 1) * n + c] = p1 + 0.5 * sp;
+        }
+      }
+  // ghost faces by the mesh boundary conditions (ReconstructIn1D :1048-1140,
+  // Reconstruct :807-880; one dof: symmetry flips the velocity only)
+  for (auto& gp : ghosts) {
+    const long c = m.idx(gp.i, gp.j, gp.k), im = m.idx(gp.ii, gp.jj, gp.kk);
+    const int a = gp.side / 2;
+    const bool lo = (gp.side % 2) == 0;
+    // the ghost's face towards the domain: plus face of a low ghost, minus
+    // face of a high one; the image's face on the boundary: its minus face
+    // for a low ghost, its plus face for a high one
+    const int gface = lo ? 2 * a + 1 : 2 * a, iface = lo ? 2 * a : 2 * a + 1;
+    switch (gp.bc) {
+      case M2CX_BC_INLET: case M2CX_BC_INLET2: case M2CX_BC_OVERSET:
+        uf[gface * n + c] = Vin[c * 5 + 1 + a];
+        pf[gface * n + c] = phi[c];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = 0.0;
+        break;
+      case M2CX_BC_OUTLET:
+        uf[gface * n + c] = uf[iface * n + im];
+        pf[gface * n + c] = pf[iface * n + im];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = sl[e * n + im];
+        break;
+      default:  // symmetry, slip and stick walls
+        uf[gface * n + c] = -uf[iface * n + im];
+        pf[gface * n + c] = pf[iface * n + im];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = -sl[e * n + im];
+        break;
+    }
+  }
+  // ComputeAdvectionFlux (:1079-1150): fluxes through the minus faces of
+  // interior cells and of the upper ghost layers, accumulated in loop order
+  std::fill(res.begin(), res.end(), 0.0);
+  for (int k = 0; k <= m.NZ; k++)
+    for (int j = 0; j <= m.NY; j++)
+      for (int i = 0; i <= m.NX; i++) {
+        const long c = m.idx(i, j, k);
+        if (k != m.NZ && j != m.NY) {
+          const long cm = m.idx(i - 1, j, k);
+          double f = ls_face_flux(sch, pf[1 * n + cm], pf[0 * n + c], uf[1 * n + cm], uf[0 * n + c]) / m.DX(i);
+          res[cm] -= f;
+          res[c] += f;
+        }
+        if (k != m.NZ && i != m.NX) {
+          const long cm = m.idx(i, j - 1, k);
+          double f = ls_face_flux(sch, pf[3 * n + cm], pf[2 * n + c], uf[3 * n + cm], uf[2 * n + c]) / m.DY(j);
+          res[cm] -= f;
+          res[c] += f;
+        }
+        if (j != m.NY && i != m.NX) {
+          const long cm = m.idx(i, j, k - 1);
+          double f = ls_face_flux(sch, pf[5 * n + cm], pf[4 * n + c], uf[5 * n + cm], uf[4 * n + c]) / m.DZ(k);
+          res[cm] -= f;
+          res[c] += f;
+        }
+      }
+  // AddSourceTerm (:1266-1285)
+  for (int k = 0; k < m.NZ; k++)
+    for (int j = 0; j < m.NY; j++)
+      for (int i = 0; i < m.NX; i++) {
+        const long c = m.idx(i, j, k);
+        res[c] += phi[c] * (sl[0 * n + c] + sl[1 * n + c] + sl[2 * n + c]);
+      }
+}
+
+// MultiPhaseOperator::UpdateStateVariablesByExtrapolation (MultiPhaseOperator.
+// cpp:1155-1272), upwind weights: a cell that changed material takes the
+// weighted average of its neighbours (3x3x3) of the same material that did
+// not change, each weighted by how much its flow points at the cell.
+int Solver::UpdateStateVariablesByExtrapolation() {
+  std::vector<std::array<int, 3>> unresolved;
+  for (int k = 0; k < m.NZ; k++)
+    for (int j = 0; j < m.NY; j++)
+      for (int i = 0; i < m.NX; i++) {
+        const long c = m.idx(i, j, k);
+        if (ID[c] == IDn[c]) continue;
+        if (ID[c] == inactive_id) continue;
+        cnt.id_changed++;
+        const double x0[3] = {m.X(i), m.Y(j), m.Z(k)};
+        double sum_weight = 0.0, vsum[5] = {0, 0, 0, 0, 0};
+        for (int nk = k - 1; nk <= k + 1; nk++)
+          for (int nj = j - 1; nj <= j + 1; nj++)
+            for (int ni = i - 1; ni <= i + 1; ni++) {
+              const long q = m.idx(ni, nj, nk);
+              if (ID[q] != ID[c]) continue;
+              if (ID[q] != IDn[q]) continue;
+              if (!m.inside(ni, nj, nk)) continue;
+              const double* vn = &V[q * 5];
+              double v1[3] = {vn[1], vn[2], vn[3]};
+              double v1n = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+              if (v1n != 0) {
+                double inv = 1.0 / v1n;
+                v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+              }
+              double d[3] = {x0[0] - m.X(ni), x0[1] - m.Y(nj), x0[2] - m.Z(nk)};
+              double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+              double inv = 1.0 / dn;
+              d[0] *= inv; d[1] *= inv; d[2] *= inv;
+              double weight = std::max(0.0, d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2]);
+              if (weight > 0) {
+                sum_weight += weight;
+                for (int p = 0; p < 5; p++) vsum[p] += weight * vn[p];
+              }
+            }
+        if (sum_weight == 0)
+          unresolved.push_back({k, j, i});
+        else
+          for (int p = 0; p < 5; p++) V[c * 5 + p] = vsum[p] / sum_weight;
+      }
+  if (!unresolved.empty()) {
+    cnt.unresolved += (long)unresolved.size();
+    FixUnresolvedNodes(unresolved);
+  }
+  return 0;
+}
+
+}  // namespace orc

@@ -19,13 +19,13 @@ static inline double ipow(double x, int k) {
 }
 
 // Reconstructor.h:96-103
-static inline double GeneralizedMinMod(double A, double B, double alpha, double dq0, double dq1) {
+double GeneralizedMinMod(double A, double B, double alpha, double dq0, double dq1) {
   if (dq0 < 0.0 && dq1 < 0.0) return std::max(alpha * dq0, std::max(B / (A + 1.0) * (dq0 + dq1), alpha * dq1));
   else if (dq0 > 0.0 && dq1 > 0.0) return std::min(alpha * dq0, std::min(B / (A + 1.0) * (dq0 + dq1), alpha * dq1));
   return 0.0;
 }
 // Reconstructor.h:104-111
-static inline double VanAlbada(double A, double B, int k, double dq0, double dq1) {
+double VanAlbada(double A, double B, int k, double dq0, double dq1) {
   if (dq0 * dq1 <= 0) return 0.0;
   if (std::fabs(dq0 * dq1) < 1e-16) return 0.0;
   double a = ipow(dq0, k), b = ipow(dq1, k);
@@ -33,7 +33,7 @@ static inline double VanAlbada(double A, double B, int k, double dq0, double dq1
 }
 
 // Reconstructor.cpp:103-121
-static int SlopeLimiterK(int limiter, double A, double B) {
+int SlopeLimiterK(int limiter, double A, double B) {
   if (limiter != M2CX_LIM_VANALBADA) return 0;
   int k;
   for (k = 2; k < 1000; k++) {
@@ -107,6 +107,7 @@ void Solver::SetupMesh() {
   A3.assign(m.ncells() * 3, 0.0);
   B3.assign(m.ncells() * 3, 0.0);
   K3.assign(m.ncells() * 3, 0);
+  LSK3.assign(m.ncells() * 3, 0);
   for (int k = -1; k <= m.NZ; k++)
     for (int j = -1; j <= m.NY; j++)
       for (int i = -1; i <= m.NX; i++) vol[m.idx(i, j, k)] = m.DX(i) * m.DY(j) * m.DZ(k);
@@ -124,6 +125,7 @@ void Solver::SetupMesh() {
         B[1] = 2.0 * m.DY(j) / (m.DY(j) + m.DY(j + 1));
         B[2] = 2.0 * m.DZ(k) / (m.DZ(k) + m.DZ(k + 1));
         for (int a = 0; a < 3; a++) K3[c * 3 + a] = SlopeLimiterK(sch.limiter, A[a], B[a]);
+        for (int a = 0; a < 3; a++) LSK3[c * 3 + a] = SlopeLimiterK(sch.ls_limiter, A[a], B[a]);
       }
 }
 
@@ -824,6 +826,10 @@ void Solver::LSApplyBoundaryConditions(std::vector<double>& phi) {
 
 void Solver::LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi,
                                std::vector<double>& res) {
+  if (sch.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME) {
+    LSComputeResidualFVM(Vin, phi, res);
+    return;
+  }
   auto P = [&](int i, int j, int k) { return phi[m.idx(i, j, k)]; };
   auto fd = [&](const std::vector<double>& C, long c, double v0, double v1, double v2, double v3) {
     const double* cc = &C[c * 4];
@@ -1164,7 +1170,10 @@ void Solver::UpdateSolutionAfterTimeStepping() {
   if (!ls_test) UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
   Reinitialize(t, dts, time_step, 0, false);
   if (fatal || ls_test) return;
-  UpdateStateVariablesByRiemannSolutions();
+  if (sch.mp_phase_change == M2CX_PHASECHANGE_EXTRAPOLATION)
+    UpdateStateVariablesByExtrapolation();
+  else
+    UpdateStateVariablesByRiemannSolutions();
   if (fatal) return;
   ClipDensityAndPressure(V);
   if (fatal) return;

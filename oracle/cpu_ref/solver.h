@@ -14,6 +14,12 @@ namespace orc {
 
 using V5 = std::array<double, 5>;
 
+// slope limiters and the van Albada exponent (Reconstructor.h:96-111,
+// Reconstructor.cpp:103-121)
+double GeneralizedMinMod(double A, double B, double alpha, double dq0, double dq1);
+double VanAlbada(double A, double B, int k, double dq0, double dq1);
+int SlopeLimiterK(int limiter, double A, double B);
+
 struct Mesh {
   int NX, NY, NZ, PX, PY, PZ;
   std::vector<double> x, y, z, dx, dy, dz;  // indexed i+1 (ghosts included)
@@ -78,6 +84,7 @@ class Solver {
   std::vector<double> Urec;  // conservative copy of V for conservative reconstruction
   std::vector<double> A3, B3;  // Reconstructor CoeffA/CoeffB (3 per cell)
   std::vector<int> K3;         // CoeffK
+  std::vector<int> LSK3;       // CoeffK of the level set's own reconstruction
   std::vector<int> ID, IDn;
   std::vector<double> Phi, Phi1, Phi_tmp, Rls;
   std::vector<double> cxL, cxR, cyL, cyR, czL, czR;  // FD3 coefficients, 4 per cell
@@ -107,6 +114,7 @@ class Solver {
   // LevelSetOperator
   void LSApplyBoundaryConditions(std::vector<double>& phi);
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
+  void LSComputeResidualFVM(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   // direction of the 1D Riemann problem at material interfaces
   void GradPhiAtFace(int d, int i, int j, int k, double* dir) const;
   void BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const;
@@ -121,6 +129,7 @@ class Solver {
   void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
   void UpdateMaterialIDAtGhostNodes();
   int UpdateStateVariablesByRiemannSolutions();
+  int UpdateStateVariablesByExtrapolation();
   // TimeIntegrator
   void AdvanceOneTimeStep(double time, double dt);
   void UpdateSolutionAfterTimeStepping();

@@ -96,10 +96,40 @@ def setup(deck):
     s.mp_flux = {"Exact": abi.MPFLUX_EXACT, "Numerical": abi.MPFLUX_NUMERICAL,
                  "LocalLaxFriedrichs": abi.MPFLUX_LLF}[mp.get("Flux", "Numerical")]
     s.mp_recon = abi.MPREC_LINEAR if mp.get("ReconstructionAtInterface", "Constant") == "Linear" else abi.MPREC_CONSTANT
+    s.mp_phase_change = {"RiemannSolution": abi.PHASECHANGE_RIEMANN_SOLUTION,
+                         "Extrapolation": abi.PHASECHANGE_EXTRAPOLATION}[mp.get("PhaseChange", "RiemannSolution")]
+    s.riemann_normal = {"LevelSet": abi.RNORMAL_LEVEL_SET, "Mesh": abi.RNORMAL_MESH,
+                        "Average": abi.RNORMAL_AVERAGE}[mp.get("RiemannNormal", "Mesh")]
     ls = [k for k in deck["Space"] if k.startswith("LevelSet")]
     s.n_levelsets = len(ls)
     if ls:
-        s.ls_materialid = int(deck["Space"][ls[0]]["MaterialID"])
+        lsd = deck["Space"][ls[0]]
+        s.ls_materialid = int(lsd["MaterialID"])
+        s.ls_solver = {"FiniteVolume": abi.LSSOLVER_FINITE_VOLUME,
+                       "FiniteDifference": abi.LSSOLVER_FINITE_DIFFERENCE}[lsd.get("Solver", "FiniteDifference")]
+        s.ls_flux = {"Roe": abi.LSFLUX_ROE, "LocalLaxFriedrichs": abi.LSFLUX_LLF,
+                     "Upwind": abi.LSFLUX_UPWIND}[lsd.get("Flux", "Roe")]
+        s.ls_delta = lsd.get("EntropyFixCoefficient", 0.2)
+        lrec = lsd.get("Reconstruction", {})
+        s.ls_rec_type = abi.REC_LINEAR if lrec.get("Type", "Linear") == "Linear" else abi.REC_CONSTANT
+        s.ls_limiter = LIM[lrec.get("Limiter", "GeneralizedMinMod")]
+        s.ls_gmm_coeff = lrec.get("GeneralizedMinModCoefficient", 1.2)
+        for q, key in enumerate(("BoundaryConditionX0", "BoundaryConditionXmax", "BoundaryConditionY0",
+                                 "BoundaryConditionYmax", "BoundaryConditionZ0", "BoundaryConditionZmax")):
+            s.ls_bc[q] = {"ZeroNeumann": abi.LSBC_ZERO_NEUMANN, "LinearExtrapolation": abi.LSBC_LINEAR_EXTRAPOLATION,
+                          "NonNegative": abi.LSBC_NON_NEGATIVE}[lsd.get(key, "ZeroNeumann")]
+        rei = lsd.get("Reinitialization", {})
+        s.reinit_frequency = int(rei.get("Frequency", -1))
+        s.reinit_frequency_dt = rei.get("TimeInterval", -1.0)
+        s.reinit_maxits = int(rei.get("MaxIts", 30))
+        s.reinit_cfl = rei.get("CFL", 0.8)
+        s.reinit_tol = rei.get("ConvergenceTolerance", 2.0e-4)
+        s.reinit_first_layer = {"Fixed": abi.REINIT_FIXED, "ConstrainedMethod1": abi.REINIT_CONSTRAINED1,
+                                "ConstrainedMethod2": abi.REINIT_CONSTRAINED2,
+                                "IterativelyConstrainedMethod1": abi.REINIT_ITERATIVE_CONSTRAINED1,
+                                "IterativelyConstrainedMethod2": abi.REINIT_ITERATIVE_CONSTRAINED2}[
+                                    rei.get("FirstLayerTreatment", "Fixed")]
+    s.time_step = ti.get("TimeStep", 0.0)
     # initial condition
     px, py, pz = n[0] + 2, n[1] + 2, n[2] + 2
     V = np.zeros((pz, py, px, 5))

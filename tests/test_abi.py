@@ -53,7 +53,8 @@ def test_argument_errors_need_no_gpu():
 
 
 _PREFIXES = ("EOS_", "BC_", "LSBC_", "FLUX_", "REC_", "LIM_", "SLOPE_", "VAR_", "TI_", "MPFLUX_", "MPREC_", "GUESS_",
-             "FIELD_", "E_", "GEOM_", "RIEMANN_", "REINIT_", "RNORMAL_")
+             "FIELD_", "E_", "GEOM_", "RIEMANN_", "REINIT_", "RNORMAL_", "PHASECHANGE_", "LSSOLVER_",
+             "LSFLUX_")
 
 
 def _consts():
