@@ -190,6 +190,11 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (sch->reinit_first_layer < M2CX_REINIT_FIXED || sch->reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2)
     return M2CX_E_ARG;
   if (sch->riemann_normal < M2CX_RNORMAL_LEVEL_SET || sch->riemann_normal > M2CX_RNORMAL_AVERAGE) return M2CX_E_ARG;
+  if (sch->mp_phase_change < M2CX_PHASECHANGE_RIEMANN_SOLUTION || sch->mp_phase_change > M2CX_PHASECHANGE_EXTRAPOLATION)
+    return M2CX_E_ARG;
+  if (sch->ls_solver < M2CX_LSSOLVER_FINITE_VOLUME || sch->ls_solver > M2CX_LSSOLVER_FINITE_DIFFERENCE ||
+      sch->ls_flux < M2CX_LSFLUX_ROE || sch->ls_flux > M2CX_LSFLUX_UPWIND)
+    return M2CX_E_ARG;
   // the level-set normal is defined for faces between material 0 and the
   // level set's material (CalculateGradPhiAtCellInterface asserts it)
   if (sch->riemann_normal != M2CX_RNORMAL_MESH && (sch->n_levelsets != 1 || nmat != 2 || sch->ls_materialid != 1))
@@ -230,7 +235,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   for (int q = 0; q < 6; q++) geo.insert(geo.end(), ax[q].begin(), ax[q].end());
   int Ns[3] = {NX, NY, NZ};
   std::vector<double> Acoef, Bcoef;
-  std::vector<int> Kcoef;
+  std::vector<int> Kcoef, LSKcoef;
   for (int a = 0; a < 3; a++) {
     const std::vector<double>& d = ax[3 + a];
     for (int i = 0; i < Ns[a]; i++) {
@@ -239,6 +244,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
       Acoef.push_back(A);
       Bcoef.push_back(B);
       Kcoef.push_back(slope_k(sch->limiter, A, B));
+      LSKcoef.push_back(slope_k(sch->ls_limiter, A, B));
     }
   }
   geo.insert(geo.end(), Acoef.begin(), Acoef.end());
@@ -258,6 +264,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   s.gmm = sch->gmm_coeff; s.del = sch->entropy_fix; s.cfl = sch->cfl;
   s.time_step = sch->time_step;
   s.rnormal = sch->riemann_normal;
+  s.phase_change = sch->mp_phase_change;
   h->reinit_cfl = sch->reinit_cfl;
   s.var_type = sch->var_type;
   DevMats& M = h->M;
@@ -303,6 +310,17 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (h->multimat) {
     int rc = mp_create(h->mp, g, *sch, ax, h->stream);
     if (rc) return fail(h, rc, "multi-material buffers: " + std::string(hipGetErrorString(hipGetLastError())));
+    LSFvm& fv = h->mp.fv;
+    fv.on = sch->ls_solver == M2CX_LSSOLVER_FINITE_VOLUME;
+    fv.flux = sch->ls_flux;
+    fv.rec = sch->ls_rec_type;
+    fv.limiter = sch->ls_limiter;
+    fv.gmm = sch->ls_gmm_coeff;
+    fv.delta = sch->ls_delta;
+    int* lsk = nullptr;
+    HIPCHK(h, hipMalloc(&lsk, LSKcoef.size() * 4));
+    HIPCHK(h, hipMemcpy(lsk, LSKcoef.data(), LSKcoef.size() * 4, hipMemcpyHostToDevice));
+    fv.lsk = lsk;
     
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));

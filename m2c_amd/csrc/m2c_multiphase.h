@@ -23,6 +23,13 @@ namespace m2c {
 // SIMD).  Each lane owns 2 x 512 integration-path points (24 KiB).
 constexpr int RIEM_LANES = 1024 * 64;
 
+// LevelSet[0] finite-volume settings (LevelSetSchemeData, IoData.h:802-830)
+struct LSFvm {
+  int on, flux, rec, limiter;
+  double gmm, delta;
+  const int* lsk;  // van Albada exponents of the level set's limiter, laid out as DevGrid::kt
+};
+
 struct MPBuffers {
   int* fidx[3] = {};
   IfaceList L{};
@@ -30,6 +37,7 @@ struct MPBuffers {
   double* fd3 = nullptr;  // FD3 coefficient tables (see mp_create)
   long off_cxL = 0, off_cxR = 0, off_cyL = 0, off_cyR3 = 0, off_cyR1 = 0, off_czL = 0, off_czR = 0;
   int lsid = 1, nls = 0;
+  LSFvm fv{};  // finite-volume level-set settings (fv.lsk owned here)
   int lsbc[6] = {1, 1, 1, 1, 1, 1};
   RiemannCfg rc{};
   double* slab = nullptr;
@@ -154,6 +162,7 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
 
 inline void mp_destroy(MPBuffers& B) {
   for (int d = 0; d < 3; d++) (void)hipFree(B.fidx[d]);
+  if (B.fv.lsk) (void)hipFree((void*)B.fv.lsk);
   for (void* p : {(void*)B.L.cell, (void*)B.L.dir, (void*)B.L.f1, (void*)B.L.f2, (void*)B.L.vsm, (void*)B.L.vsp,
                   (void*)B.L.err, (void*)B.L.stats, (void*)B.L.hist, (void*)B.phi, (void*)B.phi1, (void*)B.phin, (void*)B.rls,
                   (void*)B.phitmp, (void*)B.fd3, (void*)B.slab, (void*)B.remaining, (void*)B.changed,
@@ -515,6 +524,7 @@ __global__ void __launch_bounds__(128) k_riemann_batch(int mset, RiemannCfg rc, 
 // ---------------------------------------------------------------------------
 struct LSArgs {
   DevGrid g;
+  LSFvm fv;
   CF5 V;
   const double* phi_st;   // stencil field of this stage
   const double* phi_a;    // phi(n)
@@ -526,6 +536,9 @@ struct LSArgs {
   int which;
   const DevCounters* cnt;
 };
+
+__device__ __forceinline__ double ls_fvm_rhs(const LSArgs& a, const double* __restrict__ P, long c, int i, int j,
+                                             int k);
 
 __global__ void k_levelset(LSArgs a) {
   const DevGrid& g = a.g;
@@ -545,6 +558,10 @@ __global__ void k_levelset(LSArgs a) {
     const double* gy = g_axis(g, 1);
     const double* gz = g_axis(g, 2);
     const double xi = gx[i + 1], yj = gy[j + 1], zk = gz[k + 1];
+    double r;
+    if (a.fv.on) {
+      r = ls_fvm_rhs(a, P, c, i, j, k);
+    } else {
     double av = (i - 2 >= -1) ? fd(a.ocxL, i, sx, c - 2 * sx) : (P[c] - P[c - sx]) / (xi - gx[i]);
     double bv = (i + 2 <= g.NX) ? fd(a.ocxR, i, sx, c - sx) : (P[c + sx] - P[c]) / (gx[i + 2] - xi);
     double cv = (j - 2 >= -1) ? fd(a.ocyL, j, sy, c - 2 * sy) : (P[c] - P[c - sy]) / (yj - gy[j]);
@@ -553,8 +570,9 @@ __global__ void k_levelset(LSArgs a) {
     double ev = (k - 2 >= -1) ? fd(a.oczL, k, sz, c - 2 * sz) : (P[c] - P[c - sz]) / (zk - gz[k]);
     double fv = (k + 2 <= g.NZ) ? fd(a.oczR, k, sz, c - sz) : (P[c + sz] - P[c]) / (gz[k + 2] - zk);
     const double u = a.V(1, c), v = a.V(2, c), w = a.V(3, c);
-    double r = (u >= 0 ? av * u : bv * u) + (v >= 0 ? cv * v : dv * v) + (w >= 0 ? ev * w : fv * w);
+    r = (u >= 0 ? av * u : bv * u) + (v >= 0 ? cv * v : dv * v) + (w >= 0 ? ev * w : fv * w);
     r = r * -1.0;
+    }
     double o;
     switch (a.which) {
       case 0: o = 1.0 * a.phi_a[c] + dt * r; break;                                    // FE
@@ -567,6 +585,124 @@ __global__ void k_levelset(LSArgs a) {
     if (a.which < 0) a.rls[c] = r;
     else a.out[c] = o;
   }
+}
+
+// ---------------------------------------------------------------------------
+// level-set finite-volume residual (LevelSetOperator::ComputeResidualFVM,
+// LevelSetOperator.cpp:966-1300)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ls_sigma(const LSArgs& a, int D, int idx, double dq0, double dq1) {
+  if (a.fv.rec != M2CX_REC_LINEAR) return 0.0;
+  const double A = g_coefA(a.g, D)[idx], B = g_coefB(a.g, D)[idx];
+  if (a.fv.limiter == M2CX_LIM_GENERALIZED_MINMOD) return lim_gmm(A, B, a.fv.gmm, dq0, dq1);
+  if (a.fv.limiter == M2CX_LIM_VANALBADA) {
+    long off = 0;
+    if (D >= 1) off += a.g.NX;
+    if (D >= 2) off += a.g.NY;
+    return lim_va(A, B, a.fv.lsk[off + idx], dq0, dq1);
+  }
+  return 0.5 * (dq0 + dq1);
+}
+
+// minus and plus face values of an interior cell along axis D
+// (ReconstructIn1D for the velocity component, Reconstruct for phi)
+struct LSFaces { double um, up, pm, pp, slope; };
+__device__ __forceinline__ LSFaces ls_faces(const LSArgs& a, const double* __restrict__ P, int D, long q, int idx) {
+  const long st = axis_stride(a.g, D);
+  LSFaces f;
+  const double u0 = a.V(1 + D, q - st), u1 = a.V(1 + D, q), u2 = a.V(1 + D, q + st);
+  const double su = ls_sigma(a, D, idx, u1 - u0, u2 - u1);
+  f.um = u1 - 0.5 * su;
+  f.up = u1 + 0.5 * su;
+  f.slope = su / g_axis(a.g, 3 + D)[idx + 1];
+  const double p0 = P[q - st], p1 = P[q], p2 = P[q + st];
+  const double sp = ls_sigma(a, D, idx, p1 - p0, p2 - p1);
+  f.pm = p1 - 0.5 * sp;
+  f.pp = p1 + 0.5 * sp;
+  return f;
+}
+
+// the face of a ghost cell towards the domain, by the mesh boundary
+// condition (Reconstructor.cpp:807-880, 1048-1140): Dirichlet sides take the
+// ghost value, outlets the image's face on the boundary, walls and symmetry
+// planes that face with the velocity negated; `img` is the image's face
+__device__ __forceinline__ void ls_ghost_face(const LSArgs& a, const double* __restrict__ P, int side, long qg,
+                                              double img_u, double img_p, double& u, double& p) {
+  const int D = side / 2;
+  switch (a.g.bc[side]) {
+    case M2CX_BC_INLET: case M2CX_BC_INLET2: case M2CX_BC_OVERSET:
+      u = a.V(1 + D, qg);
+      p = P[qg];
+      break;
+    case M2CX_BC_OUTLET:
+      u = img_u;
+      p = img_p;
+      break;
+    default:
+      u = -img_u;
+      p = img_p;
+      break;
+  }
+}
+
+// ComputeLocalAdvectionFlux (LevelSetOperator.cpp:1229-1262)
+__device__ __forceinline__ double ls_face_flux(const LSArgs& a, double phim, double phip, double um, double up) {
+  if (a.fv.flux == M2CX_LSFLUX_UPWIND) {
+    const double vmid = 0.5 * (um + up);
+    if (vmid > 0) return phim * um;
+    if (vmid < 0) return phip * up;
+    return 0.5 * (phim * um + phip * up);
+  }
+  const double fu = fabs(um), fp = fabs(up);
+  const double mx = (fu < fp) ? fp : fu;  // std::max
+  if (a.fv.flux == M2CX_LSFLUX_LLF) return 0.5 * (phim * um + phip * up) - 0.5 * mx * (phip - phim);
+  double lam = fabs(0.5 * (um + up));
+  const double tol = mx * a.fv.delta;
+  if (lam < tol) lam = (lam * lam + tol * tol) / (2 * tol);
+  return 0.5 * (phim * um + phip * up) - 0.5 * lam * (phip - phim);
+}
+
+// Right-hand side at one interior cell: its six face fluxes added in the
+// order the reference's face loop accumulates them (own minus faces x, y, z,
+// then the plus faces as the loop reaches the neighbours), then the source
+// phi (u_x + v_y + w_z) of AddSourceTerm (:1266-1285).
+__device__ __forceinline__ double ls_fvm_rhs(const LSArgs& a, const double* __restrict__ P, long c, int i, int j,
+                                             int k) {
+  const int ijk[3] = {i, j, k};
+  const int Ns[3] = {a.g.NX, a.g.NY, a.g.NZ};
+  double fm[3], fp[3], sl[3];
+  for (int D = 0; D < 3; D++) {
+    const long st = axis_stride(a.g, D);
+    const double* dxs = g_axis(a.g, 3 + D);
+    const int q = ijk[D];
+    const LSFaces me = ls_faces(a, P, D, c, q);
+    sl[D] = me.slope;
+    double um, pm;  // plus face of the minus neighbour
+    if (q == 0) ls_ghost_face(a, P, 2 * D, c - st, me.um, me.pm, um, pm);
+    else {
+      const LSFaces nb = ls_faces(a, P, D, c - st, q - 1);
+      um = nb.up;
+      pm = nb.pp;
+    }
+    fm[D] = ls_face_flux(a, pm, me.pm, um, me.um) / dxs[q + 1];
+    double up, pp;  // minus face of the plus neighbour
+    if (q == Ns[D] - 1) ls_ghost_face(a, P, 2 * D + 1, c + st, me.up, me.pp, up, pp);
+    else {
+      const LSFaces nb = ls_faces(a, P, D, c + st, q + 1);
+      up = nb.um;
+      pp = nb.pm;
+    }
+    fp[D] = ls_face_flux(a, me.pp, pp, me.up, up) / dxs[q + 2];
+  }
+  double r = 0.0;
+  r += fm[0];
+  r += fm[1];
+  r += fm[2];
+  r -= fp[0];
+  r -= fp[1];
+  r -= fp[2];
+  r += P[c] * (sl[0] + sl[1] + sl[2]);
+  return r;
 }
 
 // LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:408-508);
@@ -873,6 +1009,60 @@ __global__ void __launch_bounds__(1024) k_state_update(UpdArgs a) {
   }
 }
 
+// UpdateStateVariablesByExtrapolation (:1155-1272), upwind weights: a cell
+// that changed material takes the average of its 3x3x3 neighbours of the same
+// material that kept it, each weighted by how much its flow points at the
+// cell (max(0, unit offset . unit velocity)); the neighbours never change in
+// this pass, so the cells are independent.  Cells with no weight go to the
+// unresolved list for FixUnresolvedNodes.
+__global__ void k_extrapolate(UpdArgs a) {
+  const DevGrid& g = a.g;
+  const int nch = min(a.cnt->n_changed, a.cap);
+  const double* X = g_axis(g, 0);
+  const double* Y = g_axis(g, 1);
+  const double* Z = g_axis(g, 2);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nch; q += gridDim.x * blockDim.x) {
+    const long c = a.changed[q];
+    long r = c;
+    const int i = (int)(r % g.SX) - 1; r /= g.SX;
+    const int j = (int)(r % g.SY) - 1;
+    const int k = (int)(r / g.SY) - 1;
+    const int myid = a.ID[c];
+    const double x0[3] = {X[i + 1], Y[j + 1], Z[k + 1]};
+    double sw = 0.0, vs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int nk = k - 1; nk <= k + 1; nk++)
+      for (int nj = j - 1; nj <= j + 1; nj++)
+        for (int ni = i - 1; ni <= i + 1; ni++) {
+          const long nb = gidx(g, ni, nj, nk);
+          if (a.ID[nb] != myid) continue;
+          if (a.ID[nb] != a.IDn[nb]) continue;
+          if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
+          double v1[3] = {a.V(1, nb), a.V(2, nb), a.V(3, nb)};
+          const double v1n = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+          if (v1n != 0) {
+            const double inv = 1.0 / v1n;  // Vec3D::operator/= multiplies by the reciprocal
+            v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+          }
+          double d[3] = {x0[0] - X[ni + 1], x0[1] - Y[nj + 1], x0[2] - Z[nk + 1]};
+          const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          d[0] *= inv; d[1] *= inv; d[2] *= inv;
+          const double dot = d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2];
+          const double w = (0.0 < dot) ? dot : 0.0;  // std::max(0.0, dot)
+          if (w > 0) {
+            sw += w;
+            for (int p = 0; p < 5; p++) vs[p] += w * a.V(p, nb);
+          }
+        }
+    if (sw == 0) {
+      const int slot = atomicAdd(&a.cnt->n_unresolved, 1);
+      if (slot < a.cap) a.unresolved[slot] = (int)c;
+      else atomicOr(&a.cnt->error, 4);
+    } else {
+      for (int p = 0; p < 5; p++) a.V(p, c) = vs[p] / sw;
+    }
+  }
+}
+
 // FixUnresolvedNodes (:1278-1525) with apply_failsafe_density = On and
 // phase-change direction UPWIND; one lane, nodes in (k,j,i) order, with the
 // reference's `reset` flag shared by all nodes.
@@ -1092,6 +1282,7 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
   if (!B.nls) return;
   LSArgs a{};
   a.g = g;
+  a.fv = B.fv;
   a.V = V;
   a.fd3 = B.fd3;
   a.ocxL = B.off_cxL; a.ocxR = B.off_cxR; a.ocyL = B.off_cyL; a.ocyR3 = B.off_cyR3; a.ocyR1 = B.off_cyR1;
@@ -1157,7 +1348,10 @@ inline void mp_end_states(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V,
                           DevCounters* cnt, hipStream_t st) {
   const long nin = (long)g.NX * g.NY * g.NZ;
   k_changed<<<blocks_for(nin), 256, 0, st>>>(g, s, ID, B.IDn, B.changed, B.list_cap, cnt);
-  k_state_update<<<1, 1024, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt));
+  if (s.phase_change == M2CX_PHASECHANGE_EXTRAPOLATION)
+    k_extrapolate<<<256, 256, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt));
+  else
+    k_state_update<<<1, 1024, 0, st>>>(mp_upd_args(B, g, s, V, ID, cnt));
 }
 
 // FixUnresolvedNodes, then clipping and the boundary fill of V

@@ -72,7 +72,9 @@ def test_riemann_batch_bitwise(kind):
 # Primitive (Test1, Test6), ConservativeCharacteristic (Test2 Roe SG/SG,
 # Test8 HLLC MG/MG), Conservative (Test5 HLLC, water), PrimitiveCharacteristic
 # (Test11 LLF with exact interface fluxes, Test12 LLF SG/MG next to vacuum)
-@pytest.mark.parametrize("name", ["Test1", "Test6", "Test2", "Test5", "Test8", "Test11", "Test12"])
+# Test4: level-set finite volumes (Roe, linear, no limiter) with the
+# extrapolation phase-change update
+@pytest.mark.parametrize("name", ["Test1", "Test6", "Test2", "Test4", "Test5", "Test8", "Test11", "Test12"])
 def test_line_deck_on_gpu(fixtures, name):
     d = fixtures[name]["deck"]
     g, mats, nmat, s, V, ID, Phi = dk.setup(d)
@@ -188,3 +190,55 @@ def test_riemann_histogram_matches_counters():
     it, fe = gpu.riemann_histogram()
     assert it.sum() == 0 and fe.sum() == 0
     gpu.close()
+
+
+def _moving_plane(n=(24, 12, 10)):
+    """A tilted planar interface between two gases carried across the grid
+    (material 1 where phi < 0), so that cells change material every step."""
+    g = abi.grid(n, (0, 0, 0), (1.0, 0.5, 0.4))
+    mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.6))
+    s = abi.default_scheme(n_levelsets=1, integrator=abi.TI_RK2, flux=abi.FLUX_HLLC)
+    x = np.array(configs._axis(n[0], 0, 1.0))[None, None, :]
+    y = np.array(configs._axis(n[1], 0, 0.5))[None, :, None]
+    z = np.array(configs._axis(n[2], 0, 0.4))[:, None, None]
+    nrm = np.array([1.0, 0.3, 0.2]) / np.linalg.norm([1.0, 0.3, 0.2])
+    phi = nrm[0] * (x - 0.4) + nrm[1] * (y - 0.25) + nrm[2] * (z - 0.2) + 0 * x * y * z
+    V = np.zeros(phi.shape + (5,))
+    V[..., 0] = np.where(phi < 0, 2.0, 1.0)
+    V[..., 1] = 1.0 + 0.1 * np.sin(7 * y)
+    V[..., 2] = 0.2
+    V[..., 3] = 0.1
+    V[..., 4] = 1.0
+    ID = (phi < 0).astype(np.int32)
+    return g, mats, 2, s, V, ID, phi
+
+
+# LevelSet[0] Solver = FiniteVolume with each flux and limiter, and
+# PhaseChange = Extrapolation, on a moving 3D bubble
+@pytest.mark.parametrize("lsflux,lslim,pc", [
+    (abi.LSFLUX_ROE, abi.LIM_NONE, abi.PHASECHANGE_EXTRAPOLATION),
+    (abi.LSFLUX_LLF, abi.LIM_GMM, abi.PHASECHANGE_RIEMANN_SOLUTION),
+    (abi.LSFLUX_UPWIND, abi.LIM_VANALBADA, abi.PHASECHANGE_EXTRAPOLATION)])
+def test_levelset_finite_volume_and_extrapolation_bitwise(lsflux, lslim, pc):
+    print("ls fvm", lsflux, lslim, "phase change", pc, flush=True)
+    g, mats, nmat, s, V, ID, Phi = _moving_plane()
+    s.ls_solver = abi.LSSOLVER_FINITE_VOLUME
+    s.ls_flux = lsflux
+    s.ls_limiter = lslim
+    s.mp_phase_change = pc
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(V, ID, Phi)
+    o = ol.Oracle(g, mats, nmat, s)
+    o.set_state(V, ID, Phi)
+    assert np.array_equal(gpu.levelset_residual()[1:-1, 1:-1, 1:-1], o.ls_residual()[1:-1, 1:-1, 1:-1])
+    for step in range(6):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        Vg, IDg, Pg = gpu.get_state()
+        Vo, IDo, Po = o.get_state()
+        assert np.array_equal(Pg[1:-1, 1:-1, 1:-1], Po[1:-1, 1:-1, 1:-1]), step
+        assert np.array_equal(IDg[1:-1, 1:-1, 1:-1], IDo[1:-1, 1:-1, 1:-1]), step
+        assert np.array_equal(Vg[1:-1, 1:-1, 1:-1], Vo[1:-1, 1:-1, 1:-1]), step
+    cg, co = gpu.counters(), o.counters()
+    assert cg["id_changed_cells"] == co["id_changed"] > 0
+    assert cg["unresolved_cells"] == co["unresolved_cells"]
