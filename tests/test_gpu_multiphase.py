@@ -226,10 +226,15 @@ def test_levelset_finite_volume_and_extrapolation_bitwise(lsflux, lslim, pc):
     s.ls_flux = lsflux
     s.ls_limiter = lslim
     s.mp_phase_change = pc
-    gpu = _rt().Solver(g, mats, nmat, s)
-    gpu.set_state(V, ID, Phi)
+    # the ghosts as the reference's initialization leaves them: boundary
+    # conditions of V, the IDs and the level set applied
     o = ol.Oracle(g, mats, nmat, s)
     o.set_state(V, ID, Phi)
+    o.apply_bc()
+    o.update_ghost_ids()
+    o.ls_apply_bc()
+    gpu = _rt().Solver(g, mats, nmat, s)
+    gpu.set_state(*o.get_state())
     assert np.array_equal(gpu.levelset_residual()[1:-1, 1:-1, 1:-1], o.ls_residual()[1:-1, 1:-1, 1:-1])
     for step in range(6):
         gpu.run_steps(1)
