@@ -26,6 +26,12 @@ struct Timer {
   long long launches = 0;
 };
 
+// Stage-kernel tile shapes the autotuner chooses from (all 256 threads).
+struct TileVariant {
+  int bx, by, bz;
+};
+static const TileVariant kTiles[] = {{16, 4, 4}, {32, 4, 2}, {8, 8, 4}, {16, 8, 2}};
+
 struct m2cx_solver {
   m2cx_grid grid;
   m2cx_scheme sch;
@@ -56,6 +62,13 @@ struct m2cx_solver {
   // multi-material
   MPBuffers mp{};
   bool multimat = false;
+  // the stage kernel's non-interface tiles run on stream2 while the
+  // interface Riemann problems are solved on stream (fork / join events)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  unsigned char* tflag = nullptr;
+  long tflag_n = 0;
+  bool overlap = false;
   // level-set reinitialization
   RBuf rb{};
   double reinit_cfl = 0.8;  // the reinitializer's CFL, lowered by its failsafe
@@ -131,17 +144,17 @@ static void mat_slot_release(m2cx_solver* h) {
 static F5 f5(const m2cx_solver* h, double* const* a) { return F5{a[0], h->ncells}; }
 static CF5 cf5(const m2cx_solver* h, double* const* a) { return CF5{a[0], h->ncells}; }
 
-static void tic(m2cx_solver* h, const char* name, hipEvent_t* a) {
+static void tic(m2cx_solver* h, const char* name, hipEvent_t* a, hipStream_t st = nullptr) {
   if (!h->profile) return;
   hipEventCreate(a);
-  hipEventRecord(*a, h->stream);
+  hipEventRecord(*a, st ? st : h->stream);
   (void)name;
 }
-static void toc(m2cx_solver* h, const char* name, hipEvent_t a) {
+static void toc(m2cx_solver* h, const char* name, hipEvent_t a, hipStream_t st = nullptr) {
   if (!h->profile) return;
   hipEvent_t b;
   hipEventCreate(&b);
-  hipEventRecord(b, h->stream);
+  hipEventRecord(b, st ? st : h->stream);
   h->timers[name].ev.push_back({a, b});
 }
 
@@ -321,6 +334,19 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
     HIPCHK(h, hipMalloc(&lsk, LSKcoef.size() * 4));
     HIPCHK(h, hipMemcpy(lsk, LSKcoef.data(), LSKcoef.size() * 4, hipMemcpyHostToDevice));
     fv.lsk = lsk;
+    // second stream and tile flags for overlapping the stage kernel with the
+    // interface Riemann solve (default priority, like the solver's stream)
+    if (!getenv("M2C_NO_OVERLAP")) {
+      HIPCHK(h, hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+      HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+      HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+      long mt = 0;
+      for (const TileVariant& t : kTiles)
+        mt = std::max(mt, (long)((NX + t.bx - 1) / t.bx) * ((NY + t.by - 1) / t.by) * ((NZ + t.bz - 1) / t.bz));
+      HIPCHK(h, hipMalloc(&h->tflag, mt));
+      h->tflag_n = mt;
+      h->overlap = true;
+    }
     
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -341,6 +367,11 @@ int m2cx_destroy(m2cx_solver* h) {
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
   ri_free(h->rb);
+  if (h->tflag) hipFree(h->tflag);
+  if (h->stream2) hipStreamSynchronize(h->stream2);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   mat_slot_release(h);
   for (void* p : {(void*)h->mlp.theta, (void*)h->mlp.m, (void*)h->mlp.v, (void*)h->mlp.wt1, (void*)h->mlp.wt2,
                   (void*)h->mlp.wt3, (void*)h->mlp.wt4, (void*)h->mlp.w2, (void*)h->mlp.w3, (void*)h->mlp.guess})
@@ -455,15 +486,10 @@ static StageArgs stage_args(m2cx_solver* h, int vin, int vout, int mode) {
 }
 
 extern "C++" {
-// Stage-kernel tile shapes the autotuner chooses from (all 256 threads).
-struct TileVariant {
-  int bx, by, bz;
-};
-static const TileVariant kTiles[] = {{16, 4, 4}, {32, 4, 2}, {8, 8, 4}, {16, 8, 2}};
 static constexpr int kNumTiles = 4;
 
 template <int BX, int BY, int BZ, bool GEN>
-static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode) {
+static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode, hipStream_t st) {
   // one work-group per tile, padded to a multiple of the XCD count (see
   // k_stage for the tile order)
   const long ntiles = (long)((h->g.NX + BX - 1) / BX) * ((h->g.NY + BY - 1) / BY) * ((h->g.NZ + BZ - 1) / BZ);
@@ -471,37 +497,55 @@ static void launch_stage_tile(m2cx_solver* h, const StageArgs& a, int mode) {
   const dim3 grid((unsigned)(per * NXCD));
   constexpr int nt = BX * BY * BZ;
   switch (mode) {
-    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_STAGE1: k_stage<MODE_STAGE1, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
-    case MODE_FE: k_stage<MODE_FE, BX, BY, BZ, GEN><<<grid, nt, 0, h->stream>>>(a); break;
+    case MODE_RESIDUAL: k_stage<MODE_RESIDUAL, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
+    case MODE_STAGE1: k_stage<MODE_STAGE1, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
+    case MODE_RK2_FINAL: k_stage<MODE_RK2_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
+    case MODE_RK3_STAGE2: k_stage<MODE_RK3_STAGE2, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
+    case MODE_RK3_FINAL: k_stage<MODE_RK3_FINAL, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
+    case MODE_FE: k_stage<MODE_FE, BX, BY, BZ, GEN><<<grid, nt, 0, st>>>(a); break;
   }
 }
 
 // Reconstruction in conservative or characteristic variables runs one tile
 // shape (16x4x4) of the general kernel; primitive variables choose the tuned
 // tile of the primitive-only kernel.
-static void launch_stage_variant(m2cx_solver* h, const StageArgs& a, int mode, int v) {
+static void launch_stage_variant(m2cx_solver* h, const StageArgs& a, int mode, int v, hipStream_t st = nullptr) {
+  if (!st) st = h->stream;
   if (a.s.var_type != M2CX_VAR_PRIMITIVE && a.s.rec_type == M2CX_REC_LINEAR) {
-    launch_stage_tile<16, 4, 4, true>(h, a, mode);
+    launch_stage_tile<16, 4, 4, true>(h, a, mode, st);
     return;
   }
   switch (v) {
-    case 1: launch_stage_tile<32, 4, 2, false>(h, a, mode); break;
-    case 2: launch_stage_tile<8, 8, 4, false>(h, a, mode); break;
-    case 3: launch_stage_tile<16, 8, 2, false>(h, a, mode); break;
-    default: launch_stage_tile<16, 4, 4, false>(h, a, mode); break;
+    case 1: launch_stage_tile<32, 4, 2, false>(h, a, mode, st); break;
+    case 2: launch_stage_tile<8, 8, 4, false>(h, a, mode, st); break;
+    case 3: launch_stage_tile<16, 8, 2, false>(h, a, mode, st); break;
+    default: launch_stage_tile<16, 4, 4, false>(h, a, mode, st); break;
   }
 }
 
-static void launch_stage(m2cx_solver* h, int vin, int vout, int mode) {
+// the tile shape the stage kernel runs with
+static void stage_tile_dims(const m2cx_solver* h, int& tx, int& ty, int& tz) {
+  if (h->s.var_type != M2CX_VAR_PRIMITIVE && h->s.rec_type == M2CX_REC_LINEAR) {
+    tx = 16; ty = 4; tz = 4;
+    return;
+  }
+  const TileVariant& t = kTiles[h->tile < 0 ? 0 : h->tile];
+  tx = t.bx; ty = t.by; tz = t.bz;
+}
+
+// phase 0: all tiles on the solver's stream; 1: tiles without interface
+// faces on stream2 (timed as "stage"); 2: the others on the solver's stream
+// (timed as "stage_if")
+static void launch_stage(m2cx_solver* h, int vin, int vout, int mode, int phase = 0) {
   StageArgs a = stage_args(h, vin, vout, mode);
+  a.phase = phase;
+  a.tflag = h->tflag;
+  hipStream_t st = (phase == 1) ? h->stream2 : h->stream;
+  const char* name = (phase == 2) ? "stage_if" : "stage";
   hipEvent_t e0{};
-  tic(h, "stage", &e0);
-  launch_stage_variant(h, a, mode, h->tile < 0 ? 0 : h->tile);
-  toc(h, "stage", e0);
+  tic(h, name, &e0, st);
+  launch_stage_variant(h, a, mode, h->tile < 0 ? 0 : h->tile, st);
+  toc(h, name, e0, st);
 }
 
 // Tile-shape autotuner: times the residual launch of every variant on the
@@ -810,11 +854,48 @@ static int end_of_step(m2cx_solver* h) {
   return 0;
 }
 
+// One stage with the stage kernel's tiles that touch no interface face
+// running on stream2 while the interface Riemann problems are solved: they
+// read V_in and write V_out (different fields) and no interface flux.  The
+// tiles with interface faces follow the Riemann launch; the level set waits
+// for both.  Results are those of the single launch (same code per tile).
+static void split_stage(m2cx_solver* h, const StageSpec& sp) {
+  const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  recon_stage(h, sp.vin);
+  if (sp.resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
+  hipEventRecord(h->ev_fork, h->stream);
+  hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+  launch_stage(h, sp.vin, sp.vout, sp.mode, 1);
+  hipEventRecord(h->ev_join, h->stream2);
+  guess_stage(h, sp.vin);
+  hipEvent_t e0{};
+  tic(h, "riemann", &e0);
+  mp_riemann(h->mp, h->g, h->s, cf5(h, h->V[sp.vin]), h->ID, sp.stage1, h->cnt, h->stream);
+  toc(h, "riemann", e0);
+  launch_stage(h, sp.vin, sp.vout, sp.mode, 2);
+  hipStreamWaitEvent(h->stream, h->ev_join, 0);
+  if (sp.resync)
+    k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s.mset, cf5(h, h->V[sp.vout]), f5(h, h->U1), h->ID,
+                                                       h->cnt);
+  if (sp.ls_which >= 0) levelset_stage(h, sp.vin, sp.ls_which);
+}
+
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
 static int step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
   if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
-  if (ti == M2CX_TI_FORWARD_EULER) {
+  if (h->overlap) {
+    int tx, ty, tz;
+    stage_tile_dims(h, tx, ty, tz);
+    const long nt = (long)((h->g.NX + tx - 1) / tx) * ((h->g.NY + ty - 1) / ty) * ((h->g.NZ + tz - 1) / tz);
+    k_clear_u8<<<blocks_for(nt), 256, 0, h->stream>>>(h->tflag, nt);
+    k_tile_flags<<<blocks_for(h->mp.L.cap), 256, 0, h->stream>>>(h->g, h->mp.L, h->cnt, h->tflag, tx, ty, tz);
+  }
+  if (ti == M2CX_TI_FORWARD_EULER && h->overlap) {
+    split_stage(h, StageSpec{0, 1, MODE_FE, 1, 0, -1});
+    for (int p = 0; p < 5; p++) std::swap(h->V[0][p], h->V[1][p]);
+    levelset_stage(h, 0, 0);
+  } else if (ti == M2CX_TI_FORWARD_EULER) {
     interface_stage(h, 0, 1);
     launch_stage(h, 0, 1, MODE_FE);
     // V <- new state (pointer swap), then the level set with the new
@@ -825,8 +906,12 @@ static int step_kernels(m2cx_solver* h) {
     StageSpec sp[3];
     const int ns = stage_specs(ti, sp);
     for (int q = 0; q < ns; q++) {
-      interface_stage(h, sp[q].vin, sp[q].stage1);
-      stage_after_riemann(h, sp[q]);
+      if (h->overlap) {
+        split_stage(h, sp[q]);
+      } else {
+        interface_stage(h, sp[q].vin, sp[q].stage1);
+        stage_after_riemann(h, sp[q]);
+      }
     }
   }
   return end_of_step(h);

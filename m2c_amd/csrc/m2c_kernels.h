@@ -49,6 +49,10 @@ struct StageArgs {
   const int* fidx[3];
   IfaceList L;
   int mode;
+  // 0: every tile; 1: only tiles without interface faces; 2: only tiles with
+  // them (tflag per tile, set by k_tile_flags for this tile shape)
+  int phase;
+  const unsigned char* tflag;
   DevCounters* cnt;
 };
 
@@ -157,6 +161,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   const int per = (ntiles + NXCD - 1) / NXCD;
   const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
   if (tile >= ntiles) return;
+  if (a.phase != 0 && ((a.tflag[tile] != 0) != (a.phase == 2))) return;
   const int ib = (tile % ntx) * TX, jb = ((tile / ntx) % nty) * TY, kb = (tile / (ntx * nty)) * TZ;
   const int i = ib + tx, j = jb + ty, k = kb + tz;
   const bool inside = (i < g.NX) && (j < g.NY) && (k < g.NZ);

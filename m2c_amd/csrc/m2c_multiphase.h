@@ -1262,6 +1262,30 @@ inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, c
                                          B.guess ? B.pguess : nullptr, cnt);
 }
 
+// Tiles of the stage kernel (TX x TY x TZ cells, numbered as k_stage numbers
+// them) that compute a flux through an interface face: the tiles of both
+// cells of every listed face.  The other tiles need no Riemann result and can
+// run while the interface problems are solved.
+__global__ void k_tile_flags(DevGrid g, IfaceList L, const DevCounters* cnt, unsigned char* tflag, int TX, int TY,
+                             int TZ) {
+  const int nif = min(cnt->nif, L.cap);
+  const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nif; q += gridDim.x * blockDim.x) {
+    long r = L.cell[q];
+    const int i = (int)(r % g.SX) - 1; r /= g.SX;
+    const int j = (int)(r % g.SY) - 1;
+    const int k = (int)(r / g.SY) - 1;
+    const int d = L.dir[q];
+    const int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
+    tflag[(i / TX) + ntx * ((j / TY) + nty * (k / TZ))] = 1;
+    tflag[(im / TX) + ntx * ((jm / TY) + nty * (km / TZ))] = 1;
+  }
+}
+
+__global__ void k_clear_u8(unsigned char* p, long n) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) p[c] = 0;
+}
+
 // plain copies as kernels (no memcpy nodes when a step is captured in a graph)
 __global__ void k_copy_u8(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src, long n) {
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) dst[c] = src[c];
