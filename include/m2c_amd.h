@@ -385,6 +385,14 @@ int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int ep
 int m2cx_autotune(m2cx_solver* h, int* variant, float* ms_per_variant);
 int m2cx_set_tile(m2cx_solver* h, int variant);
 
+/* No counterpart in the reference (scheduling only).  With material
+ * interfaces, each stage runs the stage kernel's tiles that touch no
+ * interface face on a second stream while the interface Riemann problems
+ * are solved, and the other tiles after them (default: on; results are the
+ * same either way).  m2cx_set_overlap(h, 0) runs one launch per stage after
+ * the Riemann solve, e.g. to time the stage kernel alone. */
+int m2cx_set_overlap(m2cx_solver* h, int on);
+
 /* ------------------------------------------------------------------ */
 /* Ensembles                                                           */
 /* ------------------------------------------------------------------ */

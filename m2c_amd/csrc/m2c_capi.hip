@@ -1518,6 +1518,12 @@ int m2cx_set_tile(m2cx_solver* h, int variant) {
   return 0;
 }
 
+int m2cx_set_overlap(m2cx_solver* h, int on) {
+  if (!h) return M2CX_E_ARG;
+  h->overlap = on != 0 && h->stream2 != nullptr;
+  return 0;
+}
+
 int m2cx_profile(m2cx_solver* h, int enable) {
   h->profile = enable != 0;
   return 0;

@@ -51,6 +51,7 @@ def lib():
         L.m2cx_levelset_residual.argtypes = [P, C.c_int]
         L.m2cx_download_ls_residual.argtypes = [P, D]
         L.m2cx_levelset_reinitialize.argtypes = [P, C.c_int]
+        L.m2cx_set_overlap.argtypes = [P, C.c_int]
         L.m2cx_advance_one_time_step.argtypes = [P, C.c_double, C.c_double, C.c_int]
         L.m2cx_run_steps.argtypes = [P, C.c_int]
         L.m2cx_get_counters.argtypes = [P, C.POINTER(abi.Counters)]
@@ -177,6 +178,11 @@ class Solver:
         out = np.zeros(self.shape)
         self._check(self.L.m2cx_download_ls_residual(self.h, _d(out)))
         return out
+
+    def set_overlap(self, on):
+        """Stage kernel tiles without interface faces during the Riemann
+        solve (on, the default) or one launch per stage after it."""
+        return self._check(self.L.m2cx_set_overlap(self.h, int(bool(on))))
 
     def reinitialize(self, special_maxits=0):
         """LevelSetOperator::Reinitialize with must_do = true: level set 0
