@@ -863,15 +863,18 @@ static void split_stage(m2cx_solver* h, const StageSpec& sp) {
   const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
   recon_stage(h, sp.vin);
   if (sp.resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
-  hipEventRecord(h->ev_fork, h->stream);
-  hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-  launch_stage(h, sp.vin, sp.vout, sp.mode, 1);
-  hipEventRecord(h->ev_join, h->stream2);
   guess_stage(h, sp.vin);
+  hipEventRecord(h->ev_fork, h->stream);
+  // the Riemann launch goes first: its waves (one per SIMD) take their
+  // places before the stage kernel fills the rest, so a long Riemann launch
+  // is not held back behind stage tiles
   hipEvent_t e0{};
   tic(h, "riemann", &e0);
   mp_riemann(h->mp, h->g, h->s, cf5(h, h->V[sp.vin]), h->ID, sp.stage1, h->cnt, h->stream);
   toc(h, "riemann", e0);
+  hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+  launch_stage(h, sp.vin, sp.vout, sp.mode, 1);
+  hipEventRecord(h->ev_join, h->stream2);
   launch_stage(h, sp.vin, sp.vout, sp.mode, 2);
   hipStreamWaitEvent(h->stream, h->ev_join, 0);
   if (sp.resync)
