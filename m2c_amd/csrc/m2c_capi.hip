@@ -883,17 +883,21 @@ static void split_stage(m2cx_solver* h, const StageSpec& sp) {
   if (sp.ls_which >= 0) levelset_stage(h, sp.vin, sp.ls_which);
 }
 
+// marks the stage-kernel tiles that hold a cell of an interface face (after
+// the step's interface list is built)
+static void tile_flags(m2cx_solver* h) {
+  int tx, ty, tz;
+  stage_tile_dims(h, tx, ty, tz);
+  const long nt = (long)((h->g.NX + tx - 1) / tx) * ((h->g.NY + ty - 1) / ty) * ((h->g.NZ + tz - 1) / tz);
+  k_clear_u8<<<blocks_for(nt), 256, 0, h->stream>>>(h->tflag, nt);
+  k_tile_flags<<<blocks_for(h->mp.L.cap), 256, 0, h->stream>>>(h->g, h->mp.L, h->cnt, h->tflag, tx, ty, tz);
+}
+
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
 static int step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
   if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
-  if (h->overlap) {
-    int tx, ty, tz;
-    stage_tile_dims(h, tx, ty, tz);
-    const long nt = (long)((h->g.NX + tx - 1) / tx) * ((h->g.NY + ty - 1) / ty) * ((h->g.NZ + tz - 1) / tz);
-    k_clear_u8<<<blocks_for(nt), 256, 0, h->stream>>>(h->tflag, nt);
-    k_tile_flags<<<blocks_for(h->mp.L.cap), 256, 0, h->stream>>>(h->g, h->mp.L, h->cnt, h->tflag, tx, ty, tz);
-  }
+  if (h->overlap) tile_flags(h);
   if (ti == M2CX_TI_FORWARD_EULER && h->overlap) {
     split_stage(h, StageSpec{0, 1, MODE_FE, 1, 0, -1});
     for (int p = 0; p < 5; p++) std::swap(h->V[0][p], h->V[1][p]);
@@ -1321,6 +1325,11 @@ struct m2cx_ensemble {
   int lanes = RIEM_LANES, pathcap = 512;  // integration-path scratch: one slot per lane
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // members' stage tiles without interface faces run on stream2 while the
+  // batched Riemann launch runs on stream
+  bool overlap = false;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
 };
 
@@ -1343,18 +1352,44 @@ static void ensemble_step(m2cx_ensemble* e) {
   }
   StageSpec sp[3];
   stage_specs(e->ti, sp);
+  // split: the member's stage launch is divided by tiles as in split_stage
+  auto split = [e](const m2cx_solver* h) { return e->overlap && h->multimat; };
+  if (e->overlap)
+    for (m2cx_solver* h : e->m)
+      if (split(h)) tile_flags(h);
   for (int q = 0; q < e->nstages; q++) {
     for (m2cx_solver* h : e->m) {
       recon_stage(h, sp[q].vin);
+      if (split(h) && sp[q].resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
       guess_stage(h, sp[q].vin);
     }
+    if (e->overlap) hipEventRecord(e->ev_fork, e->stream);
     if (e->nriem) {
       k_riem_offsets<<<1, 1, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff);
       k_riemann_multi<<<e->lanes / 64, 64, 0, e->stream>>>(e->dmem[q], e->nriem, e->doff, sp[q].stage1, e->slab,
                                                            e->lanes, e->pathcap);
       k_riem_count<<<e->nriem, 256, 0, e->stream>>>(e->dmem[q]);
     }
-    for (m2cx_solver* h : e->m) stage_after_riemann(h, sp[q]);
+    if (e->overlap) {
+      hipStreamWaitEvent(e->stream2, e->ev_fork, 0);
+      for (m2cx_solver* h : e->m)
+        if (split(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 1);
+      hipEventRecord(e->ev_join, e->stream2);
+      for (m2cx_solver* h : e->m)
+        if (split(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 2);
+      hipStreamWaitEvent(e->stream, e->ev_join, 0);
+    }
+    for (m2cx_solver* h : e->m) {
+      if (!split(h)) {
+        stage_after_riemann(h, sp[q]);
+        continue;
+      }
+      const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+      if (sp[q].resync)
+        k_resync<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s.mset, cf5(h, h->V[sp[q].vout]), f5(h, h->U1),
+                                                           h->ID, h->cnt);
+      if (sp[q].ls_which >= 0) levelset_stage(h, sp[q].vin, sp[q].ls_which);
+    }
   }
   for (m2cx_solver* h : e->m)
     if (h->multimat && h->sch.n_levelsets > 0)
@@ -1403,12 +1438,30 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
   }
   for (m2cx_solver* h : e->m)
     if (h->tile < 0 && autotune(h, nullptr)) return efail(e, M2CX_E_HIP, "autotune: " + h->err);
-  // every member works on the ensemble stream while it is captured and run
-  for (m2cx_solver* h : e->m) h->stream = e->stream;
+  // overlap when every multi-material member has it on (the M2C_NO_OVERLAP
+  // environment variable or m2cx_set_overlap turns it off) and there is one
+  e->overlap = e->nriem > 0;
+  for (m2cx_solver* h : e->m)
+    if (h->multimat && !h->overlap) e->overlap = false;
+  if (e->overlap) {
+    EHIPCHK(e, hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
+    EHIPCHK(e, hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    EHIPCHK(e, hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+  }
+  // every member works on the ensemble streams while it is captured and run
+  std::vector<hipStream_t> s2(e->m.size());
+  for (size_t i = 0; i < e->m.size(); i++) {
+    e->m[i]->stream = e->stream;
+    s2[i] = e->m[i]->stream2;
+    e->m[i]->stream2 = e->stream2;
+  }
   EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   ensemble_step(e);
   hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
-  for (m2cx_solver* h : e->m) h->stream = h->stream_saved;
+  for (size_t i = 0; i < e->m.size(); i++) {
+    e->m[i]->stream = e->m[i]->stream_saved;
+    e->m[i]->stream2 = s2[i];
+  }
   if (ce != hipSuccess) return efail(e, M2CX_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
   EHIPCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
   return 0;
@@ -1440,6 +1493,9 @@ int m2cx_ensemble_destroy(m2cx_ensemble* e) {
   for (auto* p : e->dmem) hipFree(p);
   hipFree(e->doff);
   hipFree(e->slab);
+  if (e->ev_fork) hipEventDestroy(e->ev_fork);
+  if (e->ev_join) hipEventDestroy(e->ev_join);
+  if (e->stream2) hipStreamDestroy(e->stream2);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return 0;
