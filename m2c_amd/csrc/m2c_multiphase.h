@@ -147,6 +147,11 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
   B.guess = sch.riem_guess;
   // per-lane integration-path scratch: lanes x 2 paths x cap points x 3
   B.lanes = RIEM_LANES;
+  if (const char* e = getenv("M2C_RIEMANN_LANES")) {  // launch-shape override: a multiple of 64
+    const long v = atol(e);
+    if (v < 64 || v > 4L * RIEM_LANES || v % 64) return M2CX_E_ARG;
+    B.lanes = (int)v;
+  }
   B.pathcap = 512;
   // the integration-path slab is allocated on first use (mp_slab): solvers
   // that run in an ensemble share the ensemble's slab instead
