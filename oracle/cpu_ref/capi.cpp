@@ -237,6 +237,19 @@ int orc_wave(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int wa
   return rs.RhoUStar(wave, path, rho, u, p, ps, idd, rhos0, rhos1, *rhos, *us) ? 1 : 0;
 }
 
+// Equation-of-state point values for closed-form tests: out = {e(rho, p),
+// p(rho, e), dp/drho(rho, e), Gamma(rho), c^2(rho, e)} with e = e(rho, p).
+int orc_eos_eval(const m2cx_material* mat, double rho, double p, double* out) {
+  Eos E = Eos::from(*mat);
+  const double e = E.GetInternalEnergyPerUnitMass(rho, p);
+  out[0] = e;
+  out[1] = E.GetPressure(rho, e);
+  out[2] = E.GetDpdrho(rho, e);
+  out[3] = E.GetBigGamma(rho, e);
+  out[4] = E.ComputeSoundSpeedSquare(rho, e);
+  return 0;
+}
+
 double orc_cr_root(double x, int n) { return m2c::cr_root(x, n); }
 
 void orc_set_ck_textbook(int on) { orc::g_ck_textbook = on != 0; }

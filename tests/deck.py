@@ -1,12 +1,12 @@
 """Turn a parsed reference input deck (tests/golden/riemann_problems.json) into
 ABI structures and the initial fields, as Main.cpp:85-382 and
-SpaceInitializer.cpp:98-482,1390-1514 set them up (uniform mesh, plane
-geometries, default state = Inlet, level set by signed distance)."""
+SpaceInitializer.cpp:98-482,1390-1514 set them up (uniform mesh, plane and
+sphere geometries, default state = Inlet, level set by signed distance)."""
 import numpy as np
 
 from m2c_amd import abi
 
-BC_NAMES = {"Inlet": abi.BC_INLET, "Farfield": abi.BC_INLET, "Inlet2": abi.BC_INLET2,
+BC_NAMES = {"Inlet": abi.BC_INLET, "Farfield": abi.BC_INLET, "Inlet2": abi.BC_INLET2, "Farfield2": abi.BC_INLET2,
             "SlipWall": abi.BC_SLIPWALL, "StickWall": abi.BC_STICKWALL, "Symmetry": abi.BC_SYMMETRY,
             "Outlet": abi.BC_OUTLET}
 FLUX = {"Roe": abi.FLUX_ROE, "LocalLaxFriedrichs": abi.FLUX_LLF, "HLLC": abi.FLUX_HLLC}
@@ -73,8 +73,10 @@ def setup(deck):
     bcs = [BC_NAMES[me.get(k, "Outlet")] for k in ("BoundaryConditionX0", "BoundaryConditionXmax",
                                                    "BoundaryConditionY0", "BoundaryConditionYmax",
                                                    "BoundaryConditionZ0", "BoundaryConditionZmax")]
-    inlet, inlet_id = state(deck.get("BoundaryConditions", {}).get("Inlet", {}))
-    inlet2, inlet2_id = state(deck.get("BoundaryConditions", {}).get("Inlet2", {}))
+    # "Farfield" and "Farfield2" name the same states (IoData.cpp:1974-1979)
+    bcd = deck.get("BoundaryConditions", {})
+    inlet, inlet_id = state(bcd.get("Inlet", bcd.get("Farfield", {})))
+    inlet2, inlet2_id = state(bcd.get("Inlet2", bcd.get("Farfield2", {})))
     g = abi.grid(n, (me["X0"], me["Y0"], me["Z0"]), (me["Xmax"], me["Ymax"], me["Zmax"]), bcs, inlet, inlet2,
                  inlet_id, inlet2_id)
     mats, nmat = materials(deck)
@@ -130,6 +132,18 @@ def setup(deck):
                                 "IterativelyConstrainedMethod2": abi.REINIT_ITERATIVE_CONSTRAINED2}[
                                     rei.get("FirstLayerTreatment", "Fixed")]
     s.time_step = ti.get("TimeStep", 0.0)
+    # ExactRiemannSolverData (IoData.cpp:1640-1675): keys the deck sets
+    ers = deck.get("ExactRiemannSolution", {})
+    for key, field in (("MaxIts", "riem_maxits_main"), ("MaxItsBracketing", "riem_maxits_bracket"),
+                       ("MaxItsShock", "riem_maxits_shock"), ("IntegrationSteps", "riem_steps_rarefaction")):
+        if key in ers:
+            setattr(s, field, int(ers[key]))
+    for key, field in (("Tolerance", "riem_tol_main"), ("ToleranceShock", "riem_tol_shock"),
+                       ("ToleranceRarefaction", "riem_tol_rarefaction"), ("MinPressure", "riem_min_pressure"),
+                       ("FailureThreshold", "riem_failure_threshold"),
+                       ("PrescribedPressureUponFailure", "riem_pressure_at_failure")):
+        if key in ers:
+            setattr(s, field, float(ers[key]))
     # initial condition
     px, py, pz = n[0] + 2, n[1] + 2, n[2] + 2
     V = np.zeros((pz, py, px, 5))
@@ -144,16 +158,32 @@ def setup(deck):
     geo = deck.get("InitialCondition", {}).get("GeometricEntities", {})
     X, Y, Z = np.meshgrid(x[1:-1], y[1:-1], z[1:-1], indexing="ij")
     X, Y, Z = X.transpose(2, 1, 0), Y.transpose(2, 1, 0), Z.transpose(2, 1, 0)
-    for key in sorted(geo):
-        if not key.startswith("Plane"):
+    # geometries in the reference's default order: planes, then spheres, each
+    # by index (SpaceInitializer.cpp:148-198)
+    kinds = {"Plane": 0, "Sphere": 3}
+    for key in sorted(geo, key=lambda k: (kinds.get(k.split("[")[0], 99), int(k.split("[")[1][:-1]))):
+        kind = key.split("[")[0]
+        if kind not in kinds:
             raise NotImplementedError(key)
         pl = geo[key]
-        x0 = np.array([pl.get("Point_x", 0.0), pl.get("Point_y", 0.0), pl.get("Point_z", 0.0)])
-        d = np.array([pl.get("Normal_x", 0.0), pl.get("Normal_y", 0.0), pl.get("Normal_z", 0.0)])
-        d = d * (1.0 / np.sqrt(d @ d))
-        dist = (X - x0[0]) * d[0] + (Y - x0[1]) * d[1] + (Z - x0[2]) * d[2]
+        if kind == "Plane":
+            x0 = np.array([pl.get("Point_x", 0.0), pl.get("Point_y", 0.0), pl.get("Point_z", 0.0)])
+            d = np.array([pl.get("Normal_x", 0.0), pl.get("Normal_y", 0.0), pl.get("Normal_z", 0.0)])
+            d = d * (1.0 / np.sqrt(d @ d))
+            dist = (X - x0[0]) * d[0] + (Y - x0[1]) * d[1] + (Z - x0[2]) * d[2]
+            sel = dist > 0
+        else:
+            # DistanceFromPointToSphere (GeoTools/DistancePointToSphere.h):
+            # |x - x0| - R, negative inside; Side = Interior by default
+            if pl.get("Side", "Interior") != "Interior":
+                raise NotImplementedError("Sphere Side = " + str(pl.get("Side")))
+            x0 = (pl.get("Center_x", 0.0), pl.get("Center_y", 0.0), pl.get("Center_z", 0.0))
+            dx, dy, dz = X - x0[0], Y - x0[1], Z - x0[2]
+            dist = np.sqrt(dx * dx + dy * dy + dz * dz) - pl["Radius"]
+            sel = dist < 0
+            # the level set takes +dist for an interior sphere (:1490-1514)
+            dist = -dist
         st, mid = state(pl["InitialState"])
-        sel = dist > 0
         V[1:-1, 1:-1, 1:-1][sel] = st
         ID[1:-1, 1:-1, 1:-1][sel] = mid
         if ls and mid == s.ls_materialid:
@@ -183,3 +213,40 @@ def probe_rows(deck, g, V, ID, Phi):
     _, rows = output.line_plot_rows(g, V, np.asarray(ID, dtype=np.float64), [Phi], p0, p1,
                                     int(lp["NumberOfPoints"]))
     return np.array(rows)
+
+
+_AXIS_KEYS = (("NumberOfCells%s", "XYZ"), ("%s0", "XYZ"), ("%smax", "XYZ"), ("BoundaryCondition%s0", "XYZ"),
+              ("BoundaryCondition%smax", "XYZ"), ("Velocity%s", "XYZ"), ("Point_%s", "xyz"), ("Normal_%s", "xyz"),
+              ("Center_%s", "xyz"))
+
+
+def permute_deck(deck, perm):
+    """The same deck with the coordinate axes relabelled: old axis a becomes
+    axis perm[a] (mesh extents, cell counts, boundary conditions, velocity
+    components, geometry points and normals, line-plot end points)."""
+    import copy
+
+    def walk(d):
+        out = {}
+        for key, val in d.items():
+            new = key
+            for pat, names in _AXIS_KEYS:
+                for a in range(3):
+                    if key == pat % names[a]:
+                        new = pat % names[perm[a]]
+            out[new] = walk(val) if isinstance(val, dict) else val
+        return out
+
+    return walk(copy.deepcopy(deck))
+
+
+def permute_fields(V, ID, Phi, perm):
+    """Fields of an axis-permuted run brought back to the original axes:
+    array axes (k, j, i) and velocity components (1, 2, 3).  perm is the map
+    given to permute_deck."""
+    # array axis of coordinate a is 2 - a; the permuted run stores old axis a
+    # along array axis 2 - perm[a]
+    order = [2 - perm[2 - ax] for ax in range(3)]
+    Vb = np.transpose(V, order + [3]).copy()
+    Vb[..., 1:4] = Vb[..., [1 + perm[a] for a in range(3)]]
+    return Vb, np.transpose(ID, order).copy(), None if Phi is None else np.transpose(Phi, order).copy()

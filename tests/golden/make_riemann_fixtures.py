@@ -5,7 +5,10 @@ Reads (text only, nothing is executed) from the reference checkout:
   Tests/RiemannProblems/Test{1..12}/exact/RiemannSolution.txt  exact solver profile
   Tests/RiemannProblems/Test{1..12}/results/line_0001.txt   full-pipeline line plot
   Tests/RiemannProblems/Test{1..12}/log.out                 per-step t and dt
-and writes tests/golden/riemann_problems.json.  Run:
+and writes tests/golden/riemann_problems.json; the same three files of the
+two-dimensional explosion deck
+  Tests/ExplosionShyue98/{input.st,results/line_0001.txt,log.out}
+go to tests/golden/shyue98.json.  Run:
   python tests/golden/make_riemann_fixtures.py /root/reference
 """
 import json
@@ -92,6 +95,16 @@ def main(ref):
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "riemann_problems.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=0)
+    print("wrote", dst)
+    d = os.path.join(ref, "Tests", "ExplosionShyue98")
+    with open(os.path.join(d, "input.st")) as f:
+        deck = parse_deck(f.read())
+    lh, lr = parse_table(os.path.join(d, "results", "line_0001.txt"))
+    shy = {"deck": deck, "line_header": lh, "line_rows": lr, "log_steps": parse_log(os.path.join(d, "log.out")),
+           "source": "Tests/ExplosionShyue98"}
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shyue98.json")
+    with open(dst, "w") as f:
+        json.dump(shy, f, indent=0)
     print("wrote", dst)
 
 

@@ -136,3 +136,26 @@ def unresolved_case(reinit):
     phi = np.broadcast_to(x - 0.31, V.shape[:3]).copy()
     ID = np.zeros(V.shape[:3], dtype=np.int32)
     return g, mats, 2, s, V, ID, phi
+
+
+def doubled_plane(n=(24, 20, 16), flt=abi.REINIT_FIXED, maxits=400):
+    """Twice the signed distance to a tilted plane: a linear field with
+    |grad phi| = 2, whose exact reinitialization is the distance d itself."""
+    g = abi.grid(n, (0, 0, 0), (1.2, 1.0, 0.8))
+    mats = abi.materials(abi.material("SG"), abi.material("SG"))
+    s = abi.default_scheme(n_levelsets=1, reinit_frequency=1, reinit_maxits=maxits, reinit_tol=1e-7,
+                           reinit_first_layer=flt)
+    for q in range(6):
+        s.ls_bc[q] = abi.LSBC_LINEAR_EXTRAPOLATION
+    x = np.array(configs._axis(n[0], 0, 1.2))[None, None, :]
+    y = np.array(configs._axis(n[1], 0, 1.0))[None, :, None]
+    z = np.array(configs._axis(n[2], 0, 0.8))[:, None, None]
+    nrm = np.array([0.6, 0.5, 0.3])
+    nrm = nrm / np.sqrt(nrm @ nrm)
+    d = (x - 0.61) * nrm[0] + (y - 0.47) * nrm[1] + (z - 0.38) * nrm[2]
+    phi = 2.0 * d
+    V = np.zeros(phi.shape + (5,))
+    V[..., 0] = 1.0
+    V[..., 4] = 1.0
+    ID = (phi < 0).astype(np.int32)
+    return g, mats, 2, s, V, ID, phi, d

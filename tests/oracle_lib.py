@@ -54,6 +54,7 @@ def lib():
                                         C.c_int, D, C.c_int, D, C.c_int]
         L.orc_wave.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, C.c_double,
                                C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, D, D]
+        L.orc_eos_eval.argtypes = [C.POINTER(abi.Material), C.c_double, C.c_double, D]
         L.orc_cr_root.restype = C.c_double
         L.orc_cr_root.argtypes = [C.c_double, C.c_int]
         L.orc_set_ck_textbook.argtypes = [C.c_int]
@@ -236,3 +237,10 @@ def set_ck_textbook(on):
 
 def cr_root(x, n):
     return lib().orc_cr_root(x, n)
+
+
+def eos_eval(mat, rho, p):
+    """(e(rho, p), p(rho, e), dp/drho|_e, Gamma, c^2) of one material."""
+    out = np.zeros(5)
+    lib().orc_eos_eval(C.byref(mat), rho, p, dp(out))
+    return out

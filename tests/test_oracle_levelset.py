@@ -165,3 +165,53 @@ def test_unresolved_cells_are_fixed_without_reinitialization():
         assert c["unresolved_cells"] > 0 and c["reinit_calls"] == 0
         _, IDo, _ = o.get_state()
         assert (IDo[1:-1, 1:-1, 1:-1] == 1).sum() > 0
+
+
+def _first_layer(phi):
+    """Interior nodes with a sign change to an axis neighbour."""
+    first = np.zeros(phi.shape, bool)
+    for ax in range(3):
+        sl_a = [slice(None)] * 3
+        sl_b = [slice(None)] * 3
+        sl_a[ax] = slice(1, None)
+        sl_b[ax] = slice(None, -1)
+        cross = phi[tuple(sl_a)] * phi[tuple(sl_b)] <= 0
+        first[tuple(sl_a)] |= cross
+        first[tuple(sl_b)] |= cross
+    inner = np.zeros(phi.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    return first & inner
+
+
+@pytest.mark.parametrize("flt", [abi.REINIT_FIXED, abi.REINIT_CONSTRAINED1, abi.REINIT_CONSTRAINED2])
+def test_reinit_of_doubled_plane_distance(flt):
+    """phi = 2 d for a tilted plane.  The first-layer rescaling phi/|grad phi|
+    (LevelSetReinitializer.cpp:669-716) is exact on linear data, and both
+    constrained averages (CR-1, CR-2: :784-833) then reproduce phi/2 exactly,
+    so CR-1 and CR-2 put the first layer on d to rounding; the iterations then
+    bring the band around it close to d.  Fixed keeps 2 d there, and the band
+    follows it.  (HCR-1/HCR-2 move the first layer by a forcing term during
+    the iterations, which has no such closed form.)"""
+    g, mats, nmat, s, V, ID, phi, d = lc.doubled_plane(flt=flt)
+    o = ol.Oracle(g, mats, nmat, s)
+    o.set_state(V, ID, phi)
+    o.ls_apply_bc()
+    o.reinitialize()
+    _, _, P = o.get_state()
+    # nodes two cells or more from the domain boundary: next to it the
+    # curvature test of the correction set and the first-layer differences
+    # use one-sided stencils and extrapolated ghosts
+    away = np.zeros(phi.shape, bool)
+    away[3:-3, 3:-3, 3:-3] = True
+    first = _first_layer(phi) & away
+    h = 0.05
+    band = (np.abs(d) < 4 * h) & away
+    if flt == abi.REINIT_FIXED:
+        assert np.allclose(P[first], phi[first], rtol=1e-14, atol=0)
+        assert np.abs(P[band] - d[band]).max() > 0.1 * h
+    else:
+        assert np.abs(P[first] - d[first]).max() <= 1e-13 * h, flt
+        # beyond the first layer: what the boundary stencils leave (0.04 h
+        # seen), far below the 2 d it started from
+        assert np.abs(P[band] - d[band]).max() <= 0.1 * h, flt
+    assert np.array_equal(P[1:-1, 1:-1, 1:-1] < 0, phi[1:-1, 1:-1, 1:-1] < 0)
