@@ -347,10 +347,17 @@ int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir,
  * SiLU, BF16 MFMA with FP32 accumulation) predicts p* of every interface
  * problem; the prediction replaces the acoustic-theory starting pressure of
  * FindInitialFeasiblePointsByAcousticTheory (ExactRiemannSolverBase.cpp:
- * 1107-1158) and the FP64 iteration after it is unchanged, so results agree
- * with the plain solver to the Riemann tolerance.  A prediction that is not a
- * finite positive pressure falls back to the acoustic guess (counted in
- * riemann_guess_fallbacks).  Features x[8] and target T of a problem:
+ * 1107-1158), where the acoustic pressure is not positive or the acoustic
+ * wave strength lies in [0.3, 10), and the FP64 iteration after it is
+ * unchanged.  Accuracy rule (tests/learned_rule.py): where both waves are
+ * shocks the result agrees with the converged root within 10 x the Riemann
+ * tolerance (p* on the scale max|p + rho u_n^2/2|, u* on max c, densities
+ * relative); with a rarefaction, whose integration path depends on the
+ * pressures tried before, the result is as accurate as the plain solver's
+ * against a solution with ToleranceRarefaction 1e-11 (99th percentile and
+ * maximum error within 1.25 x the plain solver's + 10 x tolerance), not
+ * equal to it.  A prediction that is not a finite positive pressure falls
+ * back to the acoustic guess (counted in riemann_guess_fallbacks).  Features x[8] and target T of a problem:
  * P = max(pL+pcL, pR+pcR), R = max(rhoL, rhoR), C = sqrt(P/R),
  * x = [log(rhoL/R), log(rhoR/R), unL/C, unR/C, log((pL+pcL)/P),
  * log((pR+pcR)/P), gammaL, gammaR], T = log((p* + min(pcL, pcR))/P).

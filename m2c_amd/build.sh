@@ -13,3 +13,13 @@ $HIPCC --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
   -I../include -o libm2c_amd.so.tmp csrc/m2c_capi.hip "$@"
 # replace the library in one step, so that a reader never sees a partial file
 mv -f libm2c_amd.so.tmp libm2c_amd.so
+# C++ facades with the reference's SpaceVariable3D / SpaceOperator /
+# TimeIntegrator signatures over the C ABI (host code only), and the test
+# driver that runs the reference's time loop through them.
+CXX=${CXX:-g++}
+$CXX -O2 -std=c++17 -fPIC -shared -Wall -Wextra -I../include -o libm2c_amd_facade.so.tmp facade/m2c_facade.cpp \
+  -L. -lm2c_amd -Wl,-rpath,'$ORIGIN'
+mv -f libm2c_amd_facade.so.tmp libm2c_amd_facade.so
+$CXX -O2 -std=c++17 -fPIC -shared -Wall -Wextra -I../include -o ../tests/libfacade_driver.so.tmp \
+  ../tests/facade_driver.cpp -L. -lm2c_amd_facade -lm2c_amd -Wl,-rpath,'$ORIGIN/../m2c_amd'
+mv -f ../tests/libfacade_driver.so.tmp ../tests/libfacade_driver.so

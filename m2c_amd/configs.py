@@ -213,3 +213,102 @@ def c5_bubble_cloud(n=(512, 256, 256), nbubbles=200, seed=5, mach=2.0, shock_x=0
     for cx, cy, cz, r in placed:
         geoms.append(abi.geometry("sphere", (cx, cy, cz), radius=r, materialid=1, state=(1.2, 0.0, 0.0, 0.0, p_w)))
     return g, mats, 2, s, (rho_w, 0.0, 0.0, 0.0, p_w), 0, geoms
+
+
+def c2_geometries(n=256, L=1.0, radius=0.15, mach=1.5, shock_x=0.2, tol=1e-10):
+    """c2_shock_bubble as geometric entities for Solver.initialize (no host
+    arrays): grid, materials, scheme, default state and ID, geometries."""
+    g, mats, nmat, s = _c2_setup(n, L, tol)
+    st, sid, geoms = shock_bubble_geometries(L, radius * L, mach, shock_x * L, WATER_SG)
+    return g, mats, nmat, s, st, sid, geoms
+
+
+def _c2_setup(n, L, tol):
+    g = abi.grid((n, n, n), (0.0, 0.0, 0.0), (L, L, L), (abi.BC_OUTLET,) * 6)
+    mats = abi.materials(abi.material("SG", **WATER_SG), abi.material("SG", **AIR_SG))
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=abi.TI_RK3, cfl=0.4,
+                           n_levelsets=1, ls_materialid=1, riem_tol_main=tol)
+    return g, mats, 2, s
+
+
+def c4_geometries(count=64, seed=4, edges=(64, 96, 128, 160, 192)):
+    """c4_instances as geometric entities: the same draws (edge, radius,
+    Mach) per instance, each built on the device by Solver.initialize."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(edges))
+        r = float(rng.uniform(0.05, 0.25))
+        m = float(rng.uniform(1.2, 3.0))
+        out.append(((n, r, m), c2_geometries(n, radius=r, mach=m)))
+    return out
+
+
+def brick_split(n, parts):
+    """Factor `parts` into (px, py, pz) dividing the grid n into bricks as
+    close to cubes as possible (cells per brick axis n_a / p_a)."""
+    best = None
+    for px in range(1, parts + 1):
+        if parts % px:
+            continue
+        for py in range(1, parts // px + 1):
+            if (parts // px) % py:
+                continue
+            pz = parts // px // py
+            if n[0] % px or n[1] % py or n[2] % pz:
+                continue
+            b = (n[0] // px, n[1] // py, n[2] // pz)
+            score = max(b) / min(b)
+            if best is None or score < best[0]:
+                best = (score, (px, py, pz))
+    if best is None:
+        return (parts, 1, 1)
+    return best[1]
+
+
+def initialize_host(grid, scheme, default_state, default_id, geoms, lo=(0, 0, 0), n=None):
+    """What Solver.initialize builds on the device (planes and spheres in
+    order, the level set as the signed distance of the nearest surface of its
+    material; SpaceInitializer.cpp:237-482, 1390-1514), on the host, for the
+    brick of `n` cells starting at cell `lo` of `grid`.  Returns the brick's
+    grid (its own coordinates, Outlet boundaries) and its ghosted V, ID, Phi."""
+    n = n or (grid.nx, grid.ny, grid.nz)
+    full = (grid.nx, grid.ny, grid.nz)
+    x0 = (grid.x0, grid.y0, grid.z0)
+    x1 = (grid.xmax, grid.ymax, grid.zmax)
+    h = [(x1[a] - x0[a]) / full[a] for a in range(3)]
+    blo = tuple(x0[a] + lo[a] * h[a] for a in range(3))
+    bhi = tuple(x0[a] + (lo[a] + n[a]) * h[a] for a in range(3))
+    bg = abi.grid(n, blo, bhi, (abi.BC_OUTLET,) * 6)
+    xs = [_axis(n[a], blo[a], bhi[a]) for a in range(3)]
+    X = xs[0][None, None, :]
+    Y = xs[1][None, :, None]
+    Z = xs[2][:, None, None]
+    shape = (n[2] + 2, n[1] + 2, n[0] + 2)
+    V = np.empty(shape + (5,))
+    V[...] = list(default_state)
+    ID = np.full(shape, default_id, dtype=np.int32)
+    diag = math.sqrt(sum((x1[a] - x0[a]) ** 2 for a in range(3)))
+    Phi = np.full(shape, diag)
+    for gm in geoms:
+        if gm.kind == abi.GEOM_PLANE:
+            nv = np.array(gm.normal[:])
+            nv = nv * (1.0 / math.sqrt(nv @ nv))
+            dist = (X - gm.point[0]) * nv[0] + (Y - gm.point[1]) * nv[1] + (Z - gm.point[2]) * nv[2]
+            inside = dist > 0
+        else:
+            a, b, e = X - gm.point[0], Y - gm.point[1], Z - gm.point[2]
+            dist = np.sqrt(a * a + b * b + e * e) - gm.radius
+            inside = dist < 0 if gm.side == 0 else dist > 0
+        dist = np.broadcast_to(dist, shape)
+        inside = np.broadcast_to(inside, shape)
+        V[inside] = list(gm.state)
+        ID[inside] = gm.materialid
+        if scheme.n_levelsets and gm.materialid == scheme.ls_materialid:
+            upd = np.abs(dist) < np.abs(Phi)
+            if gm.kind == abi.GEOM_PLANE:
+                Phi[upd] = -dist[upd]
+            else:
+                Phi[upd] = (dist if gm.side == 0 else -dist)[upd]
+    fill_ghosts(V, ID, Phi if scheme.n_levelsets else None, bg)
+    return bg, V, ID, (Phi if scheme.n_levelsets else None)

@@ -3,11 +3,13 @@
 // stream.  All compute runs in the HIP kernels of m2c_kernels.h /
 // m2c_multiphase.h; there is no CPU fallback.
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -946,6 +948,13 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
   if (int rc = ensure_slab(h)) return rc;
   if (h->tile < 0)
     if (int rc = autotune(h, nullptr)) return rc;
+  unsigned long long err0 = 0;
+  {
+    DevCounters c;
+    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    err0 = c.riemann_errors;
+  }
   const bool clamp = h->sch.max_time > 0;
   for (int q = 0; q < nsteps; q++) {
     if (clamp) {
@@ -961,7 +970,12 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
     if (int rc = step_kernels(h)) return rc;
     HIPCHK(h, hipGetLastError());
   }
-  return check_error(h);
+  if (int rc = check_error(h)) return rc;
+  // Riemann failures of this call (warnings, SpaceOperator.cpp:2031-2034)
+  DevCounters c;
+  HIPCHK(h, hipMemcpy(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost));
+  const unsigned long long nf = c.riemann_errors - err0;
+  return nf > (unsigned long long)INT_MAX ? INT_MAX : (int)nf;
 }
 
 int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out) {
@@ -1315,11 +1329,12 @@ int m2cx_download_ls_residual(m2cx_solver* h, double* host) {
 // captured once as a HIP graph and replayed.
 struct m2cx_ensemble {
   std::vector<m2cx_solver*> m;
+  std::vector<char> active;  // members still stepping (not yet at MaxTime)
   int device = 0;
   int ti = 0, nstages = 0;
   hipStream_t stream = nullptr;
   RiemMember* dmem[3] = {};  // member tables, one per RK stage
-  int nriem = 0;              // members with interface faces
+  int nriem = 0;              // members whose interface problems go to the batched launch
   int* doff = nullptr;
   double* slab = nullptr;
   int lanes = RIEM_LANES, pathcap = 512;  // integration-path scratch: one slot per lane
@@ -1330,6 +1345,18 @@ struct m2cx_ensemble {
   bool overlap = false;
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // the stage kernel of all split members that share a tile shape, one
+  // launch per stage and phase (k_stage_multi): member tables [stage][phase]
+  // and tile-count prefix sums
+  struct StageClass {
+    int v = 0;                 // tile variant (kTiles index)
+    std::vector<m2cx_solver*> members;
+    StageArgs* tab[3][2] = {};
+    int* toff = nullptr;
+    int ntiles = 0;
+  };
+  std::vector<StageClass> classes;
+  std::set<const m2cx_solver*> classed;  // members whose stage kernel runs in a class launch
   std::string err;
 };
 
@@ -1344,21 +1371,70 @@ static int efail(m2cx_ensemble* e, int code, const std::string& msg) {
     if (e_ != hipSuccess) return efail((e), M2CX_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-// one ensemble time step, enqueued on e->stream (no host synchronisation)
+// members whose interface Riemann problems the batched launch solves: the
+// exact solver is not used where MultiPhase.Flux = LocalLaxFriedrichs
+// (mp_riemann returns at once for those)
+static bool ens_riemann_member(const m2cx_solver* h) {
+  return h->multimat && h->sch.mp_flux != M2CX_MPFLUX_LLF;
+}
+
+extern "C++" {
+// k_stage_multi for one class, stage and phase (1: tiles without interface
+// faces, 2: the others) on stream `st`
+template <int BX, int BY, int BZ>
+static void launch_class_tile(const m2cx_ensemble::StageClass& c, const StageArgs* tab, int mode, hipStream_t st) {
+  const long per = ((long)c.ntiles + NXCD - 1) / NXCD;
+  const dim3 grid((unsigned)(per * NXCD));
+  const int nm = (int)c.members.size();
+  constexpr int nt = BX * BY * BZ;
+  switch (mode) {
+    case MODE_STAGE1: k_stage_multi<MODE_STAGE1, BX, BY, BZ><<<grid, nt, 0, st>>>(tab, c.toff, nm); break;
+    case MODE_RK2_FINAL: k_stage_multi<MODE_RK2_FINAL, BX, BY, BZ><<<grid, nt, 0, st>>>(tab, c.toff, nm); break;
+    case MODE_RK3_STAGE2: k_stage_multi<MODE_RK3_STAGE2, BX, BY, BZ><<<grid, nt, 0, st>>>(tab, c.toff, nm); break;
+    case MODE_RK3_FINAL: k_stage_multi<MODE_RK3_FINAL, BX, BY, BZ><<<grid, nt, 0, st>>>(tab, c.toff, nm); break;
+  }
+}
+
+static void launch_class(const m2cx_ensemble::StageClass& c, int q, int phase, int mode, hipStream_t st) {
+  const StageArgs* tab = c.tab[q][phase - 1];
+  switch (c.v) {
+    case 1: launch_class_tile<32, 4, 2>(c, tab, mode, st); break;
+    case 2: launch_class_tile<8, 8, 4>(c, tab, mode, st); break;
+    case 3: launch_class_tile<16, 8, 2>(c, tab, mode, st); break;
+    default: launch_class_tile<16, 4, 4>(c, tab, mode, st); break;
+  }
+}
+
+}  // extern "C++"
+
+static void free_classes(m2cx_ensemble* e) {
+  for (auto& c : e->classes) {
+    for (auto& row : c.tab)
+      for (auto* p : row) hipFree(p);
+    hipFree(c.toff);
+  }
+  e->classes.clear();
+}
+
+// one ensemble time step over the active members, enqueued on e->stream (no
+// host synchronisation)
 static void ensemble_step(m2cx_ensemble* e) {
-  for (m2cx_solver* h : e->m) {
+  std::vector<m2cx_solver*> act;
+  for (size_t i = 0; i < e->m.size(); i++)
+    if (e->active[i]) act.push_back(e->m[i]);
+  for (m2cx_solver* h : act) {
     k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 1);
     if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
   }
   StageSpec sp[3];
   stage_specs(e->ti, sp);
   // split: the member's stage launch is divided by tiles as in split_stage
-  auto split = [e](const m2cx_solver* h) { return e->overlap && h->multimat; };
+  auto split = [e](const m2cx_solver* h) { return e->overlap && ens_riemann_member(h); };
   if (e->overlap)
-    for (m2cx_solver* h : e->m)
+    for (m2cx_solver* h : act)
       if (split(h)) tile_flags(h);
   for (int q = 0; q < e->nstages; q++) {
-    for (m2cx_solver* h : e->m) {
+    for (m2cx_solver* h : act) {
       recon_stage(h, sp[q].vin);
       if (split(h) && sp[q].resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
       guess_stage(h, sp[q].vin);
@@ -1371,16 +1447,22 @@ static void ensemble_step(m2cx_ensemble* e) {
       k_riem_count<<<e->nriem, 256, 0, e->stream>>>(e->dmem[q]);
     }
     if (e->overlap) {
+      // the stage tiles of all members: one launch per tile shape, tiles
+      // without interface faces beside the Riemann launch, the others after it
       hipStreamWaitEvent(e->stream2, e->ev_fork, 0);
-      for (m2cx_solver* h : e->m)
-        if (split(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 1);
+      for (const auto& c : e->classes) launch_class(c, q, 1, sp[q].mode, e->stream2);
+      for (m2cx_solver* h : act)
+        if (split(h) && !e->classed.count(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 1);
       hipEventRecord(e->ev_join, e->stream2);
-      for (m2cx_solver* h : e->m)
-        if (split(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 2);
+      for (const auto& c : e->classes) launch_class(c, q, 2, sp[q].mode, e->stream);
+      for (m2cx_solver* h : act)
+        if (split(h) && !e->classed.count(h)) launch_stage(h, sp[q].vin, sp[q].vout, sp[q].mode, 2);
       hipStreamWaitEvent(e->stream, e->ev_join, 0);
     }
-    for (m2cx_solver* h : e->m) {
+    for (m2cx_solver* h : act) {
       if (!split(h)) {
+        // members outside the batched launch solve nothing here (single
+        // material, or interface LLF fluxes computed in the stage kernel)
         stage_after_riemann(h, sp[q]);
         continue;
       }
@@ -1391,9 +1473,109 @@ static void ensemble_step(m2cx_ensemble* e) {
       if (sp[q].ls_which >= 0) levelset_stage(h, sp[q].vin, sp[q].ls_which);
     }
   }
-  for (m2cx_solver* h : e->m)
+  for (m2cx_solver* h : act)
     if (h->multimat && h->sch.n_levelsets > 0)
       mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
+}
+
+// The members borrow the ensemble's streams while a step is captured; this
+// gives them their own back on every path out of the capture.
+struct EnsembleStreamSwap {
+  m2cx_ensemble* e;
+  std::vector<hipStream_t> s2;
+  explicit EnsembleStreamSwap(m2cx_ensemble* e_) : e(e_), s2(e_->m.size()) {
+    for (size_t i = 0; i < e->m.size(); i++) {
+      e->m[i]->stream = e->stream;
+      s2[i] = e->m[i]->stream2;
+      e->m[i]->stream2 = e->stream2;
+    }
+  }
+  ~EnsembleStreamSwap() {
+    for (size_t i = 0; i < e->m.size(); i++) {
+      e->m[i]->stream = e->m[i]->stream_saved;
+      e->m[i]->stream2 = s2[i];
+    }
+  }
+};
+
+// (Re)captures one time step of the active members as a HIP graph, with the
+// batched Riemann tables over the active members that use the exact solver.
+static int ensemble_build(m2cx_ensemble* e) {
+  // a replay still in flight reads the member tables rewritten below
+  EHIPCHK(e, hipStreamSynchronize(e->stream));
+  if (e->exec) { hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  StageSpec sp[3];
+  stage_specs(e->ti, sp);
+  std::vector<RiemMember> tab[3];
+  for (size_t i = 0; i < e->m.size(); i++) {
+    m2cx_solver* h = e->m[i];
+    if (!e->active[i] || !ens_riemann_member(h)) continue;
+    const bool learned = h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready && h->mlp.guess;
+    for (int q = 0; q < e->nstages; q++)
+      tab[q].push_back(RiemMember{h->g, h->s, h->mp.rc, cf5(h, h->V[sp[q].vin]), h->ID, h->mp.L,
+                                  learned ? h->mlp.guess : nullptr, h->cnt});
+  }
+  const int nriem = (int)tab[0].size();
+  if (nriem > e->nriem) return efail(e, M2CX_E_ARG, "ensemble member table grew");
+  e->nriem = nriem;
+  if (nriem) {
+    for (int q = 0; q < e->nstages; q++)
+      EHIPCHK(e, hipMemcpy(e->dmem[q], tab[q].data(), tab[q].size() * sizeof(RiemMember), hipMemcpyHostToDevice));
+  }
+  // stage classes: active members of the batched Riemann launch whose stage
+  // kernel is the primitive-variable one, grouped by tile shape
+  free_classes(e);
+  e->classed.clear();
+  if (e->overlap) {
+    std::map<int, m2cx_ensemble::StageClass> bytile;
+    for (size_t i = 0; i < e->m.size(); i++) {
+      m2cx_solver* h = e->m[i];
+      if (!e->active[i] || !ens_riemann_member(h)) continue;
+      if (h->s.var_type != M2CX_VAR_PRIMITIVE && h->s.rec_type == M2CX_REC_LINEAR) continue;
+      auto& c = bytile[h->tile < 0 ? 0 : h->tile];
+      c.v = h->tile < 0 ? 0 : h->tile;
+      c.members.push_back(h);
+    }
+    for (auto& kv : bytile) {
+      m2cx_ensemble::StageClass c = kv.second;
+      const TileVariant& tv = kTiles[c.v];
+      std::vector<int> toff(c.members.size() + 1, 0);
+      for (size_t k = 0; k < c.members.size(); k++) {
+        const DevGrid& g = c.members[k]->g;
+        toff[k + 1] = toff[k] + ((g.NX + tv.bx - 1) / tv.bx) * ((g.NY + tv.by - 1) / tv.by) *
+                                    ((g.NZ + tv.bz - 1) / tv.bz);
+      }
+      c.ntiles = toff.back();
+      e->classes.push_back(c);
+      m2cx_ensemble::StageClass& cc = e->classes.back();
+      EHIPCHK(e, hipMalloc(&cc.toff, toff.size() * sizeof(int)));
+      EHIPCHK(e, hipMemcpy(cc.toff, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice));
+      for (int q = 0; q < e->nstages; q++)
+        for (int phase = 1; phase <= 2; phase++) {
+          std::vector<StageArgs> tab;
+          for (m2cx_solver* h : cc.members) {
+            StageArgs a = stage_args(h, sp[q].vin, sp[q].vout, sp[q].mode);
+            a.phase = phase;
+            a.tflag = h->tflag;
+            tab.push_back(a);
+          }
+          EHIPCHK(e, hipMalloc(&cc.tab[q][phase - 1], tab.size() * sizeof(StageArgs)));
+          EHIPCHK(e, hipMemcpy(cc.tab[q][phase - 1], tab.data(), tab.size() * sizeof(StageArgs),
+                               hipMemcpyHostToDevice));
+        }
+      for (m2cx_solver* h : cc.members) e->classed.insert(h);
+    }
+  }
+  {
+    EnsembleStreamSwap swap(e);
+    EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    ensemble_step(e);
+    hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+    if (ce != hipSuccess) return efail(e, M2CX_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+  }
+  EHIPCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  return 0;
 }
 
 int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out) {
@@ -1401,15 +1583,20 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
   auto* e = new m2cx_ensemble;
   *out = e;
   e->m.assign(members, members + n);
+  e->active.assign(n, 1);
   e->device = members[0]->device;
   e->ti = members[0]->sch.integrator;
   for (m2cx_solver* h : e->m) {
+    if (!h) return efail(e, M2CX_E_ARG, "null member");
     if (h->device != e->device) return efail(e, M2CX_E_ARG, "ensemble members must share one device");
     if (h->sch.integrator != e->ti) return efail(e, M2CX_E_ARG, "ensemble members must share the integrator");
     if (reinit_enabled(h))
       return efail(e, M2CX_E_UNSUPPORTED, "level-set reinitialization needs host control between steps; "
                                           "step such members alone");
   }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++)
+      if (e->m[i] == e->m[j]) return efail(e, M2CX_E_ARG, "a solver appears twice in the ensemble");
   // forward Euler exchanges the state buffers every step, which a replayed
   // graph cannot follow
   if (e->ti == M2CX_TI_FORWARD_EULER) return efail(e, M2CX_E_UNSUPPORTED, "ensembles run RK2 or RK3");
@@ -1417,54 +1604,35 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
   e->nstages = stage_specs(e->ti, sp);
   EHIPCHK(e, hipSetDevice(e->device));
   EHIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-  std::vector<RiemMember> tab[3];
   for (m2cx_solver* h : e->m) {
     h->profile = false;
     h->stream_saved = h->stream;
-    if (!h->multimat) continue;
-    const bool learned = h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready && h->mlp.guess;
-    for (int q = 0; q < e->nstages; q++)
-      tab[q].push_back(RiemMember{h->g, h->s, h->mp.rc, cf5(h, h->V[sp[q].vin]), h->ID, h->mp.L,
-                                  learned ? h->mlp.guess : nullptr, h->cnt});
+    if (int rc = ensure_slab(h)) return efail(e, rc, h->err);
+    if (ens_riemann_member(h)) e->nriem++;
   }
-  e->nriem = (int)tab[0].size();
   if (e->nriem) {
-    for (int q = 0; q < e->nstages; q++) {
-      EHIPCHK(e, hipMalloc(&e->dmem[q], tab[q].size() * sizeof(RiemMember)));
-      EHIPCHK(e, hipMemcpy(e->dmem[q], tab[q].data(), tab[q].size() * sizeof(RiemMember), hipMemcpyHostToDevice));
-    }
+    for (int q = 0; q < e->nstages; q++) EHIPCHK(e, hipMalloc(&e->dmem[q], e->nriem * sizeof(RiemMember)));
     EHIPCHK(e, hipMalloc(&e->doff, (e->nriem + 1) * sizeof(int)));
     EHIPCHK(e, hipMalloc(&e->slab, (long)e->lanes * 2 * e->pathcap * 3 * 8));
   }
   for (m2cx_solver* h : e->m)
     if (h->tile < 0 && autotune(h, nullptr)) return efail(e, M2CX_E_HIP, "autotune: " + h->err);
-  // overlap when every multi-material member has it on (the M2C_NO_OVERLAP
-  // environment variable or m2cx_set_overlap turns it off) and there is one
+  // overlap when every member that uses the exact solver has it on (the
+  // M2C_NO_OVERLAP environment variable or m2cx_set_overlap turns it off)
   e->overlap = e->nriem > 0;
   for (m2cx_solver* h : e->m)
-    if (h->multimat && !h->overlap) e->overlap = false;
+    if (ens_riemann_member(h) && !h->overlap) e->overlap = false;
   if (e->overlap) {
     EHIPCHK(e, hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
     EHIPCHK(e, hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     EHIPCHK(e, hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
   }
-  // every member works on the ensemble streams while it is captured and run
-  std::vector<hipStream_t> s2(e->m.size());
-  for (size_t i = 0; i < e->m.size(); i++) {
-    e->m[i]->stream = e->stream;
-    s2[i] = e->m[i]->stream2;
-    e->m[i]->stream2 = e->stream2;
-  }
-  EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  ensemble_step(e);
-  hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
-  for (size_t i = 0; i < e->m.size(); i++) {
-    e->m[i]->stream = e->m[i]->stream_saved;
-    e->m[i]->stream2 = s2[i];
-  }
-  if (ce != hipSuccess) return efail(e, M2CX_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-  EHIPCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
-  return 0;
+  return ensemble_build(e);
+}
+
+// Main.cpp's loop condition t < MaxTime - 1e-6 dt, as m2cx_run_steps applies it
+static bool member_running(const DevCounters& c, double max_time) {
+  return !(max_time > 0) || c.t < c.tmax - 1.0e-6 * c.dts;
 }
 
 int m2cx_ensemble_run_steps(m2cx_ensemble* e, int nsteps) {
@@ -1472,14 +1640,49 @@ int m2cx_ensemble_run_steps(m2cx_ensemble* e, int nsteps) {
   EHIPCHK(e, hipSetDevice(e->device));
   // members' own streams may still hold work (uploads)
   for (m2cx_solver* h : e->m) EHIPCHK(e, hipStreamSynchronize(h->stream));
-  for (int q = 0; q < nsteps; q++) EHIPCHK(e, hipGraphLaunch(e->exec, e->stream));
+  const size_t nm = e->m.size();
+  std::vector<DevCounters> c(nm);
+  for (size_t i = 0; i < nm; i++) EHIPCHK(e, hipMemcpy(&c[i], e->m[i]->cnt, sizeof(DevCounters), hipMemcpyDeviceToHost));
+  unsigned long long err0 = 0;
+  for (size_t i = 0; i < nm; i++) err0 += c[i].riemann_errors;
+  bool clamp = false;
+  for (m2cx_solver* h : e->m) clamp = clamp || h->sch.max_time > 0;
+  for (int q = 0; q < nsteps; q++) {
+    if (clamp) {
+      // each member stops at its MaxTime exactly as m2cx_run_steps stops: a
+      // finished member leaves the captured step, which is captured anew.
+      // The counters are read once the previous step has finished (the
+      // copies below do not wait for the ensemble's non-blocking stream).
+      if (q > 0) EHIPCHK(e, hipStreamSynchronize(e->stream));
+      bool changed = false, any = false;
+      for (size_t i = 0; i < nm; i++) {
+        if (!e->active[i]) continue;
+        if (q > 0) EHIPCHK(e, hipMemcpy(&c[i], e->m[i]->cnt, sizeof(DevCounters), hipMemcpyDeviceToHost));
+        if (!member_running(c[i], e->m[i]->sch.max_time)) {
+          e->active[i] = 0;
+          changed = true;
+        } else {
+          any = true;
+        }
+      }
+      if (!any) break;
+      if (changed)
+        if (int rc = ensemble_build(e)) return rc;
+    }
+    EHIPCHK(e, hipGraphLaunch(e->exec, e->stream));
+  }
   EHIPCHK(e, hipStreamSynchronize(e->stream));
-  int riem_fail = 0;
-  for (size_t i = 0; i < e->m.size(); i++) {
+  unsigned long long err1 = 0;
+  for (size_t i = 0; i < nm; i++) {
     int rc = check_error(e->m[i]);
     if (rc) return efail(e, rc, "member " + std::to_string(i) + ": " + e->m[i]->err);
+    DevCounters ci;
+    EHIPCHK(e, hipMemcpy(&ci, e->m[i]->cnt, sizeof ci, hipMemcpyDeviceToHost));
+    err1 += ci.riemann_errors;
   }
-  return riem_fail;
+  // Riemann failures of this call, as m2cx_run_steps reports them
+  const unsigned long long nf = err1 - err0;
+  return nf > (unsigned long long)INT_MAX ? INT_MAX : (int)nf;
 }
 
 const char* m2cx_ensemble_last_error(const m2cx_ensemble* e) { return e ? e->err.c_str() : "null handle"; }
@@ -1493,6 +1696,7 @@ int m2cx_ensemble_destroy(m2cx_ensemble* e) {
   for (auto* p : e->dmem) hipFree(p);
   hipFree(e->doff);
   hipFree(e->slab);
+  free_classes(e);
   if (e->ev_fork) hipEventDestroy(e->ev_fork);
   if (e->ev_join) hipEventDestroy(e->ev_join);
   if (e->stream2) hipStreamDestroy(e->stream2);

@@ -139,10 +139,11 @@ struct TileShape {
 constexpr int MAX_MAT_SETS = 32;
 __constant__ DevMats c_mats[MAX_MAT_SETS];
 
+// One tile of the fused stage (the body of k_stage and k_stage_multi).
 // GEN: reconstruction in any variable type (false: primitive variables only,
 // the common case, compiled without the characteristic code)
 template <int MODE, int BX, int BY, int BZ, bool GEN>
-__global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
+__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile) {
   using TS = TileShape<BX, BY, BZ>;
   constexpr int TX = BX, TY = BY, TZ = BZ;
   __shared__ double sF[5][TS::NFACE];
@@ -152,15 +153,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   const DevMats& M = c_mats[a.s.mset];
   const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
   const int t = threadIdx.x;
-  // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
-  // each with its own L2; work-group w runs tile (w % 8) * per + w / 8, so
-  // every XCD sweeps one contiguous run of tiles (x fastest) and the halos
-  // its neighbouring tiles share stay in its L2.
-  const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY, ntz = (g.NZ + TZ - 1) / TZ;
-  const int ntiles = ntx * nty * ntz;
-  const int per = (ntiles + NXCD - 1) / NXCD;
-  const int tile = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
-  if (tile >= ntiles) return;
+  const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
   if (a.phase != 0 && ((a.tflag[tile] != 0) != (a.phase == 2))) return;
   const int ib = (tile % ntx) * TX, jb = ((tile / ntx) % nty) * TY, kb = (tile / (ntx * nty)) * TZ;
   const int i = ib + tx, j = jb + ty, k = kb + tz;
@@ -300,6 +293,44 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
   if (MODE == MODE_STAGE1 || MODE == MODE_RK3_STAGE2) store5(a.U1, c, u);
   store5(a.Vout, c, v);
   write_ghosts(g, a.Vout, i, j, k, v);
+}
+
+// XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
+// each with its own L2; work-group w runs tile (w % 8) * per + w / 8, so
+// every XCD sweeps one contiguous run of tiles (x fastest) and the halos its
+// neighbouring tiles share stay in its L2.
+__device__ __forceinline__ int xcd_tile(int total) {
+  const int per = (total + NXCD - 1) / NXCD;
+  return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+}
+
+template <int MODE, int BX, int BY, int BZ, bool GEN>
+__global__ void __launch_bounds__(BX * BY * BZ) k_stage(StageArgs a) {
+  const DevGrid& g = a.g;
+  const int ntiles = ((g.NX + BX - 1) / BX) * ((g.NY + BY - 1) / BY) * ((g.NZ + BZ - 1) / BZ);
+  const int tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;
+  stage_tile<MODE, BX, BY, BZ, GEN>(a, tile);
+}
+
+// Ensemble form: the tiles of many solvers (one table entry each, the same
+// stage mode) in one launch.  toff holds the prefix sums of the members'
+// tile counts; the concatenated tile list is dealt to the XCDs as above, so
+// each XCD sweeps a contiguous run of it.  Every tile runs the code it runs
+// in k_stage on the same inputs, so each member's result is the same.
+template <int MODE, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) k_stage_multi(const StageArgs* __restrict__ tab,
+                                                               const int* __restrict__ toff, int nm) {
+  const int total = toff[nm];
+  const int t = xcd_tile(total);
+  if (t >= total) return;
+  int lo = 0, hi = nm - 1;  // member m with toff[m] <= t < toff[m+1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  stage_tile<MODE, BX, BY, BZ, false>(tab[lo], t - toff[lo]);
 }
 
 // Cells whose reconstruction in the scheme's variable type fails CheckState

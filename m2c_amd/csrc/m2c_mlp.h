@@ -149,7 +149,8 @@ __device__ __forceinline__ void mlp_forward_tile(const MLPDev& W, const __bf16* 
 
 // ---------------------------------------------------------------------------
 // Guesses for the interface list of one stage: one wave per 16 faces.  The
-// face states are the ones k_riemann will see (same reconstruction).
+// face states and the normal are the ones k_riemann will see (same
+// reconstruction, same Riemann normal).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_mlp_guess(DevGrid g, Scheme s, CF5 V, const unsigned char* __restrict__ ID,
                                                   IfaceList L, MLPDev W, float* guess, const DevCounters* cnt) {
@@ -172,7 +173,10 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevGrid g, Scheme s, CF5 V, co
         const long st = axis_stride(g, d);
         double Vm[5], Vp[5];
         if_states(g, s, c_mats[s.mset], V, ID, d, c, Vm, Vp);
-        const double dir[3] = {(d == 0) ? 1.0 : 0.0, (d == 1) ? 1.0 : 0.0, (d == 2) ? 1.0 : 0.0};
+        // the direction the exact solver will solve along (Mesh, LevelSet
+        // or Average normal), so the guess is for the same problem
+        double dir[3];
+        bimat_normal(g, s.rnormal, L.phi, d, c, ID[c - st], dir);
         float x[8];
         ok = mlp_features(c_mats[s.mset], dir, Vm, ID[c - st], Vp, ID[c], x, P, pc);
         if (ok)

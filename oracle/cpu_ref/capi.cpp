@@ -180,14 +180,17 @@ int orc_flux_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, 
   return 0;
 }
 
-int orc_riemann_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, const double* dir,
-                      const double* Vm, const int* idm, const double* Vp, const int* idp, double* Vs, int* id,
-                      double* Vsm, double* Vsp, int* err, int* iters, int* fevals, int* flags) {
+int orc_riemann_batch_guess(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, const double* dir,
+                            const double* Vm, const int* idm, const double* Vp, const int* idp, double* Vs, int* id,
+                            double* Vsm, double* Vsp, int* err, int* iters, int* fevals, int* flags,
+                            const double* guess) {
   std::vector<Eos> vf;
   for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
   ExactRiemann rs(vf, RiemannParams::from(*sch));
   for (int q = 0; q < n; q++) {
     rs.st = RiemannStats();
+    // a learned starting pressure (finite and positive) or none
+    rs.p0_guess = (guess && std::isfinite(guess[q]) && guess[q] > 0) ? guess[q] : -1.0;
     int mid = -1;
     err[q] = rs.Solve(dir + 3 * q, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], Vs + 5 * q, mid, Vsm + 5 * q,
                       Vsp + 5 * q);
@@ -202,6 +205,13 @@ int orc_riemann_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sc
     if (flags) flags[q] = (rs.st.trivial ? 1 : 0) | (rs.st.fallback ? 2 : 0) | (rs.st.fatal ? 4 : 0);
   }
   return 0;
+}
+
+int orc_riemann_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int n, const double* dir,
+                      const double* Vm, const int* idm, const double* Vp, const int* idp, double* Vs, int* id,
+                      double* Vsm, double* Vsp, int* err, int* iters, int* fevals, int* flags) {
+  return orc_riemann_batch_guess(mats, nmat, sch, n, dir, Vm, idm, Vp, idp, Vs, id, Vsm, Vsp, err, iters, fevals,
+                                 flags, nullptr);
 }
 
 // One Riemann problem in the PRINT_RIEMANN_SOLUTION build: fills `rows`

@@ -384,6 +384,11 @@ int ExactRiemann::Acoustic(double rhol, double ul, double pl, double cl, int idl
   double Cl = rhol * cl;
   double Cr = rhor * cr;
   p0 = (Cr * pl + Cl * pr + Cl * Cr * (ul - ur)) / (Cl + Cr);
+  if (p0_guess > 0 && !nonadaptive) {
+    const double sl = std::fabs(p0 - pl) / (Cl * cl), sr = std::fabs(p0 - pr) / (Cr * cr);
+    const double sw = std::max(sl, sr);
+    if (!(p0 > 0) || (sw >= 0.3 && sw < 10.0)) p0 = p0_guess;
+  }
   if (!Wave(1, path1, rhol, ul, pl, p0, idl, rhol, (p0 > pl) ? rhol * 1.1 : rhol * 0.9, rhol0, ul0)) return found;
   if (!Wave(3, path3, rhor, ur, pr, p0, idr, rhor, (p0 > pr) ? rhor * 1.1 : rhor * 0.9, rhor0, ur0)) return found;
   found = 1;

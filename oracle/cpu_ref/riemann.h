@@ -54,6 +54,12 @@ class ExactRiemann {
   RiemannParams P;
   bool nonadaptive = false;   // ExactRiemannSolverNonAdaptive behaviour
   bool print_mode = false;
+  // A starting pressure that replaces the acoustic one in the adaptive pass
+  // where the acoustic pressure is not positive or the acoustic wave
+  // strength lies in [0.3, 10) -- the library's learned-guess rule, so that
+  // tests can run the learned path on the CPU.  No counterpart in the
+  // reference; <= 0: off.
+  double p0_guess = -1.0;
   std::vector<std::array<double, 5>> sol1d;  // (xi, rho, u, p, id) in print mode
   RiemannStats st;
 

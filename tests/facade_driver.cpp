@@ -1,0 +1,108 @@
+// Test driver: the reference's time loop (Main.cpp:653-712) written against
+// the C++ façades of include/m2c_amd_facade.hpp, compiled as a shared library
+// so that a test can hand it a deck's initial state and read back the result.
+// It fills and reads the variables through GetDataPointer() with the
+// reference's double*** indexing, so the façade's layout is part of what the
+// test checks.
+#include <cstring>
+#include <cstdio>
+#include <vector>
+
+#include "m2c_amd_facade.hpp"
+
+using m2c_amd::Context;
+using m2c_amd::LevelSetOperator;
+using m2c_amd::SpaceOperator;
+using m2c_amd::SpaceVariable3D;
+using m2c_amd::TimeIntegratorBase;
+using m2c_amd::TimeIntegratorFE;
+using m2c_amd::TimeIntegratorRK2;
+using m2c_amd::TimeIntegratorRK3;
+
+static void put(SpaceVariable3D& X, const double* src, int dof) {
+  double*** x = X.GetDataPointer();
+  int i0, j0, k0, imax, jmax, kmax;
+  X.GetGhostedCornerIndices(&i0, &j0, &k0, &imax, &jmax, &kmax);
+  long q = 0;
+  for (int k = k0; k < kmax; k++)
+    for (int j = j0; j < jmax; j++)
+      for (int i = i0; i < imax; i++)
+        for (int p = 0; p < dof; p++) x[k][j][i * dof + p] = src[q++];
+  X.RestoreDataPointerAndInsert();
+}
+
+static void get(SpaceVariable3D& X, double* dst, int dof) {
+  double*** x = X.GetDataPointer();
+  int i0, j0, k0, imax, jmax, kmax;
+  X.GetGhostedCornerIndices(&i0, &j0, &k0, &imax, &jmax, &kmax);
+  long q = 0;
+  for (int k = k0; k < kmax; k++)
+    for (int j = j0; j < jmax; j++)
+      for (int i = i0; i < imax; i++)
+        for (int p = 0; p < dof; p++) dst[q++] = x[k][j][i * dof + p];
+  X.RestoreDataPointerToLocalVector();
+}
+
+extern "C" {
+
+// Runs `nsteps` steps of the deck (or until MaxTime) from the given ghosted
+// state; returns the number of steps taken, or a negative error code.
+// Outputs: V (5 dof), ID (as doubles, the reference's storage), Phi, the
+// residual of the initial state (5 dof) and the final time.
+int facade_run(const m2cx_grid* grid, const m2cx_material* mats, int nmat, const m2cx_scheme* scheme,
+               const double* V0, const int* ID0, const double* Phi0, int nsteps, double* Vout, double* IDout,
+               double* Phiout, double* R0out, double* tout) {
+  try {
+    auto ctx = std::make_shared<Context>(*grid, mats, nmat, *scheme, 0);
+    SpaceOperator spo(ctx);
+    LevelSetOperator ls0(ctx);
+    std::vector<LevelSetOperator*> lso;
+    if (scheme->n_levelsets > 0) lso.push_back(&ls0);
+    SpaceVariable3D V(*grid, 5), ID(*grid, 1), Phi0v(*grid, 1), R(*grid, 5);
+    const long n = (long)(grid->nx + 2) * (grid->ny + 2) * (grid->nz + 2);
+    put(V, V0, 5);
+    std::vector<double> idd(n);
+    for (long q = 0; q < n; q++) idd[q] = ID0[q];
+    put(ID, idd.data(), 1);
+    std::vector<SpaceVariable3D*> Phi, NPhi, KappaPhi;
+    if (scheme->n_levelsets > 0) {
+      put(Phi0v, Phi0, 1);
+      Phi.push_back(&Phi0v);
+    }
+    // the residual of the initial state (SpaceOperator::ComputeResidual)
+    spo.ComputeResidual(V, ID, R, 0.0, nullptr, nullptr, scheme->n_levelsets > 0 ? &Phi : nullptr);
+    get(R, R0out, 5);
+    TimeIntegratorFE fe(spo, lso);
+    TimeIntegratorRK2 rk2(spo, lso);
+    TimeIntegratorRK3 rk3(spo, lso);
+    TimeIntegratorBase* integrator = scheme->integrator == M2CX_TI_FORWARD_EULER
+                                         ? (TimeIntegratorBase*)&fe
+                                         : (scheme->integrator == M2CX_TI_RK2 ? (TimeIntegratorBase*)&rk2 : &rk3);
+    // Main.cpp:653-712 without subcycling
+    const double tmax = scheme->max_time > 0 ? scheme->max_time : 1e300;
+    double t = 0.0, dt = 0.0, cfl = 0.0, dts = 0.0;
+    int time_step = 0;
+    while (t < tmax - 1.0e-6 * dts && time_step < nsteps) {
+      time_step++;
+      spo.ComputeTimeStepSize(V, ID, dt, cfl);
+      if (t + dt > tmax) {
+        cfl *= (tmax - t) / dt;
+        dt = tmax - t;
+      }
+      dts = dt;
+      t += dt;
+      integrator->AdvanceOneTimeStep(V, ID, Phi, NPhi, KappaPhi, nullptr, nullptr, nullptr, nullptr, t, dt,
+                                     time_step, 0, dts);
+    }
+    get(V, Vout, 5);
+    get(ID, IDout, 1);
+    if (scheme->n_levelsets > 0) get(Phi0v, Phiout, 1);
+    *tout = t;
+    return time_step;
+  } catch (const m2c_amd::Error& e) {
+    std::fprintf(stderr, "facade_run: %s\n", e.what());
+    return e.code() < 0 ? e.code() : -1;
+  }
+}
+
+}  // extern "C"

@@ -50,6 +50,8 @@ def lib():
                                      D, D, I, D]
         L.orc_riemann_batch.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, D, D, I,
                                         D, I, D, I, D, D, I, I, I, I]
+        L.orc_riemann_batch_guess.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, D, D,
+                                              I, D, I, D, I, D, D, I, I, I, I, D]
         L.orc_riemann_print.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), D, C.c_int, D,
                                         C.c_int, D, C.c_int, D, C.c_int]
         L.orc_wave.argtypes = [C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), C.c_int, C.c_double,
@@ -203,15 +205,19 @@ def flux_batch(mats, nmat, scheme, d, Vm, Vp, ids):
     return F
 
 
-def riemann_batch(mats, nmat, scheme, dirs, Vm, idm, Vp, idp):
+def riemann_batch(mats, nmat, scheme, dirs, Vm, idm, Vp, idp, guess=None):
+    """The exact solver on n problems; `guess` (n starting pressures, <= 0 or
+    NaN for none) runs the learned-guess path with those predictions."""
     n = len(Vm)
     a = [np.ascontiguousarray(x, dtype=np.float64) for x in (dirs, Vm, Vp)]
     idm = np.ascontiguousarray(idm, dtype=np.int32)
     idp = np.ascontiguousarray(idp, dtype=np.int32)
     Vs, Vsm, Vsp = np.zeros((n, 5)), np.zeros((n, 5)), np.zeros((n, 5))
     mid, err, it, fe, fl = (np.zeros(n, dtype=np.int32) for _ in range(5))
-    lib().orc_riemann_batch(mats, nmat, C.byref(scheme), n, dp(a[0]), dp(a[1]), ip(idm), dp(a[2]), ip(idp),
-                            dp(Vs), ip(mid), dp(Vsm), dp(Vsp), ip(err), ip(it), ip(fe), ip(fl))
+    g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+    lib().orc_riemann_batch_guess(mats, nmat, C.byref(scheme), n, dp(a[0]), dp(a[1]), ip(idm), dp(a[2]), ip(idp),
+                                  dp(Vs), ip(mid), dp(Vsm), dp(Vsp), ip(err), ip(it), ip(fe), ip(fl),
+                                  None if g is None else dp(g))
     return dict(Vs=Vs, id=mid, Vsm=Vsm, Vsp=Vsp, err=err, iters=it, fevals=fe, flags=fl)
 
 

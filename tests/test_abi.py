@@ -94,3 +94,28 @@ def test_ctypes_mirror_matches_header(tmp_path):
             assert got[f"{cname}.{f}"] == getattr(py, f).offset, (cname, f)
     for n in _consts():
         assert got[f"const {n}"] == getattr(abi, n), n
+
+
+def test_facades_compile_and_link():
+    """The C++ façade header stands alone, its library exports the
+    reference-signature methods, and the test driver built on it loads."""
+    import ctypes
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = "#include \"m2c_amd_facade.hpp\"\nint main() { return 0; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-I" + os.path.join(root, "include"), "-"],
+                       input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+    lib = os.path.join(root, "m2c_amd", "libm2c_amd_facade.so")
+    syms = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    for sig in ("m2c_amd::SpaceOperator::ComputeResidual(m2c_amd::SpaceVariable3D&, m2c_amd::SpaceVariable3D&, "
+                "m2c_amd::SpaceVariable3D&, double, RiemannSolutions*",
+                "m2c_amd::SpaceOperator::ComputeTimeStepSize(m2c_amd::SpaceVariable3D&, m2c_amd::SpaceVariable3D&, "
+                "double&, double&, m2c_amd::SpaceVariable3D*)",
+                "m2c_amd::TimeIntegratorBase::AdvanceOneTimeStep(",
+                "m2c_amd::SpaceVariable3D::GetDataPointer()",
+                "m2c_amd::SpaceVariable3D::RestoreDataPointerAndInsert()",
+                "m2c_amd::LevelSetOperator::ComputeResidual("):
+        assert sig in syms, sig
+    d = ctypes.CDLL(os.path.join(root, "tests", "libfacade_driver.so"))
+    assert hasattr(d, "facade_run")

@@ -11,7 +11,8 @@ dt are compared with log.out in the default suite; the full 326-step replay
 runs when M2C_LONG_FIXTURES=1.  Full replay here: every dt within 2.8e-7 of
 the printed values, line plot within 4e-9 (density), 8e-10 (u), 2.6e-7 (v,
 which is nearly zero on the line), 1.8e-9 (pressure), 1.7e-9 (phi) of the
-column maxima, material IDs equal."""
+column maxima, and 1.5e-10 on the material-ID column (the probe interpolates
+IDs between cell centres, so it is not integer next to the interface)."""
 import json
 import os
 
@@ -23,7 +24,7 @@ import oracle_lib as ol
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # column -> tolerance relative to max|column| (the files print 9 digits)
-SHYUE_TOL = {1: 2e-8, 2: 5e-9, 3: 1e-6, 5: 1e-8, 7: 0.0, 8: 1e-8}
+SHYUE_TOL = {1: 2e-8, 2: 5e-9, 3: 1e-6, 5: 1e-8, 7: 1e-9, 8: 1e-8}
 
 
 def fixture():
