@@ -74,6 +74,10 @@ struct m2cx_solver {
   // level-set reinitialization
   RBuf rb{};
   double reinit_cfl = 0.8;  // the reinitializer's CFL, lowered by its failsafe
+  // number of the step being taken when the host knows it without reading
+  // the device (run_steps counts from one read; advance_one_time_step is
+  // given it); -1 otherwise
+  int host_time_step = -1;
   long long reinit_calls = 0, reinit_iters = 0;
   hipStream_t stream_saved = nullptr;  // own stream while an ensemble borrows it
   int tile = -1;                        // stage tile variant (-1: not tuned yet)
@@ -846,10 +850,17 @@ static int end_of_step(m2cx_solver* h) {
   MPBuffers& B = h->mp;
   F5 V = f5(h, h->V[0]);
   mp_end_ids(B, h->g, h->s, h->ID, h->cnt, h->stream);
-  DevCounters c;
-  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (reinit_due(h, c.t, c.dts, c.time_step))
+  bool due;
+  if (h->sch.reinit_frequency_dt <= 0 && h->host_time_step >= 0) {
+    // Frequency mode: the step number alone decides, no device read
+    due = reinit_due(h, 0.0, 0.0, h->host_time_step);
+  } else {
+    DevCounters c;
+    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    due = reinit_due(h, c.t, c.dts, c.time_step);
+  }
+  if (due)
     if (int rc = reinit_full(h, 0)) return rc;
   mp_end_states(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
@@ -937,7 +948,10 @@ int m2cx_advance_one_time_step(m2cx_solver* h, double time, double dt, int time_
   c.t = time;
   c.time_step = time_step;
   HIPCHK(h, hipMemcpy(h->cnt, &c, sizeof c, hipMemcpyHostToDevice));
-  if (int rc = step_kernels(h)) return rc;
+  h->host_time_step = time_step;
+  const int rc = step_kernels(h);
+  h->host_time_step = -1;
+  if (rc) return rc;
   HIPCHK(h, hipGetLastError());
   return check_error(h);
 }
@@ -949,11 +963,13 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
   if (h->tile < 0)
     if (int rc = autotune(h, nullptr)) return rc;
   unsigned long long err0 = 0;
+  int ts = 0;
   {
     DevCounters c;
     HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     err0 = c.riemann_errors;
+    ts = c.time_step;
   }
   const bool clamp = h->sch.max_time > 0;
   for (int q = 0; q < nsteps; q++) {
@@ -967,7 +983,10 @@ int m2cx_run_steps(m2cx_solver* h, int nsteps) {
     tic(h, "dt", &e0);
     k_dt<<<h->dt_blocks, 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[0]), h->ID, h->dt_part, h->cnt, 1);
     toc(h, "dt", e0);
-    if (int rc = step_kernels(h)) return rc;
+    h->host_time_step = ++ts;  // k_dt advances the device's step count by one
+    const int rc = step_kernels(h);
+    h->host_time_step = -1;
+    if (rc) return rc;
     HIPCHK(h, hipGetLastError());
   }
   if (int rc = check_error(h)) return rc;
