@@ -339,6 +339,23 @@ int m2cx_riemann_batch(m2cx_solver* h, int n, const double* dir,
                        double* Vs, int32_t* id, double* Vsm, double* Vsp,
                        int32_t* err, int32_t* iters, int32_t* fevals);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The same
+ * solves as m2cx_riemann_batch, bit for bit, but every array is a DEVICE
+ * pointer (iters, fevals may be NULL), the work is enqueued on the solver's
+ * stream without waiting (m2cx_synchronize before reading the outputs), and
+ * nothing is allocated: the integration paths and staging live in one fixed
+ * device array per GPU (about 29 MB, held until the process ends), set up by
+ * m2cx_create.  At most M2CX_NOALLOC_MAX_PROBLEMS problems per call (more:
+ * M2CX_E_ARG with a message).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race. */
+#define M2CX_NOALLOC_MAX_PROBLEMS 65536
+int m2cx_riemann_batch_noalloc(m2cx_solver* h, int n, const double* dir,
+                               const double* Vm, const int32_t* idm,
+                               const double* Vp, const int32_t* idp,
+                               double* Vs, int32_t* id, double* Vsm, double* Vsp,
+                               int32_t* err, int32_t* iters, int32_t* fevals);
+
 /* ------------------------------------------------------------------ */
 /* Learned p* guess                                                    */
 /* ------------------------------------------------------------------ */

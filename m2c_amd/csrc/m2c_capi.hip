@@ -195,6 +195,51 @@ static int slope_k(int limiter, double A, double B) {
   return k;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch for m2cx_riemann_batch_noalloc, the one entry point
+// that allocates nothing: the integration-path slab of NOALLOC_LANES lanes
+// and per-problem staging (learned-guess features, prediction, scales,
+// starting pressure; iteration counts when the caller passes none).  One
+// copy per GPU, held until the process ends; every call on a GPU shares it,
+// so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_LANES = 1024;
+constexpr int NOALLOC_PATHCAP = 512;
+constexpr long NOALLOC_SLAB_BYTES = (long)NOALLOC_LANES * 2 * NOALLOC_PATHCAP * 3 * 8;
+constexpr long NOALLOC_MAXP = M2CX_NOALLOC_MAX_PROBLEMS;
+constexpr long NOALLOC_X = NOALLOC_SLAB_BYTES;              // float [max][8]
+constexpr long NOALLOC_Y = NOALLOC_X + NOALLOC_MAXP * 32;   // float [max]
+constexpr long NOALLOC_P = NOALLOC_Y + NOALLOC_MAXP * 4;    // double [max]
+constexpr long NOALLOC_PC = NOALLOC_P + NOALLOC_MAXP * 8;   // double [max]
+constexpr long NOALLOC_OK = NOALLOC_PC + NOALLOC_MAXP * 8;  // int [max]
+constexpr long NOALLOC_G = NOALLOC_OK + NOALLOC_MAXP * 4;   // float [max]
+constexpr long NOALLOC_IT = NOALLOC_G + NOALLOC_MAXP * 4;   // int [max]
+constexpr long NOALLOC_FE = NOALLOC_IT + NOALLOC_MAXP * 4;  // int [max]
+constexpr long NOALLOC_BYTES = NOALLOC_FE + NOALLOC_MAXP * 4;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char*> g_scratch;  // device -> address of `scratch` there
+
+// Looks up, once per device and with that device current, where `scratch`
+// lives on it.  Host code uses only this address, never the array's name.
+static int scratch_setup(m2cx_solver* h) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch.count(h->device)) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(h, M2CX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[h->device] = static_cast<unsigned char*>(ptr);
+  return 0;
+}
+
+static unsigned char* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  return it == g_scratch.end() ? nullptr : it->second;
+}
+
 extern "C" {
 
 const char* m2cx_last_error(const m2cx_solver* h) { return h ? h->err.c_str() : "null handle"; }
@@ -226,6 +271,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   h->device = device;
   HIPCHK(h, hipSetDevice(device));
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (int rc = scratch_setup(h)) return rc;
   const int NX = grid->nx, NY = grid->ny, NZ = grid->nz;
   DevGrid& g = h->g;
   g.NX = NX; g.NY = NY; g.NZ = NZ;
@@ -1051,6 +1097,47 @@ int m2cx_reset_counters(m2cx_solver* h) {
   c.unresolved_total = 0;
   h->reinit_calls = h->reinit_iters = 0;
   HIPCHK(h, hipMemcpy(h->cnt, &c, sizeof c, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// m2cx_riemann_batch on device arrays, with the fixed scratch above instead
+// of any allocation, enqueued on the solver's stream without waiting for it.
+int m2cx_riemann_batch_noalloc(m2cx_solver* h, int n, const double* dir, const double* Vm, const int32_t* idm,
+                               const double* Vp, const int32_t* idp, double* Vs, int32_t* id, double* Vsm,
+                               double* Vsp, int32_t* err, int32_t* iters, int32_t* fevals) {
+  if (!h) return M2CX_E_ARG;
+  if (n <= 0) return fail(h, M2CX_E_ARG, "m2cx_riemann_batch_noalloc: no problems");
+  if (n > M2CX_NOALLOC_MAX_PROBLEMS)
+    return fail(h, M2CX_E_ARG, "m2cx_riemann_batch_noalloc: " + std::to_string(n) + " problems, at most " +
+                                   std::to_string(M2CX_NOALLOC_MAX_PROBLEMS) + " per call");
+  if (!dir || !Vm || !idm || !Vp || !idp || !Vs || !id || !Vsm || !Vsp || !err)
+    return fail(h, M2CX_E_ARG, "m2cx_riemann_batch_noalloc: null array");
+  if (!h->multimat) return fail(h, M2CX_E_ARG, "Riemann batch needs a multi-material solver");
+  HIPCHK(h, hipSetDevice(h->device));
+  unsigned char* sc = scratch_ptr(h->device);
+  if (!sc) return fail(h, M2CX_E_HIP, "m2cx_riemann_batch_noalloc: scratch not set up on this device");
+  double* slab = reinterpret_cast<double*>(sc);
+  int* it = iters ? iters : reinterpret_cast<int*>(sc + NOALLOC_IT);
+  int* fe = fevals ? fevals : reinterpret_cast<int*>(sc + NOALLOC_FE);
+  float* dguess = nullptr;
+  if (h->sch.riem_guess == M2CX_GUESS_MLP && h->mlp.ready) {
+    float* X = reinterpret_cast<float*>(sc + NOALLOC_X);
+    float* y = reinterpret_cast<float*>(sc + NOALLOC_Y);
+    double* P = reinterpret_cast<double*>(sc + NOALLOC_P);
+    double* pc = reinterpret_cast<double*>(sc + NOALLOC_PC);
+    int* ok = reinterpret_cast<int*>(sc + NOALLOC_OK);
+    dguess = reinterpret_cast<float*>(sc + NOALLOC_G);
+    // every entry of X, y, P, pc, ok and the guesses is written before it is read
+    k_mlp_features<<<(n + 255) / 256, 256, 0, h->stream>>>(h->s.mset, n, dir, Vm, idm, Vp, idp, nullptr, X, P, pc,
+                                                           nullptr, ok);
+    k_mlp_predict<<<(n + 15) / 16, 64, 0, h->stream>>>(n, X, mlp_dev(h), y);
+    k_mlp_compose<<<(n + 255) / 256, 256, 0, h->stream>>>(n, y, P, pc, ok, dguess);
+  }
+  const MPBuffers& B = h->mp;
+  k_riemann_batch<<<(NOALLOC_LANES + 127) / 128, 128, 0, h->stream>>>(
+      h->s.mset, B.rc, n, dir, Vm, idm, Vp, idp, Vs, id, Vsm, Vsp, err, it, fe, slab, NOALLOC_LANES, NOALLOC_PATHCAP,
+      dguess, h->cnt);
+  HIPCHK(h, hipGetLastError());
   return 0;
 }
 

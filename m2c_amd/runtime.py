@@ -59,6 +59,8 @@ def lib():
         L.m2cx_get_riemann_histogram.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.m2cx_numerical_flux_batch.argtypes = [P, C.c_int, C.c_int, D, D, I32, D]
         L.m2cx_riemann_batch.argtypes = [P, C.c_int, D, D, I32, D, I32, D, I32, D, D, I32, I32, I32]
+        # device pointers (as integers) for the entry point that allocates nothing
+        L.m2cx_riemann_batch_noalloc.argtypes = [P, C.c_int] + [P] * 12
         L.m2cx_profile.argtypes = [P, C.c_int]
         L.m2cx_kernel_time.argtypes = [P, C.c_char_p, D, C.POINTER(C.c_int64)]
         L.m2cx_synchronize.argtypes = [P]
