@@ -44,8 +44,10 @@ enum { M2CX_BC_NONE = 0, M2CX_BC_INLET = 1, M2CX_BC_INLET2 = 2, M2CX_BC_SLIPWALL
 /* LevelSetSchemeData::BcType (IoData.h:814) */
 enum { M2CX_LSBC_ZERO_NEUMANN = 1, M2CX_LSBC_LINEAR_EXTRAPOLATION = 2, M2CX_LSBC_NON_NEGATIVE = 3 };
 
-/* SchemeData::Flux (IoData.h:710) */
-enum { M2CX_FLUX_ROE = 0, M2CX_FLUX_LLF = 1, M2CX_FLUX_HLLC = 2 };
+/* SchemeData::Flux (IoData.h:710).  GODUNOV: the physical flux of the exact
+ * Riemann solution at the face (FluxFcnGodunov.h:37-60), solved on the device
+ * for every face with one material on both sides, before each stage. */
+enum { M2CX_FLUX_ROE = 0, M2CX_FLUX_LLF = 1, M2CX_FLUX_HLLC = 2, M2CX_FLUX_GODUNOV = 3 };
 
 /* ReconstructionData (IoData.h:662-671) */
 enum { M2CX_REC_CONSTANT = 0, M2CX_REC_LINEAR = 1 };

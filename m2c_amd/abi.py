@@ -9,7 +9,7 @@ import ctypes as C
 EOS_SG, EOS_NASG, EOS_MG, EOS_DUMMY = 0, 1, 2, 8
 BC_NONE, BC_INLET, BC_INLET2, BC_SLIPWALL, BC_STICKWALL, BC_SYMMETRY, BC_OVERSET, BC_OUTLET = range(8)
 LSBC_ZERO_NEUMANN, LSBC_LINEAR_EXTRAPOLATION, LSBC_NON_NEGATIVE = 1, 2, 3
-FLUX_ROE, FLUX_LLF, FLUX_HLLC = 0, 1, 2
+FLUX_ROE, FLUX_LLF, FLUX_HLLC, FLUX_GODUNOV = 0, 1, 2, 3
 REC_CONSTANT, REC_LINEAR = 0, 1
 LIM_NONE, LIM_GMM, LIM_VANALBADA = 0, 1, 2
 SLOPE_ZERO, SLOPE_NONZERO = 0, 1

@@ -78,6 +78,11 @@ struct m2cx_solver {
   // the device (run_steps counts from one read; advance_one_time_step is
   // given it); -1 otherwise
   int host_time_step = -1;
+  // Flux = Godunov: face fluxes of the stage, the exact solver's path slots
+  double* gflux = nullptr;
+  double* gslab = nullptr;
+  int glanes = 0;
+  RiemannCfg grc{};
   long long reinit_calls = 0, reinit_iters = 0;
   hipStream_t stream_saved = nullptr;  // own stream while an ensemble borrows it
   int tile = -1;                        // stage tile variant (-1: not tuned yet)
@@ -363,6 +368,19 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   }
   HIPCHK(h, hipMemsetAsync(h->ID, 0, n, h->stream));
   HIPCHK(h, hipMemsetAsync(h->IDn, 0, n, h->stream));
+  if (sch->flux == M2CX_FLUX_GODUNOV) {
+    // 3 axes x 5 components per cell; lanes for the exact solves: one per
+    // face up to the interface solver's lane count
+    HIPCHK(h, hipMalloc(&h->gflux, 15 * n * 8));
+    HIPCHK(h, hipMemsetAsync(h->gflux, 0, 15 * n * 8, h->stream));
+    const long faces = (long)(NX + 1) * NY * NZ + (long)NX * (NY + 1) * NZ + (long)NX * NY * (NZ + 1);
+    h->glanes = (int)std::min<long>(RIEM_LANES, (faces + 63) / 64 * 64);
+    HIPCHK(h, hipMalloc(&h->gslab, (long)h->glanes * 2 * GODUNOV_PATHCAP * 3 * 8));
+    h->grc = RiemannCfg{sch->riem_maxits_main, sch->riem_maxits_bracket, sch->riem_maxits_shock,
+                        sch->riem_steps_rarefaction, sch->riem_tol_main, sch->riem_tol_shock,
+                        sch->riem_tol_rarefaction, sch->riem_min_pressure, sch->riem_failure_threshold,
+                        sch->riem_pressure_at_failure};
+  }
   HIPCHK(h, hipMalloc(&h->staging, n * 5 * 8));
   HIPCHK(h, hipMalloc(&h->staging_i, n * 4));
   h->dt_blocks = std::min(grid_blocks((long)NX * NY * NZ), DT_BLOCKS);
@@ -415,6 +433,7 @@ int m2cx_destroy(m2cx_solver* h) {
   for (int f = 0; f < 3; f++) hipFree(h->V[f][0]);
   hipFree(h->Un[0]); hipFree(h->U1[0]); hipFree(h->R[0]);
   hipFree(h->ID); hipFree(h->IDn); hipFree(h->rflag);
+  hipFree(h->gflux); hipFree(h->gslab);
   hipFree(h->geo); hipFree(h->coefK);
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
@@ -534,6 +553,7 @@ static StageArgs stage_args(m2cx_solver* h, int vin, int vout, int mode) {
   }
   a.mode = mode;
   a.cnt = h->cnt;
+  a.gflux = h->gflux;
   return a;
 }
 
@@ -679,10 +699,21 @@ static void guess_stage(m2cx_solver* h, int vin) {
 
 // Cells falling back to primitive reconstruction for this stage's input
 // (conservative / characteristic variable types only).
+// What every stage needs before its face fluxes: the cells reconstructed in
+// primitive variables instead of the scheme's (other variable types), and
+// with Flux = Godunov the exact-solver fluxes of the single-material faces.
 static void recon_stage(m2cx_solver* h, int vin) {
-  if (!h->rflag) return;
   const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
-  k_recon_flags<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[vin]), h->ID, h->rflag, h->cnt);
+  if (h->rflag)
+    k_recon_flags<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, h->s, cf5(h, h->V[vin]), h->ID, h->rflag, h->cnt);
+  if (h->gflux) {
+    hipEvent_t e0{};
+    tic(h, "godunov", &e0);
+    GodunovArgs ga{h->g, h->s, h->grc, cf5(h, h->V[vin]), h->ID, h->gflux, h->gslab, h->glanes, GODUNOV_PATHCAP,
+                   h->cnt};
+    k_godunov<<<h->glanes / 64, 64, 0, h->stream>>>(ga);
+    toc(h, "godunov", e0);
+  }
 }
 
 static void interface_stage(m2cx_solver* h, int vin, int stage1) {
@@ -1153,7 +1184,11 @@ int m2cx_numerical_flux_batch(m2cx_solver* h, int n, int dir, const double* Vm, 
   HIPCHK(h, hipMemcpy(dVm, Vm, n * 40, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(dVp, Vp, n * 40, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(did, id, n * 4, hipMemcpyHostToDevice));
-  k_flux_batch<<<(n + 255) / 256, 256, 0, h->stream>>>(h->s, n, dir, dVm, dVp, did, dF, h->cnt);
+  if (h->gflux)
+    k_godunov_batch<<<h->glanes / 64, 64, 0, h->stream>>>(h->s.mset, h->grc, n, dir, dVm, dVp, did, dF, h->gslab,
+                                                          h->glanes, GODUNOV_PATHCAP, h->cnt);
+  else
+    k_flux_batch<<<(n + 255) / 256, 256, 0, h->stream>>>(h->s, n, dir, dVm, dVp, did, dF, h->cnt);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(F, dF, n * 40, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -22,6 +22,8 @@ namespace m2c {
 // CUs (the exact solver's register use leaves room for one resident wave per
 // SIMD).  Each lane owns 2 x 512 integration-path points (24 KiB).
 constexpr int RIEM_LANES = 1024 * 64;
+// integration-path points per lane and path of the Godunov face solves
+constexpr int GODUNOV_PATHCAP = 512;
 
 // LevelSet[0] finite-volume settings (LevelSetSchemeData, IoData.h:802-830)
 struct LSFvm {
@@ -295,6 +297,25 @@ __device__ __forceinline__ void bimat_normal(const DevGrid& g, int rnormal, cons
   }
 }
 
+// FluxFcnGodunov.h:37-60: the flux through a face with one material on both
+// sides is the physical flux of the exact Riemann solution at x/t = 0 along
+// the mesh axis.  Uses R's integration-path slot; the caller keeps R's
+// statistics apart (they are not interface statistics).  Returns false on a
+// solve the reference ends the run on; `ovf` reports an integration path
+// longer than the slot holds.
+__device__ __forceinline__ bool flux_godunov(const DevMats& M, DevRiemann& R, int d, const double* Vm,
+                                             const double* Vp, int id, double* F, bool& ovf) {
+  const double dir[3] = {d == 0 ? 1.0 : 0.0, d == 1 ? 1.0 : 0.0, d == 2 ? 1.0 : 0.0};
+  double Vmid[5], Vsm[5], Vsp[5];
+  int midid = id;
+  R.st = RStats{0, 0, false, false, false};
+  R.path1.overflow = R.path3.overflow = false;
+  R.solve(dir, Vm, id, Vp, id, Vmid, midid, Vsm, Vsp);
+  ovf = R.path1.overflow || R.path3.overflow;
+  flux_phys(M.m[id], d, Vmid, F);
+  return !R.st.fatal;
+}
+
 // One interface face: the face states, the exact solve and the two fluxes
 // into the list (SpaceOperator.cpp:1613-1690), shared by the single-solver
 // and the ensemble kernels.  Returns flag bits: 1 trivial, 2 error, 4
@@ -347,6 +368,16 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   if (s.mp_flux == M2CX_MPFLUX_EXACT) {
     flux_phys(M.m[mid], d, Vs, f1);
     for (int p = 0; p < 5; p++) f2[p] = f1[p];
+  } else if (s.flux == M2CX_FLUX_GODUNOV) {
+    // the numerical flux of each side against its star state is itself a
+    // Godunov flux: two more exact solves on this lane's path slot, whose
+    // statistics are not the interface problem's
+    const RStats keep = R.st;
+    bool o1, o2;
+    if (!flux_godunov(M, R, d, Vm, Vsm, idl, f1, o1)) fatal = 1;
+    if (!flux_godunov(M, R, d, Vsp, Vp, idr, f2, o2)) fatal = 1;
+    ovf |= o1 || o2;
+    R.st = keep;
   } else {
     flux_num(M.m[idl], s.flux, s.del, d, Vm, Vsm, f1, ferr);
     flux_num(M.m[idr], s.flux, s.del, d, Vsp, Vp, f2, ferr);
@@ -527,6 +558,80 @@ __global__ void __launch_bounds__(128) k_riemann_batch(int mset, RiemannCfg rc, 
       for (int p = 0; p < 5; p++) Vs[5 * q + p] = Vsm[5 * q + p] = Vsp[5 * q + p] = __builtin_nan("");
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Flux = Godunov (FluxFcnGodunov.h:37-60).  Every stage, before the stage
+// kernel: one lane per face whose two cells hold the same (active) material,
+// faces dealt to the lanes in turn; the face states are the stage kernel's
+// (face_pair), the flux goes to gflux[axis][component][plus cell].  The stage
+// kernel reads it in place of a numerical flux; interface faces keep their
+// multi-material treatment.
+// ---------------------------------------------------------------------------
+struct GodunovArgs {
+  DevGrid g;
+  Scheme s;
+  RiemannCfg rc;
+  CF5 V;
+  const unsigned char* ID;
+  double* gflux;
+  double* slab;
+  int lanes, pathcap;
+  DevCounters* cnt;
+};
+
+__global__ void __launch_bounds__(64) k_godunov(GodunovArgs a) {
+  const int lane = blockIdx.x * 64 + threadIdx.x;
+  if (lane >= a.lanes) return;
+  const DevGrid& g = a.g;
+  const DevMats& M = c_mats[a.s.mset];
+  DevRiemann R(M, a.rc, a.slab + lane, a.lanes, a.pathcap);
+  const long nfx = (long)(g.NX + 1) * g.NY * g.NZ, nfy = (long)g.NX * (g.NY + 1) * g.NZ;
+  const long nfz = (long)g.NX * g.NY * (g.NZ + 1);
+  const long total = nfx + nfy + nfz;
+  int fatal = 0, ovf = 0;
+  for (long q = lane; q < total; q += a.lanes) {
+    int D, i, j, k;
+    long r = q;
+    if (r < nfx) {
+      D = 0; i = (int)(r % (g.NX + 1)); r /= (g.NX + 1); j = (int)(r % g.NY); k = (int)(r / g.NY);
+    } else if ((r -= nfx) < nfy) {
+      D = 1; i = (int)(r % g.NX); r /= g.NX; j = (int)(r % (g.NY + 1)); k = (int)(r / (g.NY + 1));
+    } else {
+      r -= nfy;
+      D = 2; i = (int)(r % g.NX); r /= g.NX; j = (int)(r % g.NY); k = (int)(r / g.NY);
+    }
+    const long c = gidx(g, i, j, k);
+    const int myid = a.ID[c], nid = a.ID[c - axis_stride(g, D)];
+    if (nid != myid || myid == a.s.inactive) continue;
+    double Vm[5], Vp[5], f[5];
+    int bm, bp;
+    face_pair<true>(g, a.s, M, a.V, a.ID, D, i, j, k, Vm, Vp, bm, bp);
+    bool o;
+    if (!flux_godunov(M, R, D, Vm, Vp, myid, f, o)) fatal = 1;
+    ovf |= o;
+    for (int p = 0; p < 5; p++) a.gflux[((long)D * 5 + p) * g.ncells + c] = f[p];
+  }
+  if (fatal) atomicOr(&a.cnt->error, 2);
+  if (ovf) atomicOr(&a.cnt->error, 4);
+}
+
+// m2cx_numerical_flux_batch with Flux = Godunov
+__global__ void __launch_bounds__(64) k_godunov_batch(int mset, RiemannCfg rc, int n, int d, const double* Vm,
+                                                      const double* Vp, const int* id, double* F, double* slab,
+                                                      int lanes, int pathcap, DevCounters* cnt) {
+  const int lane = blockIdx.x * 64 + threadIdx.x;
+  if (lane >= lanes) return;
+  const DevMats& M = c_mats[mset];
+  DevRiemann R(M, rc, slab + lane, lanes, pathcap);
+  int fatal = 0, ovf = 0;
+  for (int q = lane; q < n; q += lanes) {
+    bool o;
+    if (!flux_godunov(M, R, d, Vm + 5 * q, Vp + 5 * q, id[q], F + 5 * q, o)) fatal = 1;
+    ovf |= o;
+  }
+  if (fatal) atomicOr(&cnt->error, 2);
+  if (ovf) atomicOr(&cnt->error, 4);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,6 +26,7 @@ struct Path {
   double* base;   // this lane's column: element (pt, comp) at base[(pt*3+comp)*stride]
   long stride;
   int n, cap;
+  int hint;       // index the last search returned (a starting guess only)
   bool overflow;
   __device__ __forceinline__ double get(int pt, int c) const { return base[((long)pt * 3 + c) * stride]; }
   __device__ __forceinline__ void push(double p, double r, double u) {
@@ -37,6 +38,7 @@ struct Path {
   }
   __device__ __forceinline__ void reset(double p, double r, double u) {
     n = 0;
+    hint = 0;
     push(p, r, u);
   }
 };
@@ -78,11 +80,13 @@ struct DevRiemann {
     path1.stride = stride;
     path1.cap = cap;
     path1.n = 0;
+    path1.hint = 0;
     path1.overflow = false;
     path3.base = slab + (long)cap * 3 * stride;
     path3.stride = stride;
     path3.cap = cap;
     path3.n = 0;
+    path3.hint = 0;
     path3.overflow = false;
     st = RStats{0, 0, false, false, false};
   }
@@ -387,7 +391,11 @@ struct DevRiemann {
     if (count && wave == 1) st.fevals++;
     rhos = rho;
     us = u;
-    const DevEos& E = M.m[id];
+    // the material's parameters copied into registers once per call: read
+    // through the table, every EOS evaluation of the serial chain below
+    // waited on its own memory round trip (stores to the integration path
+    // may alias the table for all the compiler knows)
+    const DevEos E = M.m[id];
     if (!(p > ps)) return shock(E, rho, u, p, ps, rhos0, rhos1, wave, rhos, us);
     const int nsteps = P.numSteps;
     double rhos_0 = rho, us_0 = u, ps_0 = p, xi_0;
@@ -416,15 +424,28 @@ struct DevRiemann {
       // the last one (:1355), so pressures strictly decrease along the path and the
       // points above ps form a prefix: a binary search over the prefix gives
       // the same index with log2(n) dependent loads instead of up to n.
+      // Successive solves of one problem ask for nearby pressures, so the
+      // index the last search returned is checked first: it is the answer
+      // when its point lies above ps and the next one does not.  The three
+      // loads are independent (one memory round trip instead of log2(n)).
       int index0 = 0;
-      if (path.get(0, 0) > ps) {
-        int lo = 0, hi = path.n - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (path.get(mid, 0) > ps) lo = mid;
-          else hi = mid - 1;
+      const int hn = path.n;
+      const int h = path.hint < hn ? path.hint : hn - 1;
+      const int h1 = (h + 1 < hn) ? h + 1 : h;
+      const double p_first = path.get(0, 0), p_h = path.get(h, 0), p_h1 = path.get(h1, 0);
+      if (p_first > ps) {
+        if (p_h > ps && (h1 == h || !(p_h1 > ps))) {
+          index0 = h;
+        } else {
+          int lo = 0, hi = hn - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (path.get(mid, 0) > ps) lo = mid;
+            else hi = mid - 1;
+          }
+          index0 = lo;
         }
-        index0 = lo;
+        path.hint = index0;
       }
       ps_0 = path.get(index0, 0);
       rhos_0 = path.get(index0, 1);

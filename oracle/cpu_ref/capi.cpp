@@ -176,7 +176,23 @@ int orc_flux_batch(const m2cx_material* mats, int nmat, const m2cx_scheme* sch, 
   fl.vf = &vf;
   fl.kind = sch->flux;
   fl.del = sch->entropy_fix;
+  ExactRiemann godunov(vf, RiemannParams::from(*sch));
+  fl.godunov = &godunov;
   for (int q = 0; q < n; q++) fl.Numerical(dir, Vm + 5 * q, Vp + 5 * q, id[q], F + 5 * q);
+  return fl.godunov_fatal ? -1 : 0;
+}
+
+// The two-material LLF flux of MultiPhase.Flux = LocalLaxFriedrichs
+// (FluxFcnLLF.h:119-185): each side with its own material
+int orc_llf_interface_batch(const m2cx_material* mats, int nmat, int n, int dir, const double* Vm, const int* idm,
+                            const double* Vp, const int* idp, double* F) {
+  std::vector<Eos> vf;
+  for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
+  Flux fl;
+  fl.vf = &vf;
+  fl.kind = M2CX_FLUX_LLF;
+  fl.del = 0.0;
+  for (int q = 0; q < n; q++) fl.Llf(dir, Vm + 5 * q, idm[q], Vp + 5 * q, idp[q], F + 5 * q);
   return 0;
 }
 

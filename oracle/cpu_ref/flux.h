@@ -2,10 +2,12 @@
 // numerical flux functions.  Restates FluxFcnBase.h:83-123 (physical
 // fluxes), :139-154 (Jacobian eigen-structure used by GenRoe), :287-303 (max
 // eigenvalues), FluxFcnHLLC.h:58-245, FluxFcnGenRoe.h:64-221 and
-// FluxFcnLLF.h:47-185, keeping the reference's operation order.
+// FluxFcnLLF.h:47-185 and FluxFcnGodunov.h:37-60, keeping the reference's
+// operation order.
 #pragma once
 #include <vector>
 #include "eos.h"
+#include "riemann.h"
 
 namespace orc {
 
@@ -13,6 +15,11 @@ struct Flux {
   const std::vector<Eos>* vf = nullptr;
   int kind = M2CX_FLUX_HLLC;
   double del = 0.2;   // Harten entropy fix (FluxFcnGenRoe.h:30-31)
+  // FluxFcnGodunov's own exact solver (FluxFcnGodunov.h:30); set when kind is
+  // M2CX_FLUX_GODUNOV.  `godunov_fatal` records a solve the reference would
+  // have ended the run on.
+  ExactRiemann* godunov = nullptr;
+  mutable bool godunov_fatal = false;
   static constexpr double eps = 1e-10;  // Hu et al. (FluxFcnHLLC.h:24-25)
 
   const Eos& E(int id) const { return (*vf)[id]; }
@@ -203,10 +210,24 @@ struct Flux {
     for (int i = 0; i < 5; i++) F[i] = 0.5 * ((fm[i] + fp[i]) - a * (Up[i] - Um[i]));
   }
 
+  // FluxFcnGodunov.h:37-60: the physical flux of the exact Riemann solution
+  // (one material on both sides, along the mesh axis) at x/t = 0
+  void Godunov(int dir, const double* Vm, const double* Vp, int id, double* F) const {
+    double normal[3] = {0.0, 0.0, 0.0};
+    normal[dir] = 1.0;
+    double Vmid[5], Vsm[5], Vsp[5];
+    int midid = id;
+    godunov->st = RiemannStats();
+    godunov->Solve(normal, Vm, id, Vp, id, Vmid, midid, Vsm, Vsp);
+    if (godunov->st.fatal) godunov_fatal = true;
+    Physical(dir, Vmid, id, F);
+  }
+
   // FluxFcnBase::ComputeNumericalFluxAtCellInterface dispatch
   void Numerical(int dir, const double* Vm, const double* Vp, int id, double* F) const {
     if (kind == M2CX_FLUX_HLLC) Hllc(dir, Vm, Vp, id, F);
     else if (kind == M2CX_FLUX_ROE) Roe(dir, Vm, Vp, id, F);
+    else if (kind == M2CX_FLUX_GODUNOV) Godunov(dir, Vm, Vp, id, F);
     else Llf(dir, Vm, id, Vp, id, F);
   }
 };

@@ -65,7 +65,7 @@ static void FD3Coefficients(double d0, double d1, double d2, double d3, double* 
 }
 
 Solver::Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat_, const m2cx_scheme& s)
-    : grid(g), sch(s), nmat(nmat_), riem(vf, RiemannParams::from(s)) {
+    : grid(g), sch(s), nmat(nmat_), riem(vf, RiemannParams::from(s)), godunov_riem(vf, RiemannParams::from(s)) {
   for (int i = 0; i < nmat; i++) vf.push_back(Eos::from(mats[i]));
   // INACTIVE_MATERIAL_ID = number of materials; a dummy EOS sits there
   // (Main.cpp:127-133)
@@ -76,6 +76,7 @@ Solver::Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat_, const m
   flux.vf = &vf;
   flux.kind = s.flux;
   flux.del = s.entropy_fix;
+  if (s.flux == M2CX_FLUX_GODUNOV) flux.godunov = &godunov_riem;
   SetupMesh();
   SetupGhosts();
   SetupFD3();
@@ -684,6 +685,10 @@ void Solver::ComputeAdvectionFluxes(const std::vector<double>& Vin, std::vector<
           for (int p = 0; p < 5; p++) fc[p] -= area * f2[p];
         }
       }
+  if (flux.godunov_fatal) {
+    flux.godunov_fatal = false;
+    Fail("Exact Riemann solver reached a fatal state (Godunov flux)");
+  }
 }
 
 // SpaceOperator::ComputeResidual (SpaceOperator.cpp:2573-2641)

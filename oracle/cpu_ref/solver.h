@@ -75,6 +75,7 @@ class Solver {
   int nmat, inactive_id;
   Flux flux;
   ExactRiemann riem;
+  ExactRiemann godunov_riem;  // the Godunov flux's own solver (FluxFcnGodunov.h:30)
   // record of interface Riemann problems (17 doubles each: dir, Vm, idm, Vp,
   // idp, err, iterations, f evaluations, fallback), for tests
   bool riem_log_on = false;

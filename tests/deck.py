@@ -9,7 +9,7 @@ from m2c_amd import abi
 BC_NAMES = {"Inlet": abi.BC_INLET, "Farfield": abi.BC_INLET, "Inlet2": abi.BC_INLET2, "Farfield2": abi.BC_INLET2,
             "SlipWall": abi.BC_SLIPWALL, "StickWall": abi.BC_STICKWALL, "Symmetry": abi.BC_SYMMETRY,
             "Outlet": abi.BC_OUTLET}
-FLUX = {"Roe": abi.FLUX_ROE, "LocalLaxFriedrichs": abi.FLUX_LLF, "HLLC": abi.FLUX_HLLC}
+FLUX = {"Roe": abi.FLUX_ROE, "LocalLaxFriedrichs": abi.FLUX_LLF, "HLLC": abi.FLUX_HLLC, "Godunov": abi.FLUX_GODUNOV}
 LIM = {"None": abi.LIM_NONE, "GeneralizedMinMod": abi.LIM_GMM, "VanAlbada": abi.LIM_VANALBADA}
 VAR = {"Primitive": abi.VAR_PRIMITIVE, "Conservative": abi.VAR_CONSERVATIVE,
        "PrimitiveCharacteristic": abi.VAR_PRIMITIVE_CHARACTERISTIC,

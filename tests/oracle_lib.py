@@ -19,9 +19,10 @@ I = C.POINTER(C.c_int)
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        LIB = os.environ.get("M2C_ORACLE_LIB", LIB_PATH)
+        if not os.path.exists(LIB):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(LIB)
         L.orc_create.restype = P
         L.orc_create.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme)]
         L.orc_destroy.argtypes = [P]
@@ -202,6 +203,18 @@ def flux_batch(mats, nmat, scheme, d, Vm, Vp, ids):
     ids = np.ascontiguousarray(ids, dtype=np.int32)
     F = np.zeros((n, 5))
     lib().orc_flux_batch(mats, nmat, C.byref(scheme), n, d, dp(Vm), dp(Vp), ip(ids), dp(F))
+    return F
+
+
+def llf_interface_batch(mats, nmat, d, Vm, idm, Vp, idp):
+    """The two-material LLF flux along axis d, each side with its own material."""
+    n = len(Vm)
+    Vm = np.ascontiguousarray(Vm, dtype=np.float64)
+    Vp = np.ascontiguousarray(Vp, dtype=np.float64)
+    idm = np.ascontiguousarray(idm, dtype=np.int32)
+    idp = np.ascontiguousarray(idp, dtype=np.int32)
+    F = np.zeros((n, 5))
+    lib().orc_llf_interface_batch(mats, nmat, n, d, dp(Vm), ip(idm), dp(Vp), ip(idp), dp(F))
     return F
 
 
