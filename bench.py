@@ -184,11 +184,12 @@ def cpu_baseline(name, workers, budget_s=20.0):
         res.append(json.loads(out.strip().splitlines()[-1]))
     value = sum(r["cells"] * r["stages"] * r["steps"] / r["seconds"] for r in res)
     steps = sorted(r["steps"] for r in res)
+    nsteps = str(steps[0]) if steps[0] == steps[-1] else "%d to %d" % (steps[0], steps[-1])
     return {"value": value, "unit": "cell-stage-updates/s", "cores": workers, "kind": "port",
             "sample": ("the same configuration split into %d bricks of %s cells, one single-threaded oracle "
-                       "process per brick and core, all running at once, each for its first %d-%d steps from the "
+                       "process per brick and core, all running at once, each for its first %s steps from the "
                        "initial state (about %g s); bricks do not exchange halos"
-                       % (workers, "x".join(str(x) for x in res[0]["brick"]), steps[0], steps[-1], budget_s)),
+                       % (workers, "x".join(str(x) for x in res[0]["brick"]), nsteps, budget_s)),
             "cpu": _cpu_model(), "processes": workers}
 
 
@@ -252,25 +253,57 @@ def riemann_stats(cnt, hist=None, maxits=None):
     return out
 
 
-def riemann_kernel_figures(cnt, riem_ms, steps):
-    """The exact-Riemann kernel's own rates over the timed steps (its launch
-    time from HIP events on its stream; with the overlap it shares the CUs
-    with stage tiles).  FP64 FLOP per f(p) evaluation, where a profile of the
-    kernel's FP64 instruction counts exists (profiles/k_riemann_c5_valu.txt),
-    turns the evaluation rate into FLOP/s."""
-    sec = riem_ms * 1e-3
-    out = {"ms_per_step": riem_ms / steps, "faces_per_s": cnt["riemann_calls"] / sec,
-           "main_iters_per_s": cnt["riemann_main_iters"] / sec, "fevals_per_s": cnt["riemann_fevals"] / sec}
-    path = os.path.join(ROOT, "profiles", "k_riemann_c5_valu.txt")
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _profile_figures(name):
+    """`key: value` lines of a summary under profiles/ (numbers only)."""
+    out = {}
+    path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
-                if line.startswith("fp64_flop_per_feval:"):
-                    fpe = float(line.split(":", 1)[1])
-                    out["fp64_flop_per_feval"] = fpe
-                    out["fp64_tflops"] = fpe * out["fevals_per_s"] / 1e12
-                    out["fp64_peak_tflops"] = 78.6
+                key, sep, val = line.partition(":")
+                if sep and key.strip() and " " not in key.strip():
+                    try:
+                        out[key.strip()] = float(val.split()[0])
+                    except (ValueError, IndexError):
+                        pass
     return out
+
+
+def riemann_kernel_figures(cnt, riem_ms, steps):
+    """The exact-Riemann kernel's own rates over the timed steps (its launch
+    time from HIP events on its stream; with the overlap it shares the CUs
+    with stage tiles).  The FP64 FLOP per f(p) evaluation that a counter
+    profile of the kernel measured (profiles/k_riemann_c5_valu.txt: FP64
+    instructions x 64 lanes x the active-lane fraction, per evaluation) turns
+    the evaluation rate into FLOP/s."""
+    sec = riem_ms * 1e-3
+    out = {"ms_per_step": riem_ms / steps, "faces_per_s": cnt["riemann_calls"] / sec,
+           "main_iters_per_s": cnt["riemann_main_iters"] / sec, "fevals_per_s": cnt["riemann_fevals"] / sec}
+    prof = _profile_figures("k_riemann_c5_valu.txt")
+    if "fp64_flop_per_feval" in prof:
+        fpe = prof["fp64_flop_per_feval"]
+        out["fp64_flop_per_feval"] = fpe
+        out["fp64_tflops"] = fpe * out["fevals_per_s"] / 1e12
+        out["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
+        if "active_lane_fraction" in prof:
+            out["active_lane_fraction"] = prof["active_lane_fraction"]
+    return out
+
+
+def riemann_roofline(rk):
+    """The exact-Riemann kernel against the FP64 vector peak: it is bound by
+    the serial latency of its slowest faces, not by a throughput, so the
+    fraction is far below one by construction and says how much of the
+    chip's FP64 issue the latency-bound solve uses."""
+    if not rk or "fp64_tflops" not in rk:
+        return None
+    return {"kernel": "k_riemann (exact two-material Riemann solver, one lane per interface face)",
+            "bound": "fp64-latency", "achieved": rk["fp64_tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": rk["fp64_tflops"] / FP64_PEAK_TFLOPS, "flop_per_feval": rk["fp64_flop_per_feval"],
+            "active_lane_fraction": rk.get("active_lane_fraction")}
 
 
 def run_single(cfg, steps, warmup, learned_params=None):
@@ -493,6 +526,7 @@ def main():
                    "riemann_guess": "acoustic (plain)"},
         "riemann": plain["riemann"],
         "riemann_kernel": plain.get("riemann_kernel"),
+        "roofline_riemann": riemann_roofline(plain.get("riemann_kernel")),
         "reinit": plain.get("reinit"),
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused residual + RK stage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -508,7 +542,9 @@ def main():
                           "speedup_vs_plain": lr["value"] / plain["value"], "params": "m2c_amd/data/guess_params.npy"}
     if args.config == "c5" and not args.no_ensemble:
         print("# bench: c4 ensemble of %d" % args.instances, file=sys.stderr, flush=True)
-        out["ensemble_c4"] = c4_summary(args.instances, min(steps, 20), args.warmup)
+        # its first 10 steps: the members run alone with the plain guess take
+        # about 3.5 s a step together
+        out["ensemble_c4"] = c4_summary(args.instances, min(steps, 10), args.warmup)
     if cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

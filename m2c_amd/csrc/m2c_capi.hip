@@ -1734,6 +1734,9 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
     if (reinit_enabled(h))
       return efail(e, M2CX_E_UNSUPPORTED, "level-set reinitialization needs host control between steps; "
                                           "step such members alone");
+    if (h->sch.flux == M2CX_FLUX_GODUNOV && h->multimat && h->sch.mp_flux == M2CX_MPFLUX_NUMERICAL)
+      return efail(e, M2CX_E_UNSUPPORTED, "Godunov side fluxes at interfaces (Flux = Godunov with "
+                                          "MultiPhase.Flux = Numerical): step such members alone");
   }
   for (int i = 0; i < n; i++)
     for (int j = 0; j < i; j++)

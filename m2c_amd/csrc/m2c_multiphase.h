@@ -319,7 +319,11 @@ __device__ __forceinline__ bool flux_godunov(const DevMats& M, DevRiemann& R, in
 // One interface face: the face states, the exact solve and the two fluxes
 // into the list (SpaceOperator.cpp:1613-1690), shared by the single-solver
 // and the ensemble kernels.  Returns flag bits: 1 trivial, 2 error, 4
-// fallback solver used, 8 learned guess rejected.
+// fallback solver used, 8 learned guess rejected.  GOD: the side fluxes of
+// MultiPhase.Flux = Numerical are Godunov fluxes (Flux = Godunov); a separate
+// instantiation, so that the common kernel does not carry that code's
+// registers.
+template <bool GOD = false>
 __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, const CF5& V,
                                             const unsigned char* __restrict__ ID, const IfaceList& L, int q,
                                             int stage1, const float* guess, DevRiemann& R, int& fatal, int& ovf) {
@@ -368,7 +372,7 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   if (s.mp_flux == M2CX_MPFLUX_EXACT) {
     flux_phys(M.m[mid], d, Vs, f1);
     for (int p = 0; p < 5; p++) f2[p] = f1[p];
-  } else if (s.flux == M2CX_FLUX_GODUNOV) {
+  } else if (GOD) {
     // the numerical flux of each side against its star state is itself a
     // Godunov flux: two more exact solves on this lane's path slot, whose
     // statistics are not the interface problem's
@@ -404,6 +408,7 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
 // stream and its slowest wave sets its length; raising the waves' issue
 // priority on the SIMDs they share with the stage kernel shortens it (C5 bubble
 // cloud: 84.2 -> 80.3 ms of Riemann time per step, results unchanged).
+template <bool GOD>
 __global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg rc, CF5 V,
                                                 const unsigned char* __restrict__ ID, IfaceList L, int stage1,
                                                 double* slab, int lanes, int pathcap, const float* guess,
@@ -417,7 +422,7 @@ __global__ void __launch_bounds__(64) k_riemann(DevGrid g, Scheme s, RiemannCfg 
   unsigned long long calls = 0, triv = 0, errs = 0, its = 0, fev = 0, fb = 0, gfb = 0;
   int fatal = 0, ovf = 0;
   for (long q = (long)lane * W + wave; q < nif; q += 64L * W) {
-    const int f = riemann_face(g, s, V, ID, L, (int)q, stage1, guess, R, fatal, ovf);
+    const int f = riemann_face<GOD>(g, s, V, ID, L, (int)q, stage1, guess, R, fatal, ovf);
     calls++;
     triv += f & 1;
     errs += (f >> 1) & 1;
@@ -1374,7 +1379,8 @@ inline int mp_slab(MPBuffers& B) {
 inline void mp_riemann(MPBuffers& B, const DevGrid& g, const Scheme& s, CF5 V, const unsigned char* ID, int stage1,
                        DevCounters* cnt, hipStream_t st) {
   if (s.mp_flux == M2CX_MPFLUX_LLF) return;
-  k_riemann<<<B.lanes / 64, 64, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
+  auto* kern = (s.flux == M2CX_FLUX_GODUNOV && s.mp_flux == M2CX_MPFLUX_NUMERICAL) ? k_riemann<true> : k_riemann<false>;
+  kern<<<B.lanes / 64, 64, 0, st>>>(g, s, B.rc, V, ID, B.L, stage1, B.slab, B.lanes, B.pathcap,
                                          B.guess ? B.pguess : nullptr, cnt);
 }
 
