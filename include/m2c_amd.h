@@ -63,6 +63,12 @@ enum { M2CX_TI_FORWARD_EULER = 0, M2CX_TI_RK2 = 1, M2CX_TI_RK3 = 2 };
 enum { M2CX_MPFLUX_EXACT = 0, M2CX_MPFLUX_NUMERICAL = 1, M2CX_MPFLUX_LLF = 2 };
 enum { M2CX_MPREC_CONSTANT = 0, M2CX_MPREC_LINEAR = 1 };
 enum { M2CX_PHASECHANGE_RIEMANN_SOLUTION = 0, M2CX_PHASECHANGE_EXTRAPOLATION = 1 };
+
+/* MultiPhaseData::PhaseChangeDirection (IoData.h:901): the weights of the
+ * neighbours a cell that changed material takes its state from
+ * (MultiPhaseOperator.cpp:1220-1246, 1396) -- ALL: inverse distance, UPWIND:
+ * the neighbour's flow direction towards the cell (the reference default). */
+enum { M2CX_PHASEDIR_ALL = 0, M2CX_PHASEDIR_UPWIND = 1 };
 enum { M2CX_RNORMAL_LEVEL_SET = 0, M2CX_RNORMAL_MESH = 1, M2CX_RNORMAL_AVERAGE = 2 };
 
 /* LevelSetSchemeData::Solver and ::Flux (IoData.h:806-810) */
@@ -127,8 +133,7 @@ typedef struct {
   double max_time;             /* MaxTime; <= 0 means no clamp                 */
   int32_t mp_flux;             /* M2CX_MPFLUX_*                                */
   int32_t mp_recon;            /* M2CX_MPREC_*                                 */
-  int32_t mp_phase_change;     /* M2CX_PHASECHANGE_* (RiemannSolution; the
-                                  direction is Upwind, the default)             */
+  int32_t mp_phase_change;     /* M2CX_PHASECHANGE_* (RiemannSolution)       */
   int32_t riemann_normal;      /* RiemannNormal, M2CX_RNORMAL_* (Mesh)          */
   int32_t riem_maxits_main;    /* MaxIts (200)                                 */
   int32_t riem_maxits_bracket; /* MaxItsBracketing (100)                       */
@@ -144,7 +149,7 @@ typedef struct {
   int32_t ls_materialid;       /* LevelSet[0].MaterialID                       */
   int32_t ls_bc[6];            /* LevelSet[0].BoundaryCondition* (M2CX_LSBC_*) */
   int32_t riem_guess;          /* M2CX_GUESS_*                                 */
-  int32_t reserved;
+  int32_t mp_phase_dir;        /* PhaseChangeDirection, M2CX_PHASEDIR_* (Upwind) */
   /* LevelSet[0].Reinitialization (IoData.cpp:1421-1429); the level set is
    * reinitialized when reinit_frequency > 0 or reinit_frequency_dt > 0      */
   int32_t reinit_frequency;    /* Frequency (-1)                               */

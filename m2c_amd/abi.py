@@ -20,6 +20,7 @@ MPREC_CONSTANT, MPREC_LINEAR = 0, 1
 GUESS_ACOUSTIC, GUESS_MLP = 0, 1
 RNORMAL_LEVEL_SET, RNORMAL_MESH, RNORMAL_AVERAGE = 0, 1, 2
 PHASECHANGE_RIEMANN_SOLUTION, PHASECHANGE_EXTRAPOLATION = 0, 1
+PHASEDIR_ALL, PHASEDIR_UPWIND = 0, 1
 LSSOLVER_FINITE_VOLUME, LSSOLVER_FINITE_DIFFERENCE = 0, 1
 LSFLUX_ROE, LSFLUX_LLF, LSFLUX_UPWIND = 0, 1, 2
 REINIT_FIXED, REINIT_CONSTRAINED1, REINIT_CONSTRAINED2, REINIT_ITERATIVE_CONSTRAINED1, \
@@ -67,7 +68,7 @@ class Scheme(C.Structure):
                 ("riem_tol_rarefaction", C.c_double), ("riem_min_pressure", C.c_double),
                 ("riem_failure_threshold", C.c_double), ("riem_pressure_at_failure", C.c_double),
                 ("n_levelsets", C.c_int32), ("ls_materialid", C.c_int32), ("ls_bc", C.c_int32 * 6),
-                ("riem_guess", C.c_int32), ("reserved", C.c_int32),
+                ("riem_guess", C.c_int32), ("mp_phase_dir", C.c_int32),
                 ("reinit_frequency", C.c_int32), ("reinit_maxits", C.c_int32),
                 ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
                 ("reinit_first_layer", C.c_int32), ("reserved2", C.c_int32), ("time_step", C.c_double),
@@ -131,6 +132,7 @@ def default_scheme(**kw):
     s.mp_flux = MPFLUX_NUMERICAL
     s.mp_recon = MPREC_CONSTANT
     s.mp_phase_change = PHASECHANGE_RIEMANN_SOLUTION
+    s.mp_phase_dir = PHASEDIR_UPWIND
     s.riemann_normal = RNORMAL_MESH
     s.riem_maxits_main = 200
     s.riem_maxits_bracket = 100

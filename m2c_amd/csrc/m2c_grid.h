@@ -63,6 +63,7 @@ struct Scheme {
   double time_step;             // Time.TimeStep: fixed dt when > 0
   int rnormal;                  // MultiPhase.RiemannNormal, M2CX_RNORMAL_*
   int phase_change;             // MultiPhase.PhaseChange, M2CX_PHASECHANGE_*
+  int phase_dir;                // MultiPhase.PhaseChangeDirection, M2CX_PHASEDIR_*
   // per cell (ghosted index): 1 where a reconstruction in var_type fails
   // CheckState on some face and the cell falls back to primitive variables
   // (Reconstructor.cpp:722-778); written by k_recon_flags each stage

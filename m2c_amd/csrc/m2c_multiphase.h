@@ -1158,17 +1158,23 @@ __global__ void k_extrapolate(UpdArgs a) {
           if (a.ID[nb] != myid) continue;
           if (a.ID[nb] != a.IDn[nb]) continue;
           if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
-          double v1[3] = {a.V(1, nb), a.V(2, nb), a.V(3, nb)};
-          const double v1n = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-          if (v1n != 0) {
-            const double inv = 1.0 / v1n;  // Vec3D::operator/= multiplies by the reciprocal
-            v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+          double w;
+          if (a.s.phase_dir == M2CX_PHASEDIR_ALL) {  // inverse distance (:1220-1221)
+            const double d[3] = {x0[0] - X[ni + 1], x0[1] - Y[nj + 1], x0[2] - Z[nk + 1]};
+            w = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          } else {
+            double v1[3] = {a.V(1, nb), a.V(2, nb), a.V(3, nb)};
+            const double v1n = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+            if (v1n != 0) {
+              const double inv = 1.0 / v1n;  // Vec3D::operator/= multiplies by the reciprocal
+              v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+            }
+            double d[3] = {x0[0] - X[ni + 1], x0[1] - Y[nj + 1], x0[2] - Z[nk + 1]};
+            const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            d[0] *= inv; d[1] *= inv; d[2] *= inv;
+            const double dot = d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2];
+            w = (0.0 < dot) ? dot : 0.0;  // std::max(0.0, dot)
           }
-          double d[3] = {x0[0] - X[ni + 1], x0[1] - Y[nj + 1], x0[2] - Z[nk + 1]};
-          const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-          d[0] *= inv; d[1] *= inv; d[2] *= inv;
-          const double dot = d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2];
-          const double w = (0.0 < dot) ? dot : 0.0;  // std::max(0.0, dot)
           if (w > 0) {
             sw += w;
             for (int p = 0; p < 5; p++) vs[p] += w * a.V(p, nb);
@@ -1184,9 +1190,11 @@ __global__ void k_extrapolate(UpdArgs a) {
   }
 }
 
-// FixUnresolvedNodes (:1278-1525) with apply_failsafe_density = On and
-// phase-change direction UPWIND; one lane, nodes in (k,j,i) order, with the
-// reference's `reset` flag shared by all nodes.
+// FixUnresolvedNodes (:1278-1525) with apply_failsafe_density = On; one
+// lane, nodes in (k,j,i) order, with the reference's `reset` flag shared by
+// all nodes.  With PhaseChangeDirection = All the upwind accumulation still
+// writes the node and sets `reset`, but the inverse-distance average is
+// taken (:1396).
 __global__ void k_fix_unresolved(UpdArgs a) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nu = min(a.cnt->n_unresolved, a.cap);
@@ -1247,7 +1255,7 @@ __global__ void k_fix_unresolved(UpdArgs a) {
           sum_weight2 += 1.0 / xn;
         }
     bool fixed = false;
-    if (sum_weight > 0) {
+    if (a.s.phase_dir == M2CX_PHASEDIR_UPWIND && sum_weight > 0) {
       double cs = 1.0 / sum_weight;
       for (int p = 0; p < 5; p++) v[p] *= cs;
       fixed = true;

@@ -579,9 +579,10 @@ void Solver::LSComputeResidualFVM(const std::vector<double>& Vin, const std::vec
 }
 
 // MultiPhaseOperator::UpdateStateVariablesByExtrapolation (MultiPhaseOperator.
-// cpp:1155-1272), upwind weights: a cell that changed material takes the
-// weighted average of its neighbours (3x3x3) of the same material that did
-// not change, each weighted by how much its flow points at the cell.
+// cpp:1155-1272): a cell that changed material takes the weighted average of
+// its neighbours (3x3x3) of the same material that did not change, each
+// weighted by how much its flow points at the cell (Upwind) or by its inverse
+// distance (PhaseChangeDirection = All, :1220-1221).
 int Solver::UpdateStateVariablesByExtrapolation() {
   std::vector<std::array<int, 3>> unresolved;
   for (int k = 0; k < m.NZ; k++)
@@ -601,17 +602,23 @@ int Solver::UpdateStateVariablesByExtrapolation() {
               if (ID[q] != IDn[q]) continue;
               if (!m.inside(ni, nj, nk)) continue;
               const double* vn = &V[q * 5];
-              double v1[3] = {vn[1], vn[2], vn[3]};
-              double v1n = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-              if (v1n != 0) {
-                double inv = 1.0 / v1n;
-                v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+              double weight;
+              if (sch.mp_phase_dir == M2CX_PHASEDIR_ALL) {
+                double d[3] = {x0[0] - m.X(ni), x0[1] - m.Y(nj), x0[2] - m.Z(nk)};
+                weight = 1.0 / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+              } else {
+                double v1[3] = {vn[1], vn[2], vn[3]};
+                double v1n = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+                if (v1n != 0) {
+                  double inv = 1.0 / v1n;
+                  v1[0] *= inv; v1[1] *= inv; v1[2] *= inv;
+                }
+                double d[3] = {x0[0] - m.X(ni), x0[1] - m.Y(nj), x0[2] - m.Z(nk)};
+                double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                double inv = 1.0 / dn;
+                d[0] *= inv; d[1] *= inv; d[2] *= inv;
+                weight = std::max(0.0, d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2]);
               }
-              double d[3] = {x0[0] - m.X(ni), x0[1] - m.Y(nj), x0[2] - m.Z(nk)};
-              double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-              double inv = 1.0 / dn;
-              d[0] *= inv; d[1] *= inv; d[2] *= inv;
-              double weight = std::max(0.0, d[0] * v1[0] + d[1] * v1[1] + d[2] * v1[2]);
               if (weight > 0) {
                 sum_weight += weight;
                 for (int p = 0; p < 5; p++) vsum[p] += weight * vn[p];

@@ -977,8 +977,9 @@ int Solver::UpdateStateVariablesByRiemannSolutions() {
 }
 
 // MultiPhaseOperator::FixUnresolvedNodes (MultiPhaseOperator.cpp:1278-1525),
-// apply_failsafe_density = On, phase-change direction UPWIND.  `reset` is
-// declared once for all nodes, as in the reference.
+// apply_failsafe_density = On.  `reset` is declared once for all nodes, as in
+// the reference; with PhaseChangeDirection = All the upwind accumulation still
+// writes the node (and sets `reset`) but its result is not used (:1396).
 void Solver::FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved) {
   bool reset = false;
   auto is_unres = [&](int ni, int nj, int nk) {
@@ -1025,7 +1026,7 @@ void Solver::FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved) {
           for (int p = 0; p < 5; p++) vtmp[p] += 1.0 / x1x0norm * vn[p];
           sum_weight2 += 1.0 / x1x0norm;
         }
-    if (sum_weight > 0) {
+    if (sch.mp_phase_dir == M2CX_PHASEDIR_UPWIND && sum_weight > 0) {
       double cst = 1.0 / sum_weight;
       for (int p = 0; p < 5; p++) v[p] *= cst;
       continue;

@@ -100,6 +100,7 @@ def setup(deck):
     s.mp_recon = abi.MPREC_LINEAR if mp.get("ReconstructionAtInterface", "Constant") == "Linear" else abi.MPREC_CONSTANT
     s.mp_phase_change = {"RiemannSolution": abi.PHASECHANGE_RIEMANN_SOLUTION,
                          "Extrapolation": abi.PHASECHANGE_EXTRAPOLATION}[mp.get("PhaseChange", "RiemannSolution")]
+    s.mp_phase_dir = {"All": abi.PHASEDIR_ALL, "Upwind": abi.PHASEDIR_UPWIND}[mp.get("PhaseChangeDirection", "Upwind")]
     s.riemann_normal = {"LevelSet": abi.RNORMAL_LEVEL_SET, "Mesh": abi.RNORMAL_MESH,
                         "Average": abi.RNORMAL_AVERAGE}[mp.get("RiemannNormal", "Mesh")]
     ls = [k for k in deck["Space"] if k.startswith("LevelSet")]

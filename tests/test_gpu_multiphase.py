@@ -253,17 +253,21 @@ def _moving_plane(n=(24, 12, 10)):
 
 # LevelSet[0] Solver = FiniteVolume with each flux and limiter, and
 # PhaseChange = Extrapolation, on a moving 3D bubble
-@pytest.mark.parametrize("lsflux,lslim,pc", [
-    (abi.LSFLUX_ROE, abi.LIM_NONE, abi.PHASECHANGE_EXTRAPOLATION),
-    (abi.LSFLUX_LLF, abi.LIM_GMM, abi.PHASECHANGE_RIEMANN_SOLUTION),
-    (abi.LSFLUX_UPWIND, abi.LIM_VANALBADA, abi.PHASECHANGE_EXTRAPOLATION)])
-def test_levelset_finite_volume_and_extrapolation_bitwise(lsflux, lslim, pc):
-    print("ls fvm", lsflux, lslim, "phase change", pc, flush=True)
+# (PhaseChangeDirection All as well as Upwind)
+@pytest.mark.parametrize("lsflux,lslim,pc,pdir", [
+    (abi.LSFLUX_ROE, abi.LIM_NONE, abi.PHASECHANGE_EXTRAPOLATION, abi.PHASEDIR_UPWIND),
+    (abi.LSFLUX_LLF, abi.LIM_GMM, abi.PHASECHANGE_RIEMANN_SOLUTION, abi.PHASEDIR_UPWIND),
+    (abi.LSFLUX_UPWIND, abi.LIM_VANALBADA, abi.PHASECHANGE_EXTRAPOLATION, abi.PHASEDIR_UPWIND),
+    (abi.LSFLUX_ROE, abi.LIM_NONE, abi.PHASECHANGE_EXTRAPOLATION, abi.PHASEDIR_ALL),
+    (abi.LSFLUX_LLF, abi.LIM_GMM, abi.PHASECHANGE_RIEMANN_SOLUTION, abi.PHASEDIR_ALL)])
+def test_levelset_finite_volume_and_extrapolation_bitwise(lsflux, lslim, pc, pdir):
+    print("ls fvm", lsflux, lslim, "phase change", pc, "direction", pdir, flush=True)
     g, mats, nmat, s, V, ID, Phi = _moving_plane()
     s.ls_solver = abi.LSSOLVER_FINITE_VOLUME
     s.ls_flux = lsflux
     s.ls_limiter = lslim
     s.mp_phase_change = pc
+    s.mp_phase_dir = pdir
     # the ghosts as the reference's initialization leaves them: boundary
     # conditions of V, the IDs and the level set applied
     o = ol.Oracle(g, mats, nmat, s)
