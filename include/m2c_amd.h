@@ -158,7 +158,13 @@ typedef struct {
   double reinit_cfl;           /* CFL (0.8)                                    */
   double reinit_tol;           /* ConvergenceTolerance (2e-4)                  */
   int32_t reinit_first_layer;  /* FirstLayerTreatment, M2CX_REINIT_* (Fixed)   */
-  int32_t reserved2;
+  /* LevelSet[0].Bandwidth (IoData.cpp:1519, INT_MAX): the narrow-band level
+   * set of LevelSetOperator.cpp:84-99 when 2 <= ls_bandwidth < INT32_MAX
+   * (requires 1 <= reinit_frequency < ls_bandwidth, as the reference does);
+   * 0 or INT32_MAX: the level set over the full domain.  The band is built
+   * by m2cx_initialize, or by m2cx_levelset_construct_narrow_band after the
+   * level set is uploaded.                                                   */
+  int32_t ls_bandwidth;
   double time_step;            /* Time.TimeStep: fixed dt when > 0 (0)         */
   /* LevelSet[0] Solver, Flux, Reconstruction (IoData.cpp:1513-1535, 1265-1274) */
   int32_t ls_solver;           /* M2CX_LSSOLVER_* (FiniteDifference)           */
@@ -276,7 +282,9 @@ int m2cx_clip(m2cx_solver* h, int v_field);
 /* LevelSetOperator::ComputeResidual + ApplyBoundaryConditions
  * (LevelSetOperator.h:80-107, LevelSetOperator.cpp:768-870, :408-508)
  * for level set 0 with velocities from `v_field`; result in the device R_phi
- * buffer (download with m2cx_download_ls_residual). */
+ * buffer (download with m2cx_download_ls_residual).  With the narrow band
+ * (ComputeResidualFDM_NarrowBand, :880-955) R_phi is cleared on the band and
+ * computed on its active nodes; elsewhere it keeps its earlier values. */
 int m2cx_levelset_residual(m2cx_solver* h, int v_field);
 int m2cx_download_ls_residual(m2cx_solver* h, double* host);
 
@@ -292,6 +300,27 @@ int m2cx_download_ls_residual(m2cx_solver* h, double* host);
  * (the reference exits there).  Time stepping calls it on its own at the
  * configured frequency. */
 int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits);
+/* With scheme.ls_bandwidth set the reinitialization runs in the band
+ * (LevelSetReinitializer::ReinitializeInBand, LevelSetReinitializer.cpp:
+ * 181-300), which first moves the band to the interface. */
+
+/* LevelSetOperator::ConstructNarrowBandInReinitializer (LevelSetOperator.cpp:
+ * 1328-1334 -> LevelSetReinitializer::ConstructNarrowBand, LevelSetReinitializer
+ * .cpp:1479-1592): builds the band around the zero level set of level set 0
+ * (levels 0 .. ls_bandwidth by face-neighbour distance, edges and corners of
+ * the ghost layer excluded) and cuts phi off outside it to 10 x its largest
+ * (smallest) value in the band.  Call after uploading the level set with its
+ * boundary values, as SpaceInitializer.cpp:1787-1793 does after
+ * InitializePhiByDistance; m2cx_initialize calls it itself.  Stepping,
+ * m2cx_levelset_residual and m2cx_levelset_reinitialize need it.
+ * M2CX_E_UNSUPPORTED without a band. */
+int m2cx_levelset_construct_narrow_band(m2cx_solver* h);
+/* The band for inspection: per ghosted cell its level (31415927 outside),
+ * useful (level <= bandwidth) and active (level < bandwidth) flags, and
+ * bounds = {phi_max, phi_min, phi_out_pos, phi_out_neg}; any may be null.
+ * No counterpart in the reference (its Level, UsefulG2 and Active fields are
+ * members of LevelSetOperator, LevelSetOperator.h:60-62). */
+int m2cx_get_narrow_band(m2cx_solver* h, int32_t* level, uint8_t* useful, uint8_t* active, double* bounds);
 
 /* ------------------------------------------------------------------ */
 /* TimeIntegrator (TimeIntegrator.h:77-88, TimeIntegrator.cpp:167-843) */

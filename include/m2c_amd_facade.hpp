@@ -153,6 +153,9 @@ class LevelSetOperator {
   void ComputeResidual(SpaceVariable3D& V, SpaceVariable3D& Phi, SpaceVariable3D& R, double time);
   bool Reinitialize(double time, double dt, int time_step, SpaceVariable3D& Phi, int special_maxIts = 0,
                     bool must_do = false);
+  // LevelSetOperator.cpp:1328-1334: the narrow band around Phi's zero level
+  // set (scheme.ls_bandwidth), Phi cut off outside it
+  void ConstructNarrowBandInReinitializer(SpaceVariable3D& Phi);
   int GetMaterialID() const { return ctx_->scheme().ls_materialid; }
 
  private:

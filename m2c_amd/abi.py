@@ -35,6 +35,7 @@ OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUP
 C_NAMES = {"EOS_SG": "M2CX_EOS_SG", "LIM_GMM": "M2CX_LIM_GENERALIZED_MINMOD", "TI_FE": "M2CX_TI_FORWARD_EULER"}
 
 DBL_MAX = 1.7976931348623157e308
+INT32_MAX = 2147483647
 
 
 class Material(C.Structure):
@@ -71,7 +72,7 @@ class Scheme(C.Structure):
                 ("riem_guess", C.c_int32), ("mp_phase_dir", C.c_int32),
                 ("reinit_frequency", C.c_int32), ("reinit_maxits", C.c_int32),
                 ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
-                ("reinit_first_layer", C.c_int32), ("reserved2", C.c_int32), ("time_step", C.c_double),
+                ("reinit_first_layer", C.c_int32), ("ls_bandwidth", C.c_int32), ("time_step", C.c_double),
                 ("ls_solver", C.c_int32), ("ls_flux", C.c_int32), ("ls_delta", C.c_double),
                 ("ls_rec_type", C.c_int32), ("ls_limiter", C.c_int32), ("ls_gmm_coeff", C.c_double)]
 
@@ -155,6 +156,7 @@ def default_scheme(**kw):
     s.reinit_cfl = 0.8
     s.reinit_tol = 2.0e-4
     s.reinit_first_layer = REINIT_FIXED
+    s.ls_bandwidth = INT32_MAX
     s.time_step = 0.0
     s.ls_solver = LSSOLVER_FINITE_DIFFERENCE
     s.ls_flux = LSFLUX_ROE

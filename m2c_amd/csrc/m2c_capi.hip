@@ -16,6 +16,7 @@
 #include "m2c_kernels.h"
 #include "m2c_multiphase.h"
 #include "m2c_reinit.h"
+#include "m2c_band.h"
 #include "m2c_mlp.h"
 #include <algorithm>
 #include <random>
@@ -74,6 +75,7 @@ struct m2cx_solver {
   // level-set reinitialization
   RBuf rb{};
   double reinit_cfl = 0.8;  // the reinitializer's CFL, lowered by its failsafe
+  NBBuf nb{};               // narrow-band level set (nb.bw == 0: full domain)
   // number of the step being taken when the host knows it without reading
   // the device (run_steps counts from one read; advance_one_time_step is
   // given it); -1 otherwise
@@ -190,6 +192,10 @@ static int grid_blocks(long n, int bs = 256) {
 }
 
 // Reconstructor::CalculateSlopeLimiterCoefficientK (Reconstructor.cpp:103-121)
+// launch width of the narrow-band list kernels (grid-stride over a count
+// that lives on the device)
+static int nb_blocks(const m2cx_solver* h) { return (int)std::min<long>(blocks_for(h->ncells), 2048L); }
+
 static int slope_k(int limiter, double A, double B) {
   if (limiter != M2CX_LIM_VANALBADA) return 0;
   int k;
@@ -269,6 +275,12 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   // level set's material (CalculateGradPhiAtCellInterface asserts it)
   if (sch->riemann_normal != M2CX_RNORMAL_MESH && (sch->n_levelsets != 1 || nmat != 2 || sch->ls_materialid != 1))
     return M2CX_E_UNSUPPORTED;
+  // narrow band (LevelSetOperator.cpp:84-99): bandwidth at least 2 and the
+  // reinitialization frequency, in steps, below it; finite differences only
+  const int bw = (sch->n_levelsets > 0 && sch->ls_bandwidth > 0 && sch->ls_bandwidth < INT32_MAX)
+                     ? sch->ls_bandwidth : 0;
+  if (bw && (bw <= 1 || sch->reinit_frequency <= 0 || sch->reinit_frequency >= bw)) return M2CX_E_ARG;
+  if (bw && sch->ls_solver == M2CX_LSSOLVER_FINITE_VOLUME) return M2CX_E_UNSUPPORTED;
   auto* h = new m2cx_solver;
   *out = h;
   h->grid = *grid;
@@ -395,6 +407,13 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (h->multimat) {
     int rc = mp_create(h->mp, g, *sch, ax, h->stream);
     if (rc) return fail(h, rc, "multi-material buffers: " + std::string(hipGetErrorString(hipGetLastError())));
+    if (bw) {
+      h->nb.bw = bw;
+      const m2cx_grid& gr = *grid;
+      h->nb.diag = std::sqrt((gr.xmax - gr.x0) * (gr.xmax - gr.x0) + (gr.ymax - gr.y0) * (gr.ymax - gr.y0) +
+                             (gr.zmax - gr.z0) * (gr.zmax - gr.z0));
+      if (int rc2 = nb_alloc(h->nb, h->ncells, h->stream)) return fail(h, rc2, "narrow-band buffers");
+    }
     LSFvm& fv = h->mp.fv;
     fv.on = sch->ls_solver == M2CX_LSSOLVER_FINITE_VOLUME;
     fv.flux = sch->ls_flux;
@@ -440,6 +459,7 @@ int m2cx_destroy(m2cx_solver* h) {
   hipFree(h->staging); hipFree(h->staging_i); hipFree(h->dt_part); hipFree(h->cnt);
   if (h->multimat) mp_destroy(h->mp);
   ri_free(h->rb);
+  nb_free(h->nb);
   if (h->tflag) hipFree(h->tflag);
   if (h->stream2) hipStreamSynchronize(h->stream2);
   if (h->ev_fork) hipEventDestroy(h->ev_fork);
@@ -728,11 +748,30 @@ static void interface_stage(m2cx_solver* h, int vin, int stage1) {
   toc(h, "riemann", e0);
 }
 
+// the level-set residual in the band, then (which >= 0) the RK combination
+// on its useful nodes inside the domain and the boundary fill of the field
+// it wrote (LevelSetOperator.cpp:880-955, 152-196, 408-508)
+static void levelset_band(m2cx_solver* h, CF5 V, int which) {
+  MPBuffers& B = h->mp;
+  const int LB = nb_blocks(h);
+  LSArgs a = mp_ls_args(B, h->g, V, which, h->cnt);
+  k_nb_ls_residual<<<LB, 256, 0, h->stream>>>(a, h->nb);
+  if (which < 0) return;
+  k_nb_ls_combine<<<LB, 256, 0, h->stream>>>(h->g, h->nb, which, B.phi, B.phi1, B.rls, h->cnt);
+  double* tgt = (which == 1 || which == 3) ? B.phi1 : B.phi;
+  const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
+  k_ls_bc<<<blocks_for(nf), 256, 0, h->stream>>>(h->g, tgt, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3], B.lsbc[4],
+                                                B.lsbc[5]);
+}
+
 static void levelset_stage(m2cx_solver* h, int vin, int which) {
   if (!h->multimat || h->sch.n_levelsets == 0) return;
   hipEvent_t e0{};
   tic(h, "levelset", &e0);
-  mp_levelset(h->mp, h->g, cf5(h, h->V[vin]), which, h->cnt, h->stream);
+  if (h->nb.bw)
+    levelset_band(h, cf5(h, h->V[vin]), which);
+  else
+    mp_levelset(h->mp, h->g, cf5(h, h->V[vin]), which, h->cnt, h->stream);
   toc(h, "levelset", e0);
 }
 
@@ -879,7 +918,7 @@ static int reinit_full(m2cx_solver* h, int special_maxits) {
         k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
         if (hcr) {
           k_rcorr_f<<<lb, 256, 0, st>>>(g, b, phi, flt, tol);
-          k_rcorr_apply<<<lb, 256, 0, st>>>(g, b, phi, cfl, tol);
+          k_rcorr_apply<<<lb, 256, 0, st>>>(g, b, phi, cfl, tol, 1);
           k_rbc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], b.st, tol);
         }
         k_rcheck<<<1, 1, 0, st>>>(b.st, tol);
@@ -904,6 +943,126 @@ static int reinit_full(m2cx_solver* h, int special_maxits) {
   }
   toc(h, "reinit", e0);
   return 0;
+}
+
+// LevelSetOperator::ConstructNarrowBandInReinitializer (LevelSetOperator.cpp:
+// 1328-1334) with ConstructNarrowBand (LevelSetReinitializer.cpp:1479-1592)
+static int nb_build(m2cx_solver* h) {
+  nb_construct(h->nb, h->g, h->mp.phi, h->stream);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+// LevelSetReinitializer::ReinitializeInBand (LevelSetReinitializer.cpp:
+// 181-300) on level set 0: the running bounds, the tags of the band, the new
+// band around them, then the iterations of the full-domain method restricted
+// to the band (chunks of iterations behind the device's done flag).
+static int reinit_band(m2cx_solver* h, int special_maxits) {
+  MPBuffers& B = h->mp;
+  NBBuf& nb = h->nb;
+  if (!nb.built) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  const int flt = h->sch.reinit_first_layer;
+  const bool cr = flt == M2CX_REINIT_CONSTRAINED1 || flt == M2CX_REINIT_CONSTRAINED2;
+  const bool hcr = flt == M2CX_REINIT_ITERATIVE_CONSTRAINED1 || flt == M2CX_REINIT_ITERATIVE_CONSTRAINED2;
+  const int fix = (flt == M2CX_REINIT_FIXED || cr) ? 1 : 0;
+  if (int rc = ri_alloc(h->rb, h->ncells, B.list_cap, cr, h->stream)) return fail(h, rc, "reinitialization buffers");
+  RBuf& b = h->rb;
+  const DevGrid& g = h->g;
+  hipStream_t st = h->stream;
+  const long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+  const int* bc = B.lsbc;
+  double* phi = B.phi;
+  const int LB = nb_blocks(h);
+  const NBSkip none{nullptr, 0.0, false};
+  hipEvent_t e0{};
+  tic(h, "reinit", &e0);
+  // UpdatePhiMaxAndPhiMinInBand (:1459-1476)
+  k_nb_keys<<<1, 1, 0, st>>>(nb, 0);
+  k_nb_minmax<<<LB, 256, 0, st>>>(g, phi, nb, 1);
+  k_nb_bounds<<<1, 1, 0, st>>>(nb, 0);
+  k_rstate_reset<<<1, 1, 0, st>>>(b.st, 1);
+  k_nb_rtag<<<LB, 256, 0, st>>>(g, phi, b, nb);
+  ReinitState hs;
+  HIPCHK(h, hipMemcpyAsync(&hs, b.st, sizeof hs, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  if (hs.nfl == 0) {  // no interface (TagFirstLayerNodesInBand returns false)
+    toc(h, "reinit", e0);
+    return 0;
+  }
+  if (hs.nfl > b.cap) return fail(h, M2CX_E_CAPACITY, "first layer of the level set exceeds the list capacity");
+  h->reinit_calls++;
+  // UpdateNarrowBand (:1777-1886)
+  k_nb_reseed<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, b.tag, nb);
+  nb_propagate(nb, g, st);
+  k_nb_leave_enter<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, nb);
+  k_nb_rsign<<<LB, 256, 0, st>>>(g, phi, b.sign, nb);
+  const int lb = blocks_for(hs.nfl);
+  if (cr) {
+    k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 1, b.pa, 0.0, phi, 1.0, nullptr, 0.0, none);
+    k_nb_phig2<<<LB, 256, 0, st>>>(nb, b.pa);
+    k_rfirst<<<lb, 256, 0, st>>>(g, b, phi, nb.phig2);
+    k_nb_rnormal<<<LB, 256, 0, st>>>(g, b, nb, phi);
+    k_rnormal_fl<<<lb, 256, 0, st>>>(b);
+    k_rcurv<<<lb, 256, 0, st>>>(g, b, phi, nb.phig2, flt);
+    k_rapply<<<lb, 256, 0, st>>>(b, phi);
+    k_nb_bc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nb.useful, none);
+  }
+  k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, b.phibk, 0.0, phi, 1.0, nullptr, 0.0, none);
+  const double tol = h->sch.reinit_tol;
+  for (;;) {  // RETRY_NarrowBand
+    const int maxits = special_maxits > 0 ? special_maxits : h->sch.reinit_maxits;
+    const double cfl = h->reinit_cfl;
+    k_rstate_reset<<<1, 1, 0, st>>>(b.st, 0);
+    const NBSkip s0{b.st, tol, false}, s1{b.st, tol, true};
+    bool done = false;
+    hs.iters = 0;
+    hs.last_dphi = 0.0;
+    for (int launched = 0; launched < maxits;) {
+      const int chunk = std::min(REINIT_CHUNK, maxits - launched);
+      for (int q = 0; q < chunk; q++) {
+        k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, b.phi0, 0.0, phi, 1.0, nullptr, 0.0, s0);
+        k_nb_rres<<<LB, 256, 0, st>>>(g, b, nb, phi, fix, cfl, 1, s0);
+        k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, b.pa, 0.0, phi, 1.0, nb.r, 1.0, s1);
+        k_nb_bc<<<blocks_for(nf), 256, 0, st>>>(g, b.pa, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nb.useful, s1);
+        k_nb_rres<<<LB, 256, 0, st>>>(g, b, nb, b.pa, fix, cfl, 0, s1);
+        k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, b.pa, 0.25, phi, 0.75, nb.r, 0.25, s1);
+        k_nb_bc<<<blocks_for(nf), 256, 0, st>>>(g, b.pa, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nb.useful, s1);
+        k_nb_rres<<<LB, 256, 0, st>>>(g, b, nb, b.pa, fix, cfl, 0, s1);
+        k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, phi, 1.0 / 3.0, b.pa, 2.0 / 3.0, nb.r, 2.0 / 3.0, s1);
+        k_nb_bc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nb.useful, s1);
+        if (hcr) {
+          k_rcorr_f<<<lb, 256, 0, st>>>(g, b, phi, flt, tol);
+          k_rcorr_apply<<<lb, 256, 0, st>>>(g, b, phi, cfl, tol, 0);
+          k_nb_bc<<<blocks_for(nf), 256, 0, st>>>(g, phi, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nb.useful, s1);
+        }
+        k_nb_relerr<<<LB, 256, 0, st>>>(g, b, nb, phi, s1);
+        k_rcheck<<<1, 1, 0, st>>>(b.st, tol);
+      }
+      launched += chunk;
+      HIPCHK(h, hipGetLastError());
+      HIPCHK(h, hipMemcpyAsync(&hs, b.st, sizeof hs, hipMemcpyDeviceToHost, st));
+      HIPCHK(h, hipStreamSynchronize(st));
+      if (hs.done) {
+        done = true;
+        break;
+      }
+    }
+    h->reinit_iters += hs.iters;
+    if (done || hs.last_dphi < 0.02) break;  // converged, or close enough (:274-279)
+    if (h->reinit_cfl < 0.02) {
+      toc(h, "reinit", e0);
+      return fail(h, M2CX_E_REINIT, "level-set reinitialization failed to converge");
+    }
+    h->reinit_cfl /= 1.5;
+    k_nb_axpy<<<LB, 256, 0, st>>>(g, nb, 0, phi, 0.0, b.phibk, 1.0, nullptr, 0.0, none);
+  }
+  toc(h, "reinit", e0);
+  return 0;
+}
+
+// the reinitialization over the full domain or in the band
+static int reinitialize(m2cx_solver* h, int special_maxits) {
+  return h->nb.bw ? reinit_band(h, special_maxits) : reinit_full(h, special_maxits);
 }
 
 // isTimeToWrite (Utils.cpp:337-357) for the reinitialization frequency
@@ -940,7 +1099,7 @@ static int end_of_step(m2cx_solver* h) {
     due = reinit_due(h, c.t, c.dts, c.time_step);
   }
   if (due)
-    if (int rc = reinit_full(h, 0)) return rc;
+    if (int rc = reinitialize(h, 0)) return rc;
   mp_end_states(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   return 0;
@@ -988,7 +1147,12 @@ static void tile_flags(m2cx_solver* h) {
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
 static int step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
-  if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream);
+  if (h->nb.bw && !h->nb.built) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  // Phi_tmp = Phi: over the whole brick, with the narrow band only on its
+  // useful nodes after the first step (TimeIntegrator.cpp:333-339)
+  const bool band_copy = h->nb.bw && h->host_time_step > 1;
+  if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream, !band_copy);
+  if (band_copy) k_nb_copy_useful<<<nb_blocks(h), 256, 0, h->stream>>>(h->nb, h->mp.phin, h->mp.phi);
   if (h->overlap) tile_flags(h);
   if (ti == M2CX_TI_FORWARD_EULER && h->overlap) {
     split_stage(h, StageSpec{0, 1, MODE_FE, 1, 0, -1});
@@ -1100,7 +1264,7 @@ int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits) {
   if (!reinit_enabled(h) || !h->multimat)
     return fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
   HIPCHK(h, hipSetDevice(h->device));
-  if (int rc = reinit_full(h, special_maxits)) return rc;
+  if (int rc = reinitialize(h, special_maxits)) return rc;
   HIPCHK(h, hipGetLastError());
   return check_error(h);
 }
@@ -1451,8 +1615,13 @@ int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int ep
 }
 
 int m2cx_levelset_residual(m2cx_solver* h, int v_field) {
-  if (!h || !h->mp.phi) return M2CX_E_ARG;
-  mp_levelset(h->mp, h->g, cf5(h, h->V[v_field]), -1, h->cnt, h->stream);
+  if (!h || !h->mp.phi || v_field < 0 || v_field > 2) return M2CX_E_ARG;
+  HIPCHK(h, hipSetDevice(h->device));
+  if (h->nb.bw && !h->nb.built) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  if (h->nb.bw)
+    levelset_band(h, cf5(h, h->V[v_field]), -1);
+  else
+    mp_levelset(h->mp, h->g, cf5(h, h->V[v_field]), -1, h->cnt, h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -1913,11 +2082,43 @@ int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id,
     const MPBuffers& B = h->mp;
     k_ls_bc<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, phi, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3],
                                                     B.lsbc[4], B.lsbc[5]);
+    // then the narrow band (SpaceInitializer.cpp:1787-1793)
+    if (h->nb.bw)
+      if (int rc = nb_build(h)) return rc;
   }
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   hipFree(dG);
   hipFree(dV0);
+  return 0;
+}
+
+int m2cx_levelset_construct_narrow_band(m2cx_solver* h) {
+  if (!h) return M2CX_E_ARG;
+  if (!h->nb.bw) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band: ls_bandwidth is not set");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (int rc = nb_build(h)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int m2cx_get_narrow_band(m2cx_solver* h, int32_t* level, uint8_t* useful, uint8_t* active, double* bounds) {
+  if (!h) return M2CX_E_ARG;
+  if (!h->nb.bw || !h->nb.built) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band constructed");
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const long n = h->ncells;
+  if (level) HIPCHK(h, hipMemcpy(level, h->nb.level, n * 4, hipMemcpyDeviceToHost));
+  if (useful) HIPCHK(h, hipMemcpy(useful, h->nb.useful, n, hipMemcpyDeviceToHost));
+  if (active) HIPCHK(h, hipMemcpy(active, h->nb.active, n, hipMemcpyDeviceToHost));
+  if (bounds) {
+    NBState st;
+    HIPCHK(h, hipMemcpy(&st, h->nb.st, sizeof st, hipMemcpyDeviceToHost));
+    bounds[0] = st.phi_max;
+    bounds[1] = st.phi_min;
+    bounds[2] = st.phi_out_pos;
+    bounds[3] = st.phi_out_neg;
+  }
   return 0;
 }
 

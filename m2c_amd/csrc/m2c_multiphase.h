@@ -828,8 +828,10 @@ __device__ __forceinline__ double ls_fvm_rhs(const LSArgs& a, const double* __re
 
 // LevelSetOperator::ApplyBoundaryConditions (LevelSetOperator.cpp:408-508);
 // LevelSetReinitializer::ApplyBoundaryConditions (LevelSetReinitializer.cpp:
-// 1249-1360) fills the same face ghosts by the same rules.
-__device__ void ls_bc_faces(const DevGrid& g, double* phi, const int* bcs) {
+// 1249-1360) fills the same face ghosts by the same rules, with the narrow
+// band only those `useful` marks.
+__device__ void ls_bc_faces(const DevGrid& g, double* phi, const int* bcs,
+                            const unsigned char* __restrict__ useful = nullptr) {
   long nfaces = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nfaces; q += (long)gridDim.x * blockDim.x) {
     long r = q;
@@ -844,6 +846,7 @@ __device__ void ls_bc_faces(const DevGrid& g, double* phi, const int* bcs) {
     else if (side == 4) kk = 0; else kk = g.NZ - 1;
     const int bc = bcs[side];
     long c = gidx(g, i, j, k);
+    if (useful && !useful[c]) continue;
     double im = phi[gidx(g, ii, jj, kk)];
     double ph = phi[c];
     if (bc == M2CX_LSBC_ZERO_NEUMANN) {
@@ -1424,16 +1427,18 @@ __global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ 
   for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) dst[c] = src[c];
 }
 
+// copy_phi: Phi_tmp = Phi over the ghosted brick (the narrow band copies
+// its useful nodes itself after the first step)
 inline void mp_begin_step(MPBuffers& B, const DevGrid& g, const Scheme& s, const unsigned char* ID, DevCounters* cnt,
-                          hipStream_t st) {
+                          hipStream_t st, bool copy_phi = true) {
   mp_classify(B, g, s, ID, cnt, st);
   k_copy_u8<<<blocks_for(g.ncells), 256, 0, st>>>(B.IDn, ID, g.ncells);
-  if (B.nls) k_copy_f64<<<blocks_for(g.ncells), 256, 0, st>>>(B.phin, B.phi, g.ncells);
+  if (B.nls && copy_phi) k_copy_f64<<<blocks_for(g.ncells), 256, 0, st>>>(B.phin, B.phi, g.ncells);
 }
 
-// which: -1 residual only, 0 FE, 1 RK stage 1, 2 RK2 final, 3 RK3 stage 2, 4 RK3 final
-inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCounters* cnt, hipStream_t st) {
-  if (!B.nls) return;
+// the level-set kernel's arguments; the stencil field of stage `which` is
+// phi (stages -1, 0, 1) or phi1 (2, 3, 4)
+inline LSArgs mp_ls_args(const MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCounters* cnt) {
   LSArgs a{};
   a.g = g;
   a.fv = B.fv;
@@ -1444,6 +1449,14 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
   a.which = which;
   a.cnt = cnt;
   a.rls = B.rls;
+  a.phi_st = (which <= 1) ? B.phi : B.phi1;
+  return a;
+}
+
+// which: -1 residual only, 0 FE, 1 RK stage 1, 2 RK2 final, 3 RK3 stage 2, 4 RK3 final
+inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCounters* cnt, hipStream_t st) {
+  if (!B.nls) return;
+  LSArgs a = mp_ls_args(B, g, V, which, cnt);
   double* written = nullptr;
   switch (which) {
     case -1: a.phi_st = B.phi; break;

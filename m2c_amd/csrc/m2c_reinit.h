@@ -490,7 +490,9 @@ __global__ void k_rcorr_f(DevGrid g, RBuf b, const double* __restrict__ phi, int
   }
 }
 
-__global__ void k_rcorr_apply(DevGrid g, RBuf b, double* __restrict__ phi, double cfl, double tol) {
+// acc: fold the relative change of the corrected nodes into the iteration's
+// maximum (the full-domain stage 3 leaves the first layer to this kernel)
+__global__ void k_rcorr_apply(DevGrid g, RBuf b, double* __restrict__ phi, double cfl, double tol, int acc) {
   if (reinit_skip(b.st, tol, true)) return;
   const int n = min(b.st->nfl, b.cap);
   double m = 0.0;
@@ -510,7 +512,7 @@ __global__ void k_rcorr_apply(DevGrid g, RBuf b, double* __restrict__ phi, doubl
     const double e = rel_change(phi[c], b.phi0[c]);
     m = (m < e) ? e : m;
   }
-  block_max_atomic(&b.st->dphi_bits, m);
+  if (acc) block_max_atomic(&b.st->dphi_bits, m);
 }
 
 // end of one iteration (:118-128, :157-164): stop on a small residual before
@@ -542,9 +544,12 @@ inline int ri_alloc(RBuf& b, long ncells, int cap, bool normals, hipStream_t st)
         hipMalloc(&b.fl_r0, (long)cap * 8) != hipSuccess || hipMalloc(&b.fl_f, (long)cap * 8) != hipSuccess ||
         hipMalloc(&b.fl_n, (long)cap * 3 * 8) != hipSuccess || hipMalloc(&b.st, sizeof(ReinitState)) != hipSuccess)
       return M2CX_E_OOM;
-    // zero once: the fields are read by the first 0*x + y copies
+    // zero once: the fields are read by the first 0*x + y copies, and the
+    // narrow band keeps the tags of nodes outside it (Tag starts at 0,
+    // LevelSetReinitializer.cpp:37)
     for (double* f : {b.sign, b.phibk, b.phi0, b.pa, b.pb})
       if (hipMemsetAsync(f, 0, ncells * 8, st) != hipSuccess) return M2CX_E_HIP;
+    if (hipMemsetAsync(b.tag, 0, ncells, st) != hipSuccess) return M2CX_E_HIP;
     b.cap = cap;
   }
   if (normals && !b.normal) {

@@ -267,6 +267,13 @@ bool LevelSetOperator::Reinitialize(double time, double dt, int time_step, Space
   return true;
 }
 
+void LevelSetOperator::ConstructNarrowBandInReinitializer(SpaceVariable3D& Phi) {
+  Context& c = *ctx_;
+  Phi.ToDevice(c, M2CX_FIELD_PHI);
+  c.check(m2cx_levelset_construct_narrow_band(c.handle()));
+  Phi.ProducedOnDevice(c, M2CX_FIELD_PHI);
+}
+
 // ---------------------------------------------------------- TimeIntegrator
 
 void TimeIntegratorBase::AdvanceOneTimeStep(SpaceVariable3D& V, SpaceVariable3D& ID,

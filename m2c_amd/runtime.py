@@ -51,6 +51,8 @@ def lib():
         L.m2cx_levelset_residual.argtypes = [P, C.c_int]
         L.m2cx_download_ls_residual.argtypes = [P, D]
         L.m2cx_levelset_reinitialize.argtypes = [P, C.c_int]
+        L.m2cx_levelset_construct_narrow_band.argtypes = [P]
+        L.m2cx_get_narrow_band.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), D]
         L.m2cx_set_overlap.argtypes = [P, C.c_int]
         L.m2cx_advance_one_time_step.argtypes = [P, C.c_double, C.c_double, C.c_int]
         L.m2cx_run_steps.argtypes = [P, C.c_int]
@@ -190,6 +192,24 @@ class Solver:
         """LevelSetOperator::Reinitialize with must_do = true: level set 0
         back to a signed distance (scheme.reinit_* settings)."""
         return self._check(self.L.m2cx_levelset_reinitialize(self.h, special_maxits))
+
+    def construct_narrow_band(self):
+        """LevelSetOperator::ConstructNarrowBandInReinitializer: the band of
+        level set 0 around its zero level set, phi cut off outside it (after
+        set_state; initialize() builds it itself)."""
+        return self._check(self.L.m2cx_levelset_construct_narrow_band(self.h))
+
+    def narrow_band(self):
+        """level (int32), useful and active (bool) per ghosted cell, and
+        (phi_max, phi_min, phi_out_pos, phi_out_neg)"""
+        level = np.zeros(self.shape, dtype=np.int32)
+        useful = np.zeros(self.shape, dtype=np.uint8)
+        active = np.zeros(self.shape, dtype=np.uint8)
+        b = np.zeros(4)
+        U8 = C.POINTER(C.c_uint8)
+        self._check(self.L.m2cx_get_narrow_band(self.h, level.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                useful.ctypes.data_as(U8), active.ctypes.data_as(U8), _d(b)))
+        return level, useful.astype(bool), active.astype(bool), tuple(b)
 
     # ---- TimeIntegrator / driver ----
     def advance_one_time_step(self, time, dt, step):

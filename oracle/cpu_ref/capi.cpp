@@ -144,6 +144,34 @@ int orc_reinitialize(orc_solver* h, int special_maxits) {
 
 double orc_reinit_cfl(orc_solver* h) { return h->s->reinit_cfl; }
 
+// LevelSetOperator::ConstructNarrowBandInReinitializer (LevelSetOperator.cpp:
+// 1328-1334), after the level set and its boundary values are set
+int orc_construct_narrow_band(orc_solver* h) {
+  Solver& s = *h->s;
+  if (!s.nb_bw) return -2;
+  s.ConstructNarrowBand();
+  return s.fatal ? -1 : 0;
+}
+
+// The band of level set 0: level per ghosted cell (31415927 outside), the
+// useful and active masks, and phi_max, phi_min, phi_out_pos, phi_out_neg.
+int orc_get_narrow_band(orc_solver* h, int* level, unsigned char* useful, unsigned char* active,
+                        double* bounds) {
+  Solver& s = *h->s;
+  if (!s.nb_built) return -2;
+  const long n = s.m.ncells();
+  if (level) std::memcpy(level, s.nbLevel.data(), n * 4);
+  if (useful) std::memcpy(useful, s.nbUseful.data(), n);
+  if (active) std::memcpy(active, s.nbActive.data(), n);
+  if (bounds) {
+    bounds[0] = s.phi_max;
+    bounds[1] = s.phi_min;
+    bounds[2] = s.phi_out_pos;
+    bounds[3] = s.phi_out_neg;
+  }
+  return 0;
+}
+
 // Prescribed-velocity level-set test 1, 2 or 3 (0: off)
 int orc_set_ls_test(orc_solver* h, int test) {
   h->s->ls_test = test;

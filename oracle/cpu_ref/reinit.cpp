@@ -31,7 +31,10 @@ bool Solver::Reinitialize(double time, double dt, int time_step, int special_max
   else
     due = sch.reinit_frequency > 0 && time_step % sch.reinit_frequency == 0;
   if (!due) return false;
-  ReinitializeFullDomain(special_maxIts);
+  if (nb_bw)
+    ReinitializeInBand(special_maxIts);
+  else
+    ReinitializeFullDomain(special_maxIts);
   return true;
 }
 
@@ -125,11 +128,15 @@ static double differentiate_first_layer(double x0, double x1, double x2, double 
 // layer with one-sided or central differences chosen by the tags (Hartmann et
 // al. 2008, Eq. 21a), then the CR-1 / CR-2 averaging on the nodes whose
 // curvature points away from the interface.  The copy of phi with two ghost
-// layers (PhiG2) only ever holds the first ghost layer in the full-domain
-// method; its second layer stays zero, which the reference reads in y at the
-// top boundary (its `j+2<=NY` test has j+2 on both branches).
+// layers (PhiG2) only ever holds the first ghost layer; its second layer
+// stays zero, which the reference reads in y at the top boundary (its
+// `j+2<=NY` test has j+2 on both branches).  PopulatePhiG2 (:877-902) copies
+// every node over the full domain, only the useful nodes with the narrow band
+// (PhiG2 keeps its older values elsewhere).
 void Solver::ReinitializeFirstLayerNodes(std::vector<FirstLayerNode>& fl) {
-  const std::vector<double>& phig = rPhi1;  // copy of Phi including ghosts
+  if (nb_bw)
+    for (long c : useful_nodes) rPhiG2[c] = rPhi1[c];
+  const std::vector<double>& phig = nb_bw ? rPhiG2 : rPhi1;  // copy of Phi including ghosts
   auto G = [&](int i, int j, int k) {
     if (i < -1 || i > m.NX || j < -1 || j > m.NY || k < -1 || k > m.NZ) return 0.0;
     return phig[m.idx(i, j, k)];
@@ -223,10 +230,61 @@ void Solver::ReinitializeFirstLayerNodes(std::vector<FirstLayerNode>& fl) {
   }
 }
 
-// ComputeNormalDirectionBeyondFirstLayer, full-domain branch (:1136-1245)
+// The unit normal at one interior node of the narrow band (the narrow-band
+// branches of ComputeNormalDirectionBeyondFirstLayer, :1155-1208, and of
+// LevelSetOperator::ComputeNormalDirectionCentralDifferencing_NarrowBand,
+// LevelSetOperator.cpp:1545-1592): central differences where both
+// neighbours are useful -- in y the reference tests the upper neighbour twice,
+// so the lower one may be outside the band -- one-sided where one of them is,
+// zero where neither is; divided by its norm when that is nonzero.
+void Solver::BandNormal(const std::vector<double>& phi, int i, int j, int k, double* nv) const {
+  const long c = m.idx(i, j, k);
+  const double p = phi[c];
+  auto P = [&](int a, int b, int e) { return phi[m.idx(a, b, e)]; };
+  if (UsefulAt(i + 1, j, k) && UsefulAt(i - 1, j, k))
+    nv[0] = central_diff(P(i - 1, j, k), p, P(i + 1, j, k), m.X(i - 1), m.X(i), m.X(i + 1));
+  else if (UsefulAt(i + 1, j, k))
+    nv[0] = (P(i + 1, j, k) - p) / (m.X(i + 1) - m.X(i));
+  else if (UsefulAt(i - 1, j, k))
+    nv[0] = (p - P(i - 1, j, k)) / (m.X(i) - m.X(i - 1));
+  else
+    nv[0] = 0.0;
+  if (UsefulAt(i, j + 1, k) && UsefulAt(i, j + 1, k))
+    nv[1] = central_diff(P(i, j - 1, k), p, P(i, j + 1, k), m.Y(j - 1), m.Y(j), m.Y(j + 1));
+  else if (UsefulAt(i, j + 1, k))
+    nv[1] = (P(i, j + 1, k) - p) / (m.Y(j + 1) - m.Y(j));
+  else if (UsefulAt(i, j - 1, k))
+    nv[1] = (p - P(i, j - 1, k)) / (m.Y(j) - m.Y(j - 1));
+  else
+    nv[1] = 0.0;
+  if (UsefulAt(i, j, k + 1) && UsefulAt(i, j, k - 1))
+    nv[2] = central_diff(P(i, j, k - 1), p, P(i, j, k + 1), m.Z(k - 1), m.Z(k), m.Z(k + 1));
+  else if (UsefulAt(i, j, k + 1))
+    nv[2] = (P(i, j, k + 1) - p) / (m.Z(k + 1) - m.Z(k));
+  else if (UsefulAt(i, j, k - 1))
+    nv[2] = (p - P(i, j, k - 1)) / (m.Z(k) - m.Z(k - 1));
+  else
+    nv[2] = 0.0;
+  const double nr = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  if (nr != 0.0) {
+    const double inv = 1.0 / nr;  // Vec3D::operator/= multiplies by the reciprocal
+    nv[0] *= inv; nv[1] *= inv; nv[2] *= inv;
+  }
+}
+
+// ComputeNormalDirectionBeyondFirstLayer (:1136-1245)
 void Solver::ComputeNormalDirectionBeyondFirstLayer(std::vector<FirstLayerNode>& fl) {
   for (auto& n : fl)
     for (int d = 0; d < 3; d++) rNormal[m.idx(n.i, n.j, n.k) * 3 + d] = n.nphi0[d];
+  if (nb_bw) {
+    for (long c : useful_nodes) {
+      int i, j, k;
+      m.ijk(c, i, j, k);
+      if (OutCount(i, j, k) || rTag[c] != 0) continue;
+      BandNormal(Phi, i, j, k, &rNormal[c * 3]);
+    }
+    return;
+  }
   for (int k = 0; k < m.NZ; k++)
     for (int j = 0; j < m.NY; j++)
       for (int i = 0; i < m.NX; i++) {
@@ -323,7 +381,7 @@ void Solver::ApplyCorrectionToFirstLayerNodes(std::vector<FirstLayerNode>& fl, d
     double dt = cfl * std::min(m.DX(i), std::min(m.DY(j), m.DZ(k)));
     Phi[m.idx(i, j, k)] += dt * beta * n.f;
   }
-  LSApplyBoundaryConditions(Phi);
+  ReinitApplyBoundaryConditions(Phi);
 }
 
 // CalculateMaximumRelativeErrorFullDomain (:1966-1994)
@@ -425,6 +483,23 @@ void Solver::PrescribeVelocityForTesting(std::vector<double>& Vio, const std::ve
             v[3] = 0.0;
           }
         }
+  } else if (ls_test == 3 && nb_bw) {
+    // the useful nodes only; the others keep their velocity (:1468-1483)
+    const double s = time < 1.0 ? 1.0 : -1.0;
+    for (long c : useful_nodes) {
+      int i, j, k;
+      m.ijk(c, i, j, k);
+      double* v = &Vio[c * 5];
+      if (OutCount(i, j, k)) {
+        v[1] = v[2] = v[3] = 0.0;
+        continue;
+      }
+      double nv[3];
+      BandNormal(phi, i, j, k, nv);
+      v[1] = s * nv[0];
+      v[2] = s * nv[1];
+      v[3] = 0.0;
+    }
   } else if (ls_test == 3) {
     const double s = time < 1.0 ? 1.0 : -1.0;
     for (int k = -1; k <= m.NZ; k++)

@@ -85,6 +85,20 @@ Solver::Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat_, const m
   ID.assign(n, 0);
   IDn.assign(n, 0);
   for (auto* f : {&Phi, &Phi1, &Phi_tmp, &Rls}) f->assign(n, 0.0);
+  reinit_cfl = s.reinit_cfl;  // cfl = iod_ls.reinit.cfl (LevelSetReinitializer.cpp:39)
+  // domain_diagonal (Main.cpp:207-209)
+  domain_diagonal = std::sqrt(std::pow(g.xmax - g.x0, 2) + std::pow(g.ymax - g.y0, 2) + std::pow(g.zmax - g.z0, 2));
+  phi_max = -domain_diagonal;
+  phi_min = domain_diagonal;
+  nb_bw = (s.n_levelsets > 0 && s.ls_bandwidth > 0 && s.ls_bandwidth < INT32_MAX) ? s.ls_bandwidth : 0;
+  if (nb_bw) {
+    // LevelSetOperator.cpp:84-99
+    if (nb_bw <= 1) Fail("in the narrow-band level set method, bandwidth should be at least 3");
+    else if (s.reinit_frequency <= 0 || s.reinit_frequency >= nb_bw)
+      Fail("the level set reinitialization frequency should be smaller than bandwidth");
+    else if (s.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME)
+      Fail("the narrow-band level set is built for the finite-difference level-set solver only");
+  }
 }
 
 // MeshGenerator.cpp:161-173, SpaceOperator.cpp:173-350
@@ -777,9 +791,10 @@ void Solver::ComputeTimeStepSize(const std::vector<double>& Vin, double& dt, dou
 // LevelSetOperator (LevelSetOperator.cpp:408-508, 796-870) with
 // GradientCalculatorFD3::CalculateFirstDerivativeAtNodes (:54-171)
 // ---------------------------------------------------------------------------
-void Solver::LSApplyBoundaryConditions(std::vector<double>& phi) {
+void Solver::LSApplyBoundaryConditions(std::vector<double>& phi, const std::vector<unsigned char>* useful) {
   for (auto& gp : ghosts) {
     int i = gp.i, j = gp.j, k = gp.k;
+    if (useful && !(*useful)[m.idx(i, j, k)]) continue;
     double& ph = phi[m.idx(i, j, k)];
     double im = phi[m.idx(gp.ii, gp.jj, gp.kk)];
     if (gp.lsbc == M2CX_LSBC_ZERO_NEUMANN) {
@@ -833,6 +848,14 @@ void Solver::LSComputeResidual(const std::vector<double>& Vin, const std::vector
                                std::vector<double>& res) {
   if (sch.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME) {
     LSComputeResidualFVM(Vin, phi, res);
+    return;
+  }
+  if (nb_bw) {
+    if (!nb_built) {
+      Fail("the narrow band of the level set has not been constructed");
+      return;
+    }
+    LSComputeResidualFDMNarrowBand(Vin, phi, res);
     return;
   }
   auto P = [&](int i, int j, int k) { return phi[m.idx(i, j, k)]; };
@@ -1063,7 +1086,11 @@ void Solver::FixUnresolvedNodes(std::vector<std::array<int, 3>>& unresolved) {
 void Solver::AdvanceOneTimeStep(double time, double dt) {
   (void)time;
   const bool ls = sch.n_levelsets > 0;
-  if (ls) Phi_tmp = Phi;  // Phi_tmp = Phi(n) (copy including ghosts)
+  // Phi_tmp = Phi(n) including ghosts; with the narrow band only the useful
+  // nodes after the first step (TimeIntegrator.cpp:333-339)
+  if (ls && (!nb_bw || time_step == 1)) Phi_tmp = Phi;
+  else if (ls)
+    for (long c : useful_nodes) Phi_tmp[c] = 0.0 * Phi_tmp[c] + 1.0 * Phi[c];
   if (sch.integrator == M2CX_TI_FORWARD_EULER) {
     ComputeResidual(V, R, true);
     if (fatal) return;
@@ -1076,12 +1103,7 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     if (ls) {
       if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
-      for (int k = 0; k < m.NZ; k++)
-        for (int j = 0; j < m.NY; j++)
-          for (int i = 0; i < m.NX; i++) {
-            long c = m.idx(i, j, k);
-            Phi[c] = 1.0 * Phi[c] + dt * Rls[c];
-          }
+      LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + dt * Rls[c]; });
       LSApplyBoundaryConditions(Phi);
     }
   } else {
@@ -1105,8 +1127,8 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     if (ls) {
       if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
-      interior([&](long c) { Phi1[c] = 0.0 * Phi1[c] + 1.0 * Phi[c]; });
-      interior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + dt * Rls[c]; });
+      LSInterior([&](long c) { Phi1[c] = 0.0 * Phi1[c] + 1.0 * Phi[c]; });
+      LSInterior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + dt * Rls[c]; });
       LSApplyBoundaryConditions(Phi1);
     }
     if (!rk3) {
@@ -1123,8 +1145,8 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (ls) {
         if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
-        interior([&](long c) { Phi[c] = 0.5 * Phi[c] + 0.5 * Phi1[c]; });
-        interior([&](long c) { Phi[c] = 1.0 * Phi[c] + hdt * Rls[c]; });
+        LSInterior([&](long c) { Phi[c] = 0.5 * Phi[c] + 0.5 * Phi1[c]; });
+        LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + hdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi);
       }
     } else {
@@ -1142,8 +1164,8 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (ls) {
         if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
-        interior([&](long c) { Phi1[c] = 0.25 * Phi1[c] + 0.75 * Phi[c]; });
-        interior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + qdt * Rls[c]; });
+        LSInterior([&](long c) { Phi1[c] = 0.25 * Phi1[c] + 0.75 * Phi[c]; });
+        LSInterior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + qdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi1);
       }
       // stage 3
@@ -1159,8 +1181,8 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (ls) {
         if (ls_test) PrescribeVelocityForTesting(V2, Phi1, time - 0.5 * dt);
         LSComputeResidual(V2, Phi1, Rls);
-        interior([&](long c) { Phi[c] = 1.0 / 3.0 * Phi[c] + 2.0 / 3.0 * Phi1[c]; });
-        interior([&](long c) { Phi[c] = 1.0 * Phi[c] + tdt * Rls[c]; });
+        LSInterior([&](long c) { Phi[c] = 1.0 / 3.0 * Phi[c] + 2.0 / 3.0 * Phi1[c]; });
+        LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + tdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi);
       }
     }

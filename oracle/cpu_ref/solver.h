@@ -113,7 +113,8 @@ class Solver {
   void ApplyBoundaryConditions(std::vector<double>& Vio);
   void ComputeTimeStepSize(const std::vector<double>& Vin, double& dt, double& cfl);
   // LevelSetOperator
-  void LSApplyBoundaryConditions(std::vector<double>& phi);
+  // with `useful`, only the ghosts it marks (the narrow-band reinitializer)
+  void LSApplyBoundaryConditions(std::vector<double>& phi, const std::vector<unsigned char>* useful = nullptr);
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   void LSComputeResidualFVM(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   // direction of the 1D Riemann problem at material interfaces
@@ -126,6 +127,17 @@ class Solver {
   bool Reinitialize(double time, double dt, int time_step, int special_maxIts, bool must_do);
   void ReinitializeFullDomain(int special_maxIts);
   double reinit_cfl = 0.8;  // the reinitializer's CFL, reduced by its failsafe
+  // Narrow-band level set (LevelSetOperator.cpp:84-99; LevelSetReinitializer.
+  // cpp:181-300, 440-598, 1045-1133, 1459-2031): bandwidth (0: full domain),
+  // the band levels, the useful / active masks and node lists, the list of
+  // useful nodes plus one layer, and the persistent phi bounds.
+  int nb_bw = 0;
+  bool nb_built = false;
+  std::vector<int> nbLevel;
+  std::vector<unsigned char> nbUseful, nbActive;
+  std::vector<long> useful_nodes, active_nodes, plus1_nodes;
+  double domain_diagonal = 0.0, phi_max = 0.0, phi_min = 0.0, phi_out_pos = 0.0, phi_out_neg = 0.0;
+  void ConstructNarrowBand();
   // MultiPhaseOperator
   void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
   void UpdateMaterialIDAtGhostNodes();
@@ -156,6 +168,51 @@ class Solver {
   double ComputeReinitResidual(const std::vector<double>& phi, double cfl);
   void ApplyCorrectionToFirstLayerNodes(std::vector<FirstLayerNode>& fl, double cfl);
   double MaxRelativeError();
+  // narrow band
+  std::vector<double> rPhiG2;  // PhiG2 of the narrow-band reinitializer (ghost width 1 here)
+  int OutCount(int i, int j, int k) const {
+    return (i < 0 || i >= m.NX) + (j < 0 || j >= m.NY) + (k < 0 || k >= m.NZ);
+  }
+  bool InGhosted(int i, int j, int k) const {
+    return i >= -1 && i <= m.NX && j >= -1 && j <= m.NY && k >= -1 && k <= m.NZ;
+  }
+  // UsefulG2 has two ghost layers; the second is never set (zero)
+  bool UsefulAt(int i, int j, int k) const { return InGhosted(i, j, k) && nbUseful[m.idx(i, j, k)] != 0; }
+  void NbAlloc();
+  void PropagateNarrowBand();
+  void CutOffPhiOutsideBand();
+  void CreateUsefulNodesPlusOneLayer();
+  void UpdateNarrowBand(const std::vector<long>& first_layer_inc_ghost);
+  void UpdatePhiMaxAndPhiMinInBand();
+  bool TagFirstLayerNodesInBand(std::vector<FirstLayerNode>& fl, std::vector<long>& flg);
+  void EvaluateSignFunctionInBand(double eps);
+  void AXPlusBYInBandPlusOne(double a, std::vector<double>& X, double b, const std::vector<double>& Y,
+                             bool work_on_ghost);
+  double ComputeReinitResidualInBand(const std::vector<double>& phi, double cfl);
+  double MaxRelativeErrorInBand();
+  void ReinitializeInBand(int special_maxIts);
+  // LevelSetReinitializer::ApplyBoundaryConditions (:1249-1360): with the
+  // narrow band only the useful ghosts
+  void ReinitApplyBoundaryConditions(std::vector<double>& phi);
+  void BandNormal(const std::vector<double>& phi, int i, int j, int k, double* nv) const;
+  void LSComputeResidualFDMNarrowBand(const std::vector<double>& Vin, const std::vector<double>& phi,
+                                      std::vector<double>& res);
+  // LevelSetOperator::AXPlusBY (LevelSetOperator.cpp:152-196): the interior,
+  // or with the narrow band the useful nodes of the interior
+  template <class F>
+  void LSInterior(F fn) {
+    if (nb_bw) {
+      for (long c : useful_nodes) {
+        int i, j, k;
+        m.ijk(c, i, j, k);
+        if (!OutCount(i, j, k)) fn(c);
+      }
+      return;
+    }
+    for (int k = 0; k < m.NZ; k++)
+      for (int j = 0; j < m.NY; j++)
+        for (int i = 0; i < m.NX; i++) fn(m.idx(i, j, k));
+  }
 
  public:
   // Prescribed-velocity level-set tests (LevelSetOperator.cpp:1398-1490,

@@ -1,14 +1,17 @@
-"""Convert the reference's full-domain level-set test results into fixtures.
+"""Convert the reference's level-set test results into fixtures.
 
 Reads (data only, nothing is executed) from the reference checkout:
-  Tests/LevelSetTests/{MergingCircles,SlottedDisk,VortexDeformation}/FullDomain/
+  Tests/LevelSetTests/{MergingCircles,SlottedDisk,VortexDeformation}/
+      {FullDomain,NarrowBand}/
     input.st                       deck (mesh, time step, reinitialization)
-    log.out                        number of steps
+    log.out                        number of steps and of reinitializations
     results/solution_0000.vtr      phi at t = 0 (PETSc VTK, appended raw FP64)
     results/solution_00NN.vtr      phi at the final time
 and writes tests/golden/levelset_tests.npz: the final phi of each test in
-full precision, and the SHA-256 of its t = 0 field (the tests regenerate that
-field from the analytic initial condition and check the digest).  Run:
+full precision, and the SHA-256 of its t = 0 field (the full-domain tests
+regenerate that field from the analytic initial condition and check the
+digest; in the narrow-band tests it is the field after the band is built and
+phi is cut off outside it).  Narrow-band keys carry the suffix "NB".  Run:
   python tests/golden/make_levelset_fixtures.py /root/reference
 """
 import glob
@@ -46,15 +49,20 @@ def read_vtr(path, name="levelset0"):
 def main(ref):
     out = {}
     for t in TESTS:
-        d = os.path.join(ref, "Tests", "LevelSetTests", t, "FullDomain")
-        snaps = sorted(glob.glob(os.path.join(d, "results", "solution_*.vtr")))
-        phi0 = read_vtr(snaps[0])
-        phin = read_vtr(snaps[-1])
-        steps = [int(m) for m in re.findall(r"^Step (\d+):", open(os.path.join(d, "log.out")).read(), re.M)]
-        out[t + "_final"] = phin[0]
-        out[t + "_initial_sha256"] = np.frombuffer(hashlib.sha256(phi0[0].tobytes()).digest(), np.uint8)
-        out[t + "_steps"] = np.int64(steps[-1])
-        print(t, phin.shape, "steps", steps[-1], os.path.basename(snaps[-1]))
+        for sub, suffix in (("FullDomain", ""), ("NarrowBand", "NB")):
+            d = os.path.join(ref, "Tests", "LevelSetTests", t, sub)
+            snaps = sorted(glob.glob(os.path.join(d, "results", "solution_*.vtr")))
+            phi0 = read_vtr(snaps[0])
+            phin = read_vtr(snaps[-1])
+            log = open(os.path.join(d, "log.out")).read()
+            steps = [int(m) for m in re.findall(r"^Step (\d+):", log, re.M)]
+            key = t + suffix
+            out[key + "_final"] = phin[0]
+            out[key + "_initial_sha256"] = np.frombuffer(hashlib.sha256(phi0[0].tobytes()).digest(), np.uint8)
+            out[key + "_steps"] = np.int64(steps[-1])
+            out[key + "_reinit_calls"] = np.int64(len(re.findall(r"^- Reinitializing the level set", log, re.M)))
+            print(key, phin.shape, "steps", steps[-1], "reinit", int(out[key + "_reinit_calls"]),
+                  os.path.basename(snaps[-1]))
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "levelset_tests.npz"), **out)
 
 

@@ -1,9 +1,9 @@
-"""The reference's full-domain level-set tests (Tests/LevelSetTests/*/
-FullDomain/input.st): prescribed velocity, FD3 advection with RK2, full-domain
-reinitialization at a fixed step frequency.  Initial conditions restate
-SpaceInitializer.cpp:1606-1700 (compiled there with LEVELSET_TEST = 1, 2, 3);
-the velocity fields are LevelSetOperator.cpp:1398-1490, restated in the
-oracle."""
+"""The reference's level-set tests (Tests/LevelSetTests/*/{FullDomain,
+NarrowBand}/input.st): prescribed velocity, FD3 advection with RK2,
+reinitialization at a fixed step frequency over the full domain or in a
+narrow band.  Initial conditions restate SpaceInitializer.cpp:1606-1700
+(compiled there with LEVELSET_TEST = 1, 2, 3); the velocity fields are
+LevelSetOperator.cpp:1398-1490, restated in the oracle."""
 import math
 
 import numpy as np
@@ -17,6 +17,16 @@ CASES = {
                         freq=104, maxits=1000, tol=1.0e-5, cfl=0.8, flt=abi.REINIT_FIXED),
     "VortexDeformation": dict(test=2, n=(256, 256, 1), lo=(0.0, 0.0, -2.0e-3), hi=(1.0, 1.0, 2.0e-3), dt=0.003,
                               tmax=8.0, freq=10, maxits=500, tol=1.0e-5, cfl=0.8,
+                              flt=abi.REINIT_ITERATIVE_CONSTRAINED2),
+}
+
+# the NarrowBand decks differ from the FullDomain ones in Bandwidth and the
+# Reinitialization block only
+CASES_NB = {
+    "MergingCircles": dict(CASES["MergingCircles"], bw=6, freq=2, maxits=500, tol=1.0e-4, flt=abi.REINIT_FIXED),
+    "SlottedDisk": dict(CASES["SlottedDisk"], bw=5, freq=2, maxits=1000, tol=1.0e-5,
+                        flt=abi.REINIT_ITERATIVE_CONSTRAINED1),
+    "VortexDeformation": dict(CASES["VortexDeformation"], bw=6, freq=2, maxits=100, tol=1.0e-4,
                               flt=abi.REINIT_ITERATIVE_CONSTRAINED2),
 }
 
@@ -76,15 +86,16 @@ def initial_phi(name):
     return out
 
 
-def setup(name):
+def setup(name, narrow_band=False):
     """grid, materials, scheme, V, ID, Phi (ghosted) of a test deck; the two
     materials and the flow state are placeholders (no flow is solved)."""
-    c = CASES[name]
+    c = (CASES_NB if narrow_band else CASES)[name]
     g = abi.grid(c["n"], c["lo"], c["hi"])
     mats = abi.materials(abi.material("SG", gamma=1.4), abi.material("SG", gamma=1.4))
     s = abi.default_scheme(integrator=abi.TI_RK2, n_levelsets=1, ls_materialid=1, max_time=c["tmax"],
                            time_step=c["dt"], reinit_frequency=c["freq"], reinit_maxits=c["maxits"],
-                           reinit_tol=c["tol"], reinit_cfl=c["cfl"], reinit_first_layer=c["flt"])
+                           reinit_tol=c["tol"], reinit_cfl=c["cfl"], reinit_first_layer=c["flt"],
+                           ls_bandwidth=c.get("bw", abi.INT32_MAX))
     for q, b in enumerate([abi.LSBC_LINEAR_EXTRAPOLATION] * 4 + [abi.LSBC_ZERO_NEUMANN] * 2):
         s.ls_bc[q] = b
     nx, ny, nz = c["n"]

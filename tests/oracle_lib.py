@@ -65,6 +65,8 @@ def lib():
         L.orc_reinit_cfl.restype = C.c_double
         L.orc_reinit_cfl.argtypes = [P]
         L.orc_set_ls_test.argtypes = [P, C.c_int]
+        L.orc_construct_narrow_band.argtypes = [P]
+        L.orc_get_narrow_band.argtypes = [P, I, C.POINTER(C.c_ubyte), C.POINTER(C.c_ubyte), D]
         L.orc_bimaterial_normal.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D]
         _lib = L
     return _lib
@@ -177,6 +179,23 @@ class Oracle:
         out = np.zeros(3)
         self.L.orc_bimaterial_normal(self.h, d, i, j, k, myid, neighborid, dp(out))
         return out
+
+    def construct_narrow_band(self):
+        rc = self.L.orc_construct_narrow_band(self.h)
+        if rc != 0:
+            raise RuntimeError("construct_narrow_band: %d %s" % (rc, self.error()))
+
+    def narrow_band(self):
+        """level (int32), useful and active (bool) per ghosted cell, and
+        (phi_max, phi_min, phi_out_pos, phi_out_neg)"""
+        level = np.zeros(self.shape, dtype=np.int32)
+        useful = np.zeros(self.shape, dtype=np.uint8)
+        active = np.zeros(self.shape, dtype=np.uint8)
+        b = np.zeros(4)
+        U8 = C.POINTER(C.c_ubyte)
+        assert self.L.orc_get_narrow_band(self.h, ip(level), useful.ctypes.data_as(U8), active.ctypes.data_as(U8),
+                                          dp(b)) == 0
+        return level, useful.astype(bool), active.astype(bool), tuple(b)
 
     def set_ls_test(self, test):
         """Prescribed-velocity level-set test 1 (slotted disk), 2 (vortex
