@@ -86,7 +86,9 @@ enum { M2CX_GUESS_ACOUSTIC = 0,   /* FindInitialFeasiblePointsByAcousticTheory *
 /* Error codes */
 enum { M2CX_OK = 0, M2CX_E_ARG = -1, M2CX_E_HIP = -2, M2CX_E_OOM = -3,
        M2CX_E_HYPERBOLICITY = -4, M2CX_E_RIEMANN_FATAL = -5, M2CX_E_UNRESOLVED = -6,
-       M2CX_E_UNSUPPORTED = -7, M2CX_E_CAPACITY = -8, M2CX_E_REINIT = -9 };
+       M2CX_E_UNSUPPORTED = -7, M2CX_E_CAPACITY = -8, M2CX_E_REINIT = -9,
+       M2CX_E_LEVELSETS = -10 };  /* a node inside two level-set subdomains
+                                     (MultiPhaseOperator.cpp:200-206 exits)  */
 
 /* ------------------------------------------------------------------ */
 /* Problem description                                                 */
@@ -145,7 +147,7 @@ typedef struct {
   double riem_min_pressure;    /* -1e8                                         */
   double riem_failure_threshold; /* 0.2                                        */
   double riem_pressure_at_failure; /* 1e-8                                     */
-  int32_t n_levelsets;         /* 0 or 1                                       */
+  int32_t n_levelsets;         /* 0 or 1 (m2cx_create_levelsets: up to 4)     */
   int32_t ls_materialid;       /* LevelSet[0].MaterialID                       */
   int32_t ls_bc[6];            /* LevelSet[0].BoundaryCondition* (M2CX_LSBC_*) */
   int32_t riem_guess;          /* M2CX_GUESS_*                                 */
@@ -173,7 +175,34 @@ typedef struct {
   int32_t ls_rec_type;         /* M2CX_REC_* (Linear)                          */
   int32_t ls_limiter;          /* M2CX_LIM_* (GeneralizedMinMod)               */
   double ls_gmm_coeff;         /* GeneralizedMinModCoefficient (1.2)           */
+  /* MultiPhase (IoData.cpp:1706-1708): LevelSetCorrectionFrequency (-1; only
+   * -1 / 0 are supported: the optional gap corrections of
+   * MultiPhaseOperator.cpp:1764-1830 are not built) and
+   * CorrectionsUntilReinitialization (1): with several level sets, the
+   * overlaps ResolveConflictsInLevelSets resolves are counted and every
+   * level set is reinitialized once the count reaches this number
+   * (TimeIntegrator.cpp:737-768).                                           */
+  int32_t mp_ls_correction_frequency;
+  int32_t mp_corrections_until_reinit;
 } m2cx_scheme;
+
+/* Several level sets (LevelSet[i] blocks, IoData.cpp:1513-1568; one
+ * LevelSetOperator each, Main.cpp:285-330): level set q tracks material
+ * `materialid` (1 .. nmat-1, one level set per material); the fields are
+ * those of LevelSet[0] in m2cx_scheme.  Passed to m2cx_create_levelsets;
+ * m2cx_create takes level set 0 from the scheme. */
+#define M2CX_MAX_LEVELSETS 4
+typedef struct {
+  int32_t materialid;
+  int32_t bc[6];                /* M2CX_LSBC_* */
+  int32_t solver, flux;         /* M2CX_LSSOLVER_*, M2CX_LSFLUX_* */
+  double delta;
+  int32_t rec_type, limiter;
+  double gmm_coeff;
+  int32_t reinit_frequency, reinit_maxits;
+  double reinit_frequency_dt, reinit_cfl, reinit_tol;
+  int32_t reinit_first_layer, bandwidth;
+} m2cx_levelset;
 
 /* Fields held on the device by a solver handle. */
 enum {
@@ -214,6 +243,18 @@ typedef struct m2cx_solver m2cx_solver;
  * device field; `device` selects the HIP device. */
 int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat,
                 const m2cx_scheme* scheme, int device, m2cx_solver** out);
+/* The same with `nls` level sets (0 .. M2CX_MAX_LEVELSETS) described by
+ * `ls`; the scheme's n_levelsets and LevelSet[0] fields are replaced by
+ * them.  Several level sets are advanced, reinitialized and banded each by
+ * its own settings; the material IDs follow all of them
+ * (MultiPhaseOperator::UpdateMaterialIDByLevelSet, MultiPhaseOperator.cpp:
+ * 153-357) after ResolveConflictsInLevelSets (:1705-1762) has given each
+ * node inside two subdomains to one of them.  Level set q is read and
+ * written with m2cx_upload_levelset / m2cx_download_levelset
+ * (M2CX_FIELD_PHI is level set 0). */
+int m2cx_create_levelsets(const m2cx_grid* grid, const m2cx_material* mats, int nmat,
+                          const m2cx_scheme* scheme, const m2cx_levelset* ls, int nls, int device,
+                          m2cx_solver** out);
 /* Replaces the Destroy() calls in Main.cpp:793-855. */
 int m2cx_destroy(m2cx_solver* h);
 /* Message of the last failure on this handle (empty string if none). */
@@ -228,6 +269,12 @@ const char* m2cx_last_error(const m2cx_solver* h);
 int m2cx_upload(m2cx_solver* h, int field, const void* host);
 /* Copies a device field to the host (reference layout, ghosts included). */
 int m2cx_download(m2cx_solver* h, int field, void* host);
+/* Level set q (0 .. nls-1) of a solver made by m2cx_create_levelsets, or its
+ * level-set residual R_phi (what = M2CX_LS_RESIDUAL, after
+ * m2cx_levelset_residual); ghosts are copied as given. */
+enum { M2CX_LS_PHI = 0, M2CX_LS_RESIDUAL = 1 };
+int m2cx_upload_levelset(m2cx_solver* h, int ls, const double* host);
+int m2cx_download_levelset(m2cx_solver* h, int ls, int what, double* host);
 /* Simulation time / step counter held on the device (Main.cpp:653-712). */
 int m2cx_set_time(m2cx_solver* h, double t, int time_step);
 int m2cx_get_time(m2cx_solver* h, double* t, double* dt, int* time_step);
@@ -306,8 +353,8 @@ int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits);
 
 /* LevelSetOperator::ConstructNarrowBandInReinitializer (LevelSetOperator.cpp:
  * 1328-1334 -> LevelSetReinitializer::ConstructNarrowBand, LevelSetReinitializer
- * .cpp:1479-1592): builds the band around the zero level set of level set 0
- * (levels 0 .. ls_bandwidth by face-neighbour distance, edges and corners of
+ * .cpp:1479-1592): builds the band around the zero level set of every level
+ * set with a bandwidth (levels 0 .. bandwidth by face-neighbour distance, edges and corners of
  * the ghost layer excluded) and cuts phi off outside it to 10 x its largest
  * (smallest) value in the band.  Call after uploading the level set with its
  * boundary values, as SpaceInitializer.cpp:1787-1793 does after
@@ -315,12 +362,13 @@ int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits);
  * m2cx_levelset_residual and m2cx_levelset_reinitialize need it.
  * M2CX_E_UNSUPPORTED without a band. */
 int m2cx_levelset_construct_narrow_band(m2cx_solver* h);
-/* The band for inspection: per ghosted cell its level (31415927 outside),
+/* The band of level set `ls` for inspection: per ghosted cell its level (31415927 outside),
  * useful (level <= bandwidth) and active (level < bandwidth) flags, and
  * bounds = {phi_max, phi_min, phi_out_pos, phi_out_neg}; any may be null.
  * No counterpart in the reference (its Level, UsefulG2 and Active fields are
  * members of LevelSetOperator, LevelSetOperator.h:60-62). */
-int m2cx_get_narrow_band(m2cx_solver* h, int32_t* level, uint8_t* useful, uint8_t* active, double* bounds);
+int m2cx_get_narrow_band(m2cx_solver* h, int ls, int32_t* level, uint8_t* useful, uint8_t* active,
+                         double* bounds);
 
 /* ------------------------------------------------------------------ */
 /* TimeIntegrator (TimeIntegrator.h:77-88, TimeIntegrator.cpp:167-843) */

@@ -29,7 +29,7 @@ FIELD_V, FIELD_V1, FIELD_V2, FIELD_R, FIELD_PHI, FIELD_ID = range(6)
 GEOM_PLANE, GEOM_SPHERE = 0, 3
 RIEMANN_HIST_BINS = 512
 OK, E_ARG, E_HIP, E_OOM, E_HYPERBOLICITY, E_RIEMANN_FATAL, E_UNRESOLVED, E_UNSUPPORTED, E_CAPACITY, \
-    E_REINIT = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9
+    E_REINIT, E_LEVELSETS = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10
 
 # Python name -> C name where they differ beyond the M2CX_ prefix
 C_NAMES = {"EOS_SG": "M2CX_EOS_SG", "LIM_GMM": "M2CX_LIM_GENERALIZED_MINMOD", "TI_FE": "M2CX_TI_FORWARD_EULER"}
@@ -74,7 +74,21 @@ class Scheme(C.Structure):
                 ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
                 ("reinit_first_layer", C.c_int32), ("ls_bandwidth", C.c_int32), ("time_step", C.c_double),
                 ("ls_solver", C.c_int32), ("ls_flux", C.c_int32), ("ls_delta", C.c_double),
-                ("ls_rec_type", C.c_int32), ("ls_limiter", C.c_int32), ("ls_gmm_coeff", C.c_double)]
+                ("ls_rec_type", C.c_int32), ("ls_limiter", C.c_int32), ("ls_gmm_coeff", C.c_double),
+                ("mp_ls_correction_frequency", C.c_int32), ("mp_corrections_until_reinit", C.c_int32)]
+
+
+MAX_LEVELSETS = 4
+LS_PHI, LS_RESIDUAL = 0, 1
+
+
+class Levelset(C.Structure):
+    """m2cx_levelset: one LevelSet[i] block"""
+    _fields_ = [("materialid", C.c_int32), ("bc", C.c_int32 * 6), ("solver", C.c_int32), ("flux", C.c_int32),
+                ("delta", C.c_double), ("rec_type", C.c_int32), ("limiter", C.c_int32), ("gmm_coeff", C.c_double),
+                ("reinit_frequency", C.c_int32), ("reinit_maxits", C.c_int32),
+                ("reinit_frequency_dt", C.c_double), ("reinit_cfl", C.c_double), ("reinit_tol", C.c_double),
+                ("reinit_first_layer", C.c_int32), ("bandwidth", C.c_int32)]
 
 
 class Counters(C.Structure):
@@ -118,6 +132,33 @@ def grid(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), bc=(BC_OUTLET,) * 6, inlet=N
     return g
 
 
+def levelset_from_scheme(s, **kw):
+    """The LevelSet[0] fields of a scheme as one m2cx_levelset, with overrides."""
+    L = Levelset()
+    L.materialid = s.ls_materialid
+    for q in range(6):
+        L.bc[q] = s.ls_bc[q]
+    L.solver, L.flux, L.delta = s.ls_solver, s.ls_flux, s.ls_delta
+    L.rec_type, L.limiter, L.gmm_coeff = s.ls_rec_type, s.ls_limiter, s.ls_gmm_coeff
+    L.reinit_frequency, L.reinit_maxits = s.reinit_frequency, s.reinit_maxits
+    L.reinit_frequency_dt, L.reinit_cfl, L.reinit_tol = s.reinit_frequency_dt, s.reinit_cfl, s.reinit_tol
+    L.reinit_first_layer, L.bandwidth = s.reinit_first_layer, s.ls_bandwidth
+    for k, v in kw.items():
+        if k == "bc":
+            for q in range(6):
+                L.bc[q] = v[q]
+        else:
+            setattr(L, k, v)
+    return L
+
+
+def levelsets(*ls):
+    arr = (Levelset * max(1, len(ls)))()
+    for i, L in enumerate(ls):
+        arr[i] = L
+    return arr
+
+
 def default_scheme(**kw):
     s = Scheme()
     s.flux = FLUX_HLLC
@@ -157,6 +198,8 @@ def default_scheme(**kw):
     s.reinit_tol = 2.0e-4
     s.reinit_first_layer = REINIT_FIXED
     s.ls_bandwidth = INT32_MAX
+    s.mp_ls_correction_frequency = -1
+    s.mp_corrections_until_reinit = 1
     s.time_step = 0.0
     s.ls_solver = LSSOLVER_FINITE_DIFFERENCE
     s.ls_flux = LSFLUX_ROE

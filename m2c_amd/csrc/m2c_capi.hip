@@ -35,6 +35,20 @@ struct TileVariant {
 };
 static const TileVariant kTiles[] = {{16, 4, 4}, {32, 4, 2}, {8, 8, 4}, {16, 8, 2}};
 
+// A level set after the first (m2cx_create_levelsets): the fields, settings
+// and reinitializer state the solver's own members hold for level set 0,
+// exchanged with them (swap_ls) while that level set's operator runs.
+struct LSSlot {
+  double *phi = nullptr, *phi1 = nullptr, *phin = nullptr, *rls = nullptr, *phitmp = nullptr;
+  LSFvm fv{};
+  int lsbc[6] = {1, 1, 1, 1, 1, 1};
+  int lsid = 0;
+  RBuf rb{};
+  NBBuf nb{};
+  double reinit_cfl = 0.8;
+  m2cx_levelset set{};
+};
+
 struct m2cx_solver {
   m2cx_grid grid;
   m2cx_scheme sch;
@@ -76,6 +90,12 @@ struct m2cx_solver {
   RBuf rb{};
   double reinit_cfl = 0.8;  // the reinitializer's CFL, lowered by its failsafe
   NBBuf nb{};               // narrow-band level set (nb.bw == 0: full domain)
+  // several level sets: level set 0 in mp / rb / nb / reinit_cfl / the
+  // scheme's LevelSet[0] fields, level set q >= 1 in lsx[q - 1]
+  int nls = 0;
+  std::vector<LSSlot> lsx;
+  int ls_bucket = 0;        // TimeIntegratorBase::ls_correction_bucket (TimeIntegrator.h:31)
+  int* lsr_list = nullptr;  // candidates of ResolveConflictsInLevelSets
   // number of the step being taken when the host knows it without reading
   // the device (run_steps counts from one read; advance_one_time_step is
   // given it); -1 otherwise
@@ -105,6 +125,93 @@ struct m2cx_solver {
 static int fail(m2cx_solver* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
+}
+
+// a LevelSet[i] block from / into the LevelSet[0] fields of a scheme
+static m2cx_levelset ls_from_scheme(const m2cx_scheme& s) {
+  m2cx_levelset L{};
+  L.materialid = s.ls_materialid;
+  for (int q = 0; q < 6; q++) L.bc[q] = s.ls_bc[q];
+  L.solver = s.ls_solver;
+  L.flux = s.ls_flux;
+  L.delta = s.ls_delta;
+  L.rec_type = s.ls_rec_type;
+  L.limiter = s.ls_limiter;
+  L.gmm_coeff = s.ls_gmm_coeff;
+  L.reinit_frequency = s.reinit_frequency;
+  L.reinit_maxits = s.reinit_maxits;
+  L.reinit_frequency_dt = s.reinit_frequency_dt;
+  L.reinit_cfl = s.reinit_cfl;
+  L.reinit_tol = s.reinit_tol;
+  L.reinit_first_layer = s.reinit_first_layer;
+  L.bandwidth = s.ls_bandwidth;
+  return L;
+}
+
+static void scheme_set_ls(m2cx_scheme& s, const m2cx_levelset& L) {
+  s.ls_materialid = L.materialid;
+  for (int q = 0; q < 6; q++) s.ls_bc[q] = L.bc[q];
+  s.ls_solver = L.solver;
+  s.ls_flux = L.flux;
+  s.ls_delta = L.delta;
+  s.ls_rec_type = L.rec_type;
+  s.ls_limiter = L.limiter;
+  s.ls_gmm_coeff = L.gmm_coeff;
+  s.reinit_frequency = L.reinit_frequency;
+  s.reinit_maxits = L.reinit_maxits;
+  s.reinit_frequency_dt = L.reinit_frequency_dt;
+  s.reinit_cfl = L.reinit_cfl;
+  s.reinit_tol = L.reinit_tol;
+  s.reinit_first_layer = L.reinit_first_layer;
+  s.ls_bandwidth = L.bandwidth;
+}
+
+// exchanges level set q >= 1 with the solver's level-set members; a second
+// call puts it back
+static void swap_ls(m2cx_solver* h, int q) {
+  if (q <= 0) return;
+  LSSlot& S = h->lsx[q - 1];
+  MPBuffers& B = h->mp;
+  std::swap(B.phi, S.phi);
+  std::swap(B.phi1, S.phi1);
+  std::swap(B.phin, S.phin);
+  std::swap(B.rls, S.rls);
+  std::swap(B.phitmp, S.phitmp);
+  std::swap(B.fv, S.fv);
+  std::swap(B.lsbc, S.lsbc);
+  std::swap(B.lsid, S.lsid);
+  std::swap(h->rb, S.rb);
+  std::swap(h->nb, S.nb);
+  std::swap(h->reinit_cfl, S.reinit_cfl);
+  const m2cx_levelset mine = ls_from_scheme(h->sch);
+  scheme_set_ls(h->sch, S.set);
+  S.set = mine;
+}
+
+// every level set's fields (for the routines that look at all of them); while
+// m2cx_create_levelsets is still allocating the slots, only those that exist
+static LSPtrs ls_ptrs(const m2cx_solver* h) {
+  LSPtrs P{};
+  P.n = std::min(h->nls, 1 + (int)h->lsx.size());
+  for (int q = 0; q < P.n; q++) {
+    const bool own = q == 0;
+    P.phi[q] = own ? h->mp.phi : h->lsx[q - 1].phi;
+    P.phin[q] = own ? h->mp.phin : h->lsx[q - 1].phin;
+    P.mat[q] = own ? h->mp.lsid : h->lsx[q - 1].lsid;
+    P.tracked |= 1u << P.mat[q];
+  }
+  return P;
+}
+
+// the level sets at time n for the Riemann normals of the interface list
+static void fill_iface_ls(m2cx_solver* h) {
+  const LSPtrs P = ls_ptrs(h);
+  IfaceList& L = h->mp.L;
+  L.nls = P.n;
+  for (int q = 0; q < M2CX_MAX_LEVELSETS; q++) {
+    L.lsphi[q] = q < P.n ? P.phi[q] : nullptr;
+    L.lsmat[q] = q < P.n ? P.mat[q] : -1;
+  }
 }
 
 // Slots of c_mats per device: solvers with byte-identical material tables
@@ -255,13 +362,97 @@ extern "C" {
 
 const char* m2cx_last_error(const m2cx_solver* h) { return h ? h->err.c_str() : "null handle"; }
 
+static int create_impl(const m2cx_grid* grid, const m2cx_material* mats, int nmat, const m2cx_scheme* sch,
+                       int device, m2cx_solver** out, bool several);
+
 int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, const m2cx_scheme* sch, int device,
                 m2cx_solver** out) {
+  return create_impl(grid, mats, nmat, sch, device, out, false);
+}
+
+// CoeffK of a level set's own reconstruction on the (uniform) mesh, per axis
+static std::vector<int> ls_k_coefs(const m2cx_grid& gr, int limiter) {
+  std::vector<int> K;
+  const int Ns[3] = {gr.nx, gr.ny, gr.nz};
+  const double lo[3] = {gr.x0, gr.y0, gr.z0}, hi[3] = {gr.xmax, gr.ymax, gr.zmax};
+  for (int a = 0; a < 3; a++) {
+    const double hh = (hi[a] - lo[a]) / Ns[a];
+    for (int i = 0; i < Ns[a]; i++) K.push_back(slope_k(limiter, (hh + hh) / (hh + hh), 2.0 * hh / (hh + hh)));
+  }
+  return K;
+}
+
+// Main.cpp:285-330 with each LevelSet[i] block
+int m2cx_create_levelsets(const m2cx_grid* grid, const m2cx_material* mats, int nmat, const m2cx_scheme* sch,
+                          const m2cx_levelset* ls, int nls, int device, m2cx_solver** out) {
+  if (!grid || !mats || !sch || !out || nls < 0 || nls > M2CX_MAX_LEVELSETS || (nls > 0 && !ls)) return M2CX_E_ARG;
+  *out = nullptr;
+  unsigned seen = 0;
+  for (int q = 0; q < nls; q++) {
+    const m2cx_levelset& L = ls[q];
+    // a level set tracks a material other than 0, one level set per material
+    if (L.materialid <= 0 || L.materialid >= nmat || ((seen >> L.materialid) & 1u)) return M2CX_E_ARG;
+    seen |= 1u << L.materialid;
+    if (L.reinit_first_layer < M2CX_REINIT_FIXED || L.reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2 ||
+        L.solver < M2CX_LSSOLVER_FINITE_VOLUME || L.solver > M2CX_LSSOLVER_FINITE_DIFFERENCE ||
+        L.flux < M2CX_LSFLUX_ROE || L.flux > M2CX_LSFLUX_UPWIND)
+      return M2CX_E_ARG;
+    const int bw = (L.bandwidth > 0 && L.bandwidth < INT32_MAX) ? L.bandwidth : 0;
+    if (bw && (bw <= 1 || L.reinit_frequency <= 0 || L.reinit_frequency >= bw)) return M2CX_E_ARG;
+    if (bw && L.solver == M2CX_LSSOLVER_FINITE_VOLUME) return M2CX_E_UNSUPPORTED;
+  }
+  // the optional corrections of ResolveConflictsInLevelSets are not built
+  if (sch->mp_ls_correction_frequency > 0) return M2CX_E_UNSUPPORTED;
+  m2cx_scheme s0 = *sch;
+  s0.n_levelsets = nls;
+  if (nls > 0) scheme_set_ls(s0, ls[0]);
+  if (int rc = create_impl(grid, mats, nmat, &s0, device, out, true)) return rc;
+  m2cx_solver* h = *out;
+  const long n = h->ncells;
+  const double diag = std::sqrt((grid->xmax - grid->x0) * (grid->xmax - grid->x0) +
+                                (grid->ymax - grid->y0) * (grid->ymax - grid->y0) +
+                                (grid->zmax - grid->z0) * (grid->zmax - grid->z0));
+  for (int q = 1; q < nls; q++) {
+    LSSlot S;
+    S.set = ls[q];
+    S.lsid = ls[q].materialid;
+    for (int f = 0; f < 6; f++) S.lsbc[f] = ls[q].bc[f];
+    for (double** p : {&S.phi, &S.phi1, &S.phin, &S.rls, &S.phitmp}) {
+      HIPCHK(h, hipMalloc(p, n * 8));
+      HIPCHK(h, hipMemsetAsync(*p, 0, n * 8, h->stream));
+    }
+    S.fv.on = ls[q].solver == M2CX_LSSOLVER_FINITE_VOLUME;
+    S.fv.flux = ls[q].flux;
+    S.fv.rec = ls[q].rec_type;
+    S.fv.limiter = ls[q].limiter;
+    S.fv.gmm = ls[q].gmm_coeff;
+    S.fv.delta = ls[q].delta;
+    const std::vector<int> K = ls_k_coefs(*grid, ls[q].limiter);
+    int* lsk = nullptr;
+    HIPCHK(h, hipMalloc(&lsk, K.size() * 4));
+    HIPCHK(h, hipMemcpy(lsk, K.data(), K.size() * 4, hipMemcpyHostToDevice));
+    S.fv.lsk = lsk;
+    S.reinit_cfl = ls[q].reinit_cfl;
+    S.nb.diag = diag;
+    S.nb.bw = (ls[q].bandwidth > 0 && ls[q].bandwidth < INT32_MAX) ? ls[q].bandwidth : 0;
+    if (S.nb.bw)
+      if (int rc = nb_alloc(S.nb, n, h->stream)) return fail(h, rc, "narrow-band buffers");
+    h->lsx.push_back(S);
+  }
+  if (nls >= 2) HIPCHK(h, hipMalloc(&h->lsr_list, (long)h->mp.list_cap * 4));
+  fill_iface_ls(h);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static int create_impl(const m2cx_grid* grid, const m2cx_material* mats, int nmat, const m2cx_scheme* sch,
+                       int device, m2cx_solver** out, bool several) {
   if (!grid || !mats || !sch || !out || nmat < 1 || nmat > M2C_MAX_MAT) return M2CX_E_ARG;
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return M2CX_E_ARG;
   if ((long)grid->nx * grid->ny * grid->nz >= (1L << 31)) return M2CX_E_ARG;  // 32-bit cell numbering
   if (sch->var_type < M2CX_VAR_PRIMITIVE || sch->var_type > M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) return M2CX_E_ARG;
-  if (sch->n_levelsets > 1) return M2CX_E_UNSUPPORTED;
+  if (sch->n_levelsets < 0 || sch->n_levelsets > (several ? M2CX_MAX_LEVELSETS : 1)) return M2CX_E_UNSUPPORTED;
+  if (sch->n_levelsets > 0 && (sch->ls_materialid <= 0 || sch->ls_materialid >= nmat)) return M2CX_E_ARG;
   if (sch->reinit_first_layer < M2CX_REINIT_FIXED || sch->reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2)
     return M2CX_E_ARG;
   if (sch->riemann_normal < M2CX_RNORMAL_LEVEL_SET || sch->riemann_normal > M2CX_RNORMAL_AVERAGE) return M2CX_E_ARG;
@@ -271,10 +462,10 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   if (sch->ls_solver < M2CX_LSSOLVER_FINITE_VOLUME || sch->ls_solver > M2CX_LSSOLVER_FINITE_DIFFERENCE ||
       sch->ls_flux < M2CX_LSFLUX_ROE || sch->ls_flux > M2CX_LSFLUX_UPWIND)
     return M2CX_E_ARG;
-  // the level-set normal is defined for faces between material 0 and the
-  // level set's material (CalculateGradPhiAtCellInterface asserts it)
-  if (sch->riemann_normal != M2CX_RNORMAL_MESH && (sch->n_levelsets != 1 || nmat != 2 || sch->ls_materialid != 1))
-    return M2CX_E_UNSUPPORTED;
+  // the level-set normal needs level sets (faces next to a material that
+  // no level set tracks, other than 0, take the mesh normal: the reference
+  // asserts there, CalculateGradPhiAtCellInterface)
+  if (sch->riemann_normal != M2CX_RNORMAL_MESH && sch->n_levelsets < 1) return M2CX_E_UNSUPPORTED;
   // narrow band (LevelSetOperator.cpp:84-99): bandwidth at least 2 and the
   // reinitialization frequency, in steps, below it; finite differences only
   const int bw = (sch->n_levelsets > 0 && sch->ls_bandwidth > 0 && sch->ls_bandwidth < INT32_MAX)
@@ -404,6 +595,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
   c0.tmax = (sch->max_time > 0) ? sch->max_time : DBL_MAX;
   HIPCHK(h, hipMemcpy(h->cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
   h->multimat = nmat > 1 || sch->n_levelsets > 0;
+  h->nls = sch->n_levelsets;
   if (h->multimat) {
     int rc = mp_create(h->mp, g, *sch, ax, h->stream);
     if (rc) return fail(h, rc, "multi-material buffers: " + std::string(hipGetErrorString(hipGetLastError())));
@@ -440,6 +632,7 @@ int m2cx_create(const m2cx_grid* grid, const m2cx_material* mats, int nmat, cons
     }
     
   }
+  if (h->multimat) fill_iface_ls(h);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -460,6 +653,13 @@ int m2cx_destroy(m2cx_solver* h) {
   if (h->multimat) mp_destroy(h->mp);
   ri_free(h->rb);
   nb_free(h->nb);
+  for (LSSlot& S : h->lsx) {
+    for (void* p : {(void*)S.phi, (void*)S.phi1, (void*)S.phin, (void*)S.rls, (void*)S.phitmp, (void*)S.fv.lsk})
+      if (p) hipFree(p);
+    ri_free(S.rb);
+    nb_free(S.nb);
+  }
+  if (h->lsr_list) hipFree(h->lsr_list);
   if (h->tflag) hipFree(h->tflag);
   if (h->stream2) hipStreamSynchronize(h->stream2);
   if (h->ev_fork) hipEventDestroy(h->ev_fork);
@@ -478,6 +678,28 @@ static double** field_ptr(m2cx_solver* h, int field) {
   if (field >= M2CX_FIELD_V && field <= M2CX_FIELD_V2) return h->V[field];
   if (field == M2CX_FIELD_R) return h->R;
   return nullptr;
+}
+
+int m2cx_upload_levelset(m2cx_solver* h, int ls, const double* host) {
+  if (!h || !host) return M2CX_E_ARG;
+  if (ls < 0 || ls >= h->nls) return fail(h, M2CX_E_ARG, "no such level set");
+  HIPCHK(h, hipSetDevice(h->device));
+  double* dst = ls == 0 ? h->mp.phi : h->lsx[ls - 1].phi;
+  HIPCHK(h, hipMemcpyAsync(dst, host, h->ncells * 8, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int m2cx_download_levelset(m2cx_solver* h, int ls, int what, double* host) {
+  if (!h || !host || (what != M2CX_LS_PHI && what != M2CX_LS_RESIDUAL)) return M2CX_E_ARG;
+  if (ls < 0 || ls >= h->nls) return fail(h, M2CX_E_ARG, "no such level set");
+  HIPCHK(h, hipSetDevice(h->device));
+  const bool own = ls == 0;
+  const double* src = what == M2CX_LS_PHI ? (own ? h->mp.phi : h->lsx[ls - 1].phi)
+                                          : (own ? h->mp.rls : h->lsx[ls - 1].rls);
+  HIPCHK(h, hipMemcpyAsync(host, src, h->ncells * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
 }
 
 int m2cx_upload(m2cx_solver* h, int field, const void* host) {
@@ -530,6 +752,7 @@ static int check_error(m2cx_solver* h) {
   if (c.error & 2) return fail(h, M2CX_E_RIEMANN_FATAL, "exact Riemann solver reached a fatal state");
   if (c.error & 4) return fail(h, M2CX_E_CAPACITY, "device work-list capacity exceeded");
   if (c.error & 8) return fail(h, M2CX_E_UNRESOLVED, "unresolved phase-change cells");
+  if (c.error & 16) return fail(h, M2CX_E_LEVELSETS, "a node belongs to two material subdomains");
   return 0;
 }
 
@@ -741,6 +964,7 @@ static void recon_stage(m2cx_solver* h, int vin) {
 static void interface_stage(m2cx_solver* h, int vin, int stage1) {
   recon_stage(h, vin);
   if (!h->multimat) return;
+  fill_iface_ls(h);
   guess_stage(h, vin);
   hipEvent_t e0{};
   tic(h, "riemann", &e0);
@@ -765,13 +989,17 @@ static void levelset_band(m2cx_solver* h, CF5 V, int which) {
 }
 
 static void levelset_stage(m2cx_solver* h, int vin, int which) {
-  if (!h->multimat || h->sch.n_levelsets == 0) return;
+  if (!h->multimat || h->nls == 0) return;
   hipEvent_t e0{};
   tic(h, "levelset", &e0);
-  if (h->nb.bw)
-    levelset_band(h, cf5(h, h->V[vin]), which);
-  else
-    mp_levelset(h->mp, h->g, cf5(h, h->V[vin]), which, h->cnt, h->stream);
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    if (h->nb.bw)
+      levelset_band(h, cf5(h, h->V[vin]), which);
+    else
+      mp_levelset(h->mp, h->g, cf5(h, h->V[vin]), which, h->cnt, h->stream);
+    swap_ls(h, q);
+  }
   toc(h, "levelset", e0);
 }
 
@@ -1076,10 +1304,61 @@ static bool reinit_due(const m2cx_solver* h, double time, double dt, int time_st
 // 720-800), one level set, no embedded surfaces.  FixUnresolvedNodes with the
 // failsafe density leaves no cell unresolved, so the level-set correction of
 // :771-793 never runs.
+// MultiPhaseOperator::ResolveConflictsInLevelSets, part I (MultiPhaseOperator
+// .cpp:1705-1762), with its boundary fill (TimeIntegrator.cpp:737-741): the
+// candidates (nodes inside two or more subdomains) are collected on the
+// device, sorted into the reference's (k, j, i) order here, and resolved in
+// that order by one device thread.
+static int resolve_conflicts(m2cx_solver* h, int* resolved) {
+  *resolved = 0;
+  if (h->nls < 2) return 0;
+  const LSPtrs P = ls_ptrs(h);
+  const int cap = h->mp.list_cap;
+  hipStream_t st = h->stream;
+  k_lsr_reset<<<1, 1, 0, st>>>(h->cnt);
+  k_lsr_mark<<<blocks_for(h->ncells), 256, 0, st>>>(h->g, P, h->lsr_list, cap, h->cnt);
+  DevCounters c;
+  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  if (c.n_lsr == 0) return 0;
+  if (c.n_lsr > cap) return fail(h, M2CX_E_CAPACITY, "overlapping level-set subdomains exceed the list capacity");
+  std::vector<int> list(c.n_lsr);
+  HIPCHK(h, hipMemcpy(list.data(), h->lsr_list, list.size() * 4, hipMemcpyDeviceToHost));
+  std::sort(list.begin(), list.end());  // linear index order is (k, j, i) order
+  HIPCHK(h, hipMemcpy(h->lsr_list, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+  k_lsr_resolve<<<1, 64, 0, st>>>(h->g, P, h->lsr_list, c.n_lsr, h->cnt);
+  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  *resolved = c.n_resolved;
+  if (*resolved) {
+    const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
+    for (int q = 0; q < h->nls; q++) {
+      swap_ls(h, q);
+      const int* b = h->mp.lsbc;
+      k_ls_bc<<<blocks_for(nf), 256, 0, st>>>(h->g, h->mp.phi, b[0], b[1], b[2], b[3], b[4], b[5]);
+      swap_ls(h, q);
+    }
+  }
+  return 0;
+}
+
+// TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
+// 720-800), no embedded surfaces: overlaps of several level sets resolved,
+// the material IDs from every level set, each level set reinitialized at
+// its own frequency (all of them, at once, when the overlaps resolved reach
+// CorrectionsUntilReinitialization), then the state update.  FixUnresolvedNodes
+// with the failsafe density leaves no cell unresolved, so the level-set
+// correction of :771-793 never runs.
 static int end_of_step(m2cx_solver* h) {
-  if (!(h->multimat && h->sch.n_levelsets > 0)) return 0;
+  if (!(h->multimat && h->nls > 0)) return 0;
   hipEvent_t e0{};
-  if (!reinit_enabled(h)) {
+  bool any_reinit = false;
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    any_reinit |= reinit_enabled(h);
+    swap_ls(h, q);
+  }
+  if (h->nls == 1 && !any_reinit && h->sch.mp_corrections_until_reinit > 0) {
     tic(h, "endstep", &e0);
     mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
     toc(h, "endstep", e0);
@@ -1087,19 +1366,34 @@ static int end_of_step(m2cx_solver* h) {
   }
   MPBuffers& B = h->mp;
   F5 V = f5(h, h->V[0]);
-  mp_end_ids(B, h->g, h->s, h->ID, h->cnt, h->stream);
-  bool due;
-  if (h->sch.reinit_frequency_dt <= 0 && h->host_time_step >= 0) {
-    // Frequency mode: the step number alone decides, no device read
-    due = reinit_due(h, 0.0, 0.0, h->host_time_step);
-  } else {
-    DevCounters c;
-    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    due = reinit_due(h, c.t, c.dts, c.time_step);
+  int resolved = 0;
+  if (int rc = resolve_conflicts(h, &resolved)) return rc;
+  mp_end_ids(B, h->g, h->s, ls_ptrs(h), h->ID, h->cnt, h->stream);
+  h->ls_bucket += resolved;
+  const bool must = h->ls_bucket >= h->sch.mp_corrections_until_reinit;
+  if (must) h->ls_bucket = 0;
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    bool due = must;
+    if (!must && reinit_enabled(h)) {
+      if (h->sch.reinit_frequency_dt <= 0 && h->host_time_step >= 0) {
+        // Frequency mode: the step number alone decides, no device read
+        due = reinit_due(h, 0.0, 0.0, h->host_time_step);
+      } else {
+        DevCounters c;
+        HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        due = reinit_due(h, c.t, c.dts, c.time_step);
+      }
+    }
+    int rc = 0;
+    if (due && !reinit_enabled(h))
+      rc = fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
+    else if (due)
+      rc = reinitialize(h, 0);
+    swap_ls(h, q);
+    if (rc) return rc;
   }
-  if (due)
-    if (int rc = reinitialize(h, 0)) return rc;
   mp_end_states(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   mp_end_fix(B, h->g, h->s, V, h->ID, h->cnt, h->stream);
   return 0;
@@ -1112,6 +1406,7 @@ static int end_of_step(m2cx_solver* h) {
 // for both.  Results are those of the single launch (same code per tile).
 static void split_stage(m2cx_solver* h, const StageSpec& sp) {
   const long nin = (long)h->g.NX * h->g.NY * h->g.NZ;
+  fill_iface_ls(h);
   recon_stage(h, sp.vin);
   if (sp.resync) k_reset_stage<<<1, 1, 0, h->stream>>>(h->cnt);
   guess_stage(h, sp.vin);
@@ -1147,12 +1442,21 @@ static void tile_flags(m2cx_solver* h) {
 // One TimeIntegrator step with dt already in cnt->dt (TimeIntegrator.cpp:167-715)
 static int step_kernels(m2cx_solver* h) {
   const int ti = h->sch.integrator;
-  if (h->nb.bw && !h->nb.built) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream, false);
   // Phi_tmp = Phi: over the whole brick, with the narrow band only on its
   // useful nodes after the first step (TimeIntegrator.cpp:333-339)
-  const bool band_copy = h->nb.bw && h->host_time_step > 1;
-  if (h->multimat) mp_begin_step(h->mp, h->g, h->s, h->ID, h->cnt, h->stream, !band_copy);
-  if (band_copy) k_nb_copy_useful<<<nb_blocks(h), 256, 0, h->stream>>>(h->nb, h->mp.phin, h->mp.phi);
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    const bool unbuilt = h->nb.bw && !h->nb.built;
+    if (!unbuilt) {
+      if (h->nb.bw && h->host_time_step > 1)
+        k_nb_copy_useful<<<nb_blocks(h), 256, 0, h->stream>>>(h->nb, h->mp.phin, h->mp.phi);
+      else
+        k_copy_f64<<<blocks_for(h->g.ncells), 256, 0, h->stream>>>(h->mp.phin, h->mp.phi, h->g.ncells);
+    }
+    swap_ls(h, q);
+    if (unbuilt) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  }
   if (h->overlap) tile_flags(h);
   if (ti == M2CX_TI_FORWARD_EULER && h->overlap) {
     split_stage(h, StageSpec{0, 1, MODE_FE, 1, 0, -1});
@@ -1261,10 +1565,19 @@ int m2cx_get_counters(m2cx_solver* h, m2cx_counters* out) {
 
 int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits) {
   if (!h) return M2CX_E_ARG;
-  if (!reinit_enabled(h) || !h->multimat)
+  if (!h->multimat || h->nls == 0)
     return fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
   HIPCHK(h, hipSetDevice(h->device));
-  if (int rc = reinitialize(h, special_maxits)) return rc;
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    int rc = 0;
+    if (!reinit_enabled(h))
+      rc = fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
+    else
+      rc = reinitialize(h, special_maxits);
+    swap_ls(h, q);
+    if (rc) return rc;
+  }
   HIPCHK(h, hipGetLastError());
   return check_error(h);
 }
@@ -1617,11 +1930,16 @@ int m2cx_mlp_train(m2cx_solver* h, int n, const float* X, const float* T, int ep
 int m2cx_levelset_residual(m2cx_solver* h, int v_field) {
   if (!h || !h->mp.phi || v_field < 0 || v_field > 2) return M2CX_E_ARG;
   HIPCHK(h, hipSetDevice(h->device));
-  if (h->nb.bw && !h->nb.built) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
-  if (h->nb.bw)
-    levelset_band(h, cf5(h, h->V[v_field]), -1);
-  else
-    mp_levelset(h->mp, h->g, cf5(h, h->V[v_field]), -1, h->cnt, h->stream);
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    const bool unbuilt = h->nb.bw && !h->nb.built;
+    if (!unbuilt && h->nb.bw)
+      levelset_band(h, cf5(h, h->V[v_field]), -1);
+    else if (!unbuilt)
+      mp_levelset(h->mp, h->g, cf5(h, h->V[v_field]), -1, h->cnt, h->stream);
+    swap_ls(h, q);
+    if (unbuilt) return fail(h, M2CX_E_ARG, "the narrow band of the level set has not been constructed");
+  }
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -2032,6 +2350,8 @@ int m2cx_ensemble_destroy(m2cx_ensemble* e) {
   return 0;
 }
 
+static int nb_build_all(m2cx_solver* h, bool* any);
+
 int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id, const m2cx_geometry* geoms,
                     int ngeom) {
   if (!h || !default_state || ngeom < 0 || (ngeom > 0 && !geoms)) return M2CX_E_ARG;
@@ -2079,12 +2399,21 @@ int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id,
   const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
   k_ghost_ids<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, h->ID);
   if (phi) {
-    const MPBuffers& B = h->mp;
-    k_ls_bc<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, phi, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3],
-                                                    B.lsbc[4], B.lsbc[5]);
-    // then the narrow band (SpaceInitializer.cpp:1787-1793)
-    if (h->nb.bw)
-      if (int rc = nb_build(h)) return rc;
+    // every level set from the geometries of its material, with its own
+    // boundary fill (InitializePhiByDistance per level set,
+    // SpaceInitializer.cpp:1314-1360), then the bands (:1787-1793)
+    for (int q = 0; q < h->nls; q++) {
+      swap_ls(h, q);
+      const MPBuffers& B = h->mp;
+      if (q > 0)
+        k_initialize<<<grid_blocks(nin), 256, 0, h->stream>>>(h->g, f5(h, h->V[0]), h->ID, B.phi, dV0, default_id,
+                                                              dG, ngeom, B.lsid, diag, 0);
+      k_ls_bc<<<grid_blocks(nf), 256, 0, h->stream>>>(h->g, B.phi, B.lsbc[0], B.lsbc[1], B.lsbc[2], B.lsbc[3],
+                                                      B.lsbc[4], B.lsbc[5]);
+      swap_ls(h, q);
+    }
+    bool any = false;
+    if (int rc = nb_build_all(h, &any)) return rc;
   }
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2093,27 +2422,47 @@ int m2cx_initialize(m2cx_solver* h, const double* default_state, int default_id,
   return 0;
 }
 
+// the band of every level set that has one
+static int nb_build_all(m2cx_solver* h, bool* any) {
+  *any = false;
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    int rc = 0;
+    if (h->nb.bw) {
+      rc = nb_build(h);
+      *any = true;
+    }
+    swap_ls(h, q);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 int m2cx_levelset_construct_narrow_band(m2cx_solver* h) {
   if (!h) return M2CX_E_ARG;
-  if (!h->nb.bw) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band: ls_bandwidth is not set");
   HIPCHK(h, hipSetDevice(h->device));
-  if (int rc = nb_build(h)) return rc;
+  bool any = false;
+  if (int rc = nb_build_all(h, &any)) return rc;
+  if (!any) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band: ls_bandwidth is not set");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
-int m2cx_get_narrow_band(m2cx_solver* h, int32_t* level, uint8_t* useful, uint8_t* active, double* bounds) {
+int m2cx_get_narrow_band(m2cx_solver* h, int ls, int32_t* level, uint8_t* useful, uint8_t* active,
+                         double* bounds) {
   if (!h) return M2CX_E_ARG;
-  if (!h->nb.bw || !h->nb.built) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band constructed");
+  if (ls < 0 || ls >= h->nls) return fail(h, M2CX_E_ARG, "no such level set");
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  const NBBuf& nb = ls == 0 ? h->nb : h->lsx[ls - 1].nb;
+  if (!nb.bw || !nb.built) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band constructed");
   const long n = h->ncells;
-  if (level) HIPCHK(h, hipMemcpy(level, h->nb.level, n * 4, hipMemcpyDeviceToHost));
-  if (useful) HIPCHK(h, hipMemcpy(useful, h->nb.useful, n, hipMemcpyDeviceToHost));
-  if (active) HIPCHK(h, hipMemcpy(active, h->nb.active, n, hipMemcpyDeviceToHost));
+  if (level) HIPCHK(h, hipMemcpy(level, nb.level, n * 4, hipMemcpyDeviceToHost));
+  if (useful) HIPCHK(h, hipMemcpy(useful, nb.useful, n, hipMemcpyDeviceToHost));
+  if (active) HIPCHK(h, hipMemcpy(active, nb.active, n, hipMemcpyDeviceToHost));
   if (bounds) {
     NBState st;
-    HIPCHK(h, hipMemcpy(&st, h->nb.st, sizeof st, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(&st, nb.st, sizeof st, hipMemcpyDeviceToHost));
     bounds[0] = st.phi_max;
     bounds[1] = st.phi_min;
     bounds[2] = st.phi_out_pos;

@@ -10,11 +10,14 @@ struct DevCounters {
   unsigned long long stage_clipped;  // hits in the current intermediate stage
   unsigned long long riemann_calls, riemann_trivial, riemann_errors, riemann_iters, riemann_fevals;
   unsigned long long riemann_fallbacks, guess_fallbacks, id_changed, steps;
-  int error;          // bit 0 hyperbolicity (abort), bit 1 Riemann fatal, bit 2 capacity, bit 3 unresolved
+  int error;          // bit 0 hyperbolicity (abort), bit 1 Riemann fatal, bit 2 capacity, bit 3 unresolved,
+                      // bit 4 a node inside two level-set subdomains
   int done;           // time loop reached MaxTime
   unsigned int dt_ticket;
   int nif;            // interface faces in the current list
-  int n_remaining, n_changed, n_unresolved, pad;
+  int n_remaining, n_changed, n_unresolved;
+  int n_resolved;     // nodes given to one subdomain by ResolveConflictsInLevelSets this step
+  int n_lsr;          // nodes inside two or more level-set subdomains (its candidates)
   double t, dt, cfl, dts, tmax;
   int time_step, pad2;
   unsigned long long unresolved_total;  // cells left unresolved by the state update, summed
@@ -36,7 +39,11 @@ struct IfaceList {
   int* err;       // Riemann error of the capture
   int* stats;     // [3][cap]: main-loop iterations, f(p) evaluations, flags (last solve)
   unsigned long long* hist;  // [2][M2CX_RIEMANN_HIST_BINS]: counts of iterations, of f(p) evaluations
-  const double* phi;         // level set at time n (the Riemann normal of LevelSet / Average)
+  // the level sets at time n (the Riemann normal of LevelSet / Average) and
+  // the material each tracks
+  const double* lsphi[M2CX_MAX_LEVELSETS];
+  int lsmat[M2CX_MAX_LEVELSETS];
+  int nls;
   int cap;
 };
 

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevGrid g, Scheme s, CF5 V, co
         // the direction the exact solver will solve along (Mesh, LevelSet
         // or Average normal), so the guess is for the same problem
         double dir[3];
-        bimat_normal(g, s.rnormal, L.phi, d, c, ID[c - st], dir);
+        bimat_normal(g, s.rnormal, L, d, c, ID[c - st], ID[c], dir);
         float x[8];
         ok = mlp_features(c_mats[s.mset], dir, Vm, ID[c - st], Vp, ID[c], x, P, pc);
         if (ok)

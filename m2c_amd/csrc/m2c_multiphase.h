@@ -99,7 +99,6 @@ inline int mp_create(MPBuffers& B, DevGrid& g, const m2cx_scheme& sch, std::vect
     if (hipMalloc(p, n * 8) != hipSuccess) return M2CX_E_OOM;
     if (hipMemsetAsync(*p, 0, n * 8, st) != hipSuccess) return M2CX_E_HIP;
   }
-  B.L.phi = B.phi;
   // FD3 tables (GradientCalculatorFD3.cpp:300-453), per axis because the mesh
   // is a tensor product; cyR1 holds the first-order right-biased y stencil the
   // reference uses where its `i<NY-1` test (:400) fails.
@@ -240,23 +239,15 @@ __device__ __forceinline__ double cdiff3(double phi0, double phi1, double phi2, 
   return c0 * phi0 + c1 * phi1 + c2 * phi2;
 }
 
-// GetNormalForBimaterialRiemann / CalculateGradPhiAtCellInterface /
-// CalculateGradientAtCellInterface (SpaceOperator.cpp:2186-2415) for one
-// level set at the face of cell c on axis d (minus side c - stride): grad phi
-// with its normal part from the two cells and tangential parts from central
-// differences in both columns weighted to the face, normalized; pointing from
-// the level set's material into material 0; its axis component made
-// non-negative; Average adds the axis and normalizes again.  Interface faces
-// lie between interior cells, so the columns' neighbours are interior cells
-// or face ghosts (the reference's one-sided branches at ghost rows are not
-// reached).
-__device__ __forceinline__ void bimat_normal(const DevGrid& g, int rnormal, const double* __restrict__ phi, int d,
-                                             long c, int idl, double* dir) {
+// CalculateGradientAtCellInterface (SpaceOperator.cpp:2274-2415) at the face
+// of cell c on axis d (minus side c - stride): grad phi with its normal part
+// from the two cells and tangential parts from central differences in both
+// columns weighted to the face, normalized.  Interface faces lie between
+// interior cells, so the columns' neighbours are interior cells or face
+// ghosts (the reference's one-sided branches at ghost rows are not reached).
+__device__ __forceinline__ void grad_at_face(const DevGrid& g, const double* __restrict__ phi, int d, long c,
+                                             double* dir) {
   dir[0] = dir[1] = dir[2] = 0.0;
-  if (rnormal == M2CX_RNORMAL_MESH) {
-    dir[d] = 1.0;
-    return;
-  }
   long r = c;
   const int i = (int)(r % g.SX) - 1; r /= g.SX;
   const int j = (int)(r % g.SY) - 1;
@@ -286,9 +277,51 @@ __device__ __forceinline__ void bimat_normal(const DevGrid& g, int rnormal, cons
     const double inv = 1.0 / nrm;  // Vec3D::operator/= multiplies by the reciprocal
     dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
   }
-  if (idl == 0)
-    for (int e = 0; e < 3; e++) dir[e] *= -1.0;
-  if (sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+}
+
+// GetNormalForBimaterialRiemann / CalculateGradPhiAtCellInterface
+// (SpaceOperator.cpp:2186-2270) at face c - 1/2 e_d between material idl
+// (minus side) and idr: one level set when one side is material 0 (its
+// gradient, turned when the minus side is material 0), two otherwise (the
+// difference of their unit gradients, normalized); then aligned with +e_d.
+// A material no level set tracks (the reference asserts) gives the mesh
+// normal.
+__device__ __forceinline__ void bimat_normal(const DevGrid& g, int rnormal, const IfaceList& L, int d, long c,
+                                             int idl, int idr, double* dir) {
+  dir[0] = dir[1] = dir[2] = 0.0;
+  if (rnormal == M2CX_RNORMAL_MESH) {
+    dir[d] = 1.0;
+    return;
+  }
+  const int my_mat = (idr == 0) ? idl : idr;
+  const int nb_mat = (idl == 0) ? idr : idl;
+  int my_ls = -1, nb_ls = -1;
+  for (int q = 0; q < L.nls; q++) {
+    if (my_ls < 0 && L.lsmat[q] == my_mat) my_ls = q;
+    if (nb_ls < 0 && L.lsmat[q] == nb_mat) nb_ls = q;
+  }
+  if (my_ls < 0 || nb_ls < 0) {
+    dir[d] = 1.0;
+    return;
+  }
+  if (my_ls == nb_ls) {
+    grad_at_face(g, L.lsphi[my_ls], d, c, dir);
+    if (idl == 0)
+      for (int e = 0; e < 3; e++) dir[e] *= -1.0;
+    if (sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  } else {
+    double n1[3], n2[3];
+    grad_at_face(g, L.lsphi[my_ls], d, c, n1);
+    grad_at_face(g, L.lsphi[nb_ls], d, c, n2);
+    for (int e = 0; e < 3; e++) dir[e] = n2[e] - n1[e];
+    const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (nr != 0.0) {
+      const double inv = 1.0 / nr;
+      dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+    } else {
+      dir[d] = 1.0;
+    }
+  }
   if (dir[d] < 0.0) dir[d] *= -1.0;
   if (rnormal == M2CX_RNORMAL_AVERAGE) {
     dir[d] += 1.0;
@@ -335,7 +368,7 @@ __device__ __forceinline__ int riemann_face(const DevGrid& g, const Scheme& s, c
   double Vm[5], Vp[5];
   if_states(g, s, M, V, ID, d, c, Vm, Vp);
   double dir[3];
-  bimat_normal(g, s.rnormal, L.phi, d, c, idl, dir);
+  bimat_normal(g, s.rnormal, L, d, c, idl, idr, dir);
   // learned p* guess for this face (m2c_mlp.h), rejected when not a finite
   // positive pressure: the acoustic guess is then used
   double p0g = -1.0;
@@ -891,20 +924,45 @@ __global__ void k_end_reset(DevCounters* cnt) {
   cnt->n_unresolved = 0;
 }
 
-// UpdateMaterialIDByLevelSet, sign-change part (MultiPhaseOperator.cpp:177-221)
-__global__ void k_id_sweep(DevGrid g, Scheme s, const double* __restrict__ phi0, const double* __restrict__ phi,
-                           unsigned char* ID, unsigned char* tag, int* remaining, int cap, int lsid,
-                           DevCounters* cnt) {
+// every level set of a solver, for the routines that look at all of them
+struct LSPtrs {
+  double* phi[M2CX_MAX_LEVELSETS];
+  const double* phin[M2CX_MAX_LEVELSETS];  // Phi_tmp: the level sets at the start of the step
+  int mat[M2CX_MAX_LEVELSETS];
+  int n;
+  unsigned tracked;  // bit m: material m has a level set
+};
+
+// UpdateMaterialIDByLevelSet, sign-change part (MultiPhaseOperator.cpp:177-221):
+// a node whose phi changed sign or reached zero for some level sets takes
+// the material of the one it is now inside (two: error bit 16, the
+// reference exits); if it is inside none it is left to the vote
+__global__ void k_id_sweep(DevGrid g, Scheme s, LSPtrs P, unsigned char* ID, unsigned char* tag, int* remaining,
+                           int cap, DevCounters* cnt) {
   long n = (long)g.NX * g.NY * g.NZ;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     int i, j, k;
     cell_ijk(g, q, i, j, k);
     long c = gidx(g, i, j, k);
     if (ID[c] == s.inactive) continue;
-    double a = phi[c], b = phi0[c];
-    if (a * b > 0.0) continue;
-    if (a < 0) {
-      ID[c] = (unsigned char)lsid;
+    bool swept = false;
+    int myls = -1, inside = 0;
+    for (int l = 0; l < P.n; l++) {
+      const double a = P.phi[l][c], b = P.phin[l][c];
+      if (a * b > 0.0) continue;
+      swept = true;
+      if (a < 0) {
+        inside++;
+        myls = l;
+      }
+    }
+    if (!swept) continue;
+    if (inside > 1) {
+      atomicOr(&cnt->error, 16);
+      continue;
+    }
+    if (myls >= 0) {
+      ID[c] = (unsigned char)P.mat[myls];
     } else {
       tag[c] = 1;
       int slot = atomicAdd(&cnt->n_remaining, 1);
@@ -916,11 +974,13 @@ __global__ void k_id_sweep(DevGrid g, Scheme s, const double* __restrict__ phi0,
 
 // Neighbour vote for swept cells outside every level-set subdomain
 // (MultiPhaseOperator.cpp:236-338), Jacobi sweeps in one work-group.  With a
-// single material not tracked by a level set (all configurations here) every
-// candidate set is that material or empty, so the result is the same as the
-// reference's in-order sweep.
+// single material not tracked by a level set every candidate set is that
+// material or empty, so the result is the same as the reference's in-order
+// sweep; with several untracked materials a node decided in the same sweep
+// is seen by its neighbours one sweep later than in the reference.
 __global__ void __launch_bounds__(1024) k_id_vote(DevGrid g, Scheme s, unsigned char* ID, unsigned char* tag,
-                                                  const int* remaining, int cap, int lsid, DevCounters* cnt) {
+                                                  const int* remaining, int cap, unsigned tracked,
+                                                  DevCounters* cnt) {
   __shared__ int progress, left;
   const int nrem = min(cnt->n_remaining, cap);
   for (int iter = 0; iter < 100; iter++) {
@@ -951,7 +1011,7 @@ __global__ void __launch_bounds__(1024) k_id_vote(DevGrid g, Scheme s, unsigned 
             long nc = gidx(g, ni, nj, nk);
             if (tag[nc] & 1) continue;
             int nid = ID[nc];
-            if (nid == s.inactive || nid == lsid) continue;
+            if (nid == s.inactive || ((tracked >> nid) & 1u)) continue;
             if (nid < best) best = nid;
           }
       if (best < 256) tag[c] = (unsigned char)(1 | 2);  // decided this sweep
@@ -981,6 +1041,65 @@ __global__ void __launch_bounds__(1024) k_id_vote(DevGrid g, Scheme s, unsigned 
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nrem; q += blockDim.x) tag[remaining[q]] = 0;
+}
+
+// ResolveConflictsInLevelSets, part I (MultiPhaseOperator.cpp:1705-1762).
+// Only a node inside two or more subdomains can be changed, so those are
+// collected first (any order; the host sorts them); one thread then takes
+// them in the reference's (k, j, i) order, since a node's update changes the
+// sign tests of its neighbours, which the reference sees in that order.
+__global__ void k_lsr_reset(DevCounters* cnt) {
+  cnt->n_lsr = 0;
+  cnt->n_resolved = 0;
+}
+
+__global__ void k_lsr_mark(DevGrid g, LSPtrs P, int* list, int cap, DevCounters* cnt) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < g.ncells; q += (long)gridDim.x * blockDim.x) {
+    int neg = 0;
+    for (int l = 0; l < P.n; l++) neg += P.phi[l][q] < 0;
+    if (neg < 2) continue;
+    const int slot = atomicAdd(&cnt->n_lsr, 1);
+    if (slot < cap) list[slot] = (int)q;
+  }
+}
+
+__global__ void k_lsr_resolve(DevGrid g, LSPtrs P, const int* __restrict__ list, int n, DevCounters* cnt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int resolved = 0;
+  for (int t = 0; t < n; t++) {
+    const long c = list[t];
+    const unsigned u = (unsigned)c, sx = (unsigned)g.SX, sy = (unsigned)g.SY;
+    const unsigned rr = u / sx;
+    const int i = (int)(u - rr * sx) - 1;
+    const int k = (int)(rr / sy) - 1;
+    const int j = (int)(rr - (unsigned)(k + 1) * sy) - 1;
+    int owner[M2CX_MAX_LEVELSETS];
+    int nb = 0, no = 0;
+    for (int l = 0; l < P.n; l++) {
+      const double* ph = P.phi[l];
+      const double p = ph[c];
+      const bool bnd = (i - 1 >= -1 && p * ph[c - 1] <= 0) || (i + 1 <= g.NX && p * ph[c + 1] <= 0) ||
+                       (j - 1 >= -1 && p * ph[c - g.SX] <= 0) || (j + 1 <= g.NY && p * ph[c + g.SX] <= 0) ||
+                       (k - 1 >= -1 && p * ph[c - g.S] <= 0) || (k + 1 <= g.NZ && p * ph[c + g.S] <= 0);
+      if (!bnd) continue;
+      nb++;
+      if (p < 0) owner[no++] = l;
+    }
+    if (nb <= 1 || no <= 1) continue;  // the other branches act only with LevelSetCorrectionFrequency
+    int new_owner = owner[0];
+    double max_phi = fabs(P.phi[owner[0]][c]);
+    for (int w = 1; w < no; w++)
+      if (fabs(P.phi[owner[w]][c]) > max_phi) {
+        new_owner = owner[w];
+        max_phi = fabs(P.phi[owner[w]][c]);
+      }
+    double new_phi = 0.0;
+    for (int w = 0; w < no; w++) new_phi += fabs(P.phi[owner[w]][c]);
+    new_phi /= (double)no;
+    for (int w = 0; w < no; w++) P.phi[owner[w]][c] = (owner[w] == new_owner) ? -new_phi : new_phi;
+    resolved++;
+  }
+  cnt->n_resolved = resolved;
 }
 
 // UpdateMaterialIDAtGhostNodes (MultiPhaseOperator.cpp:919-942)
@@ -1329,8 +1448,10 @@ struct InitGeom {
   double V[5];
 };
 
+// state: write V and ID too (0: only phi, for level sets after the first)
 __global__ void k_initialize(DevGrid g, F5 V, unsigned char* ID, double* phi, const double* __restrict__ V0,
-                             int id0, const InitGeom* __restrict__ geo, int ngeo, int lsid, double diag) {
+                             int id0, const InitGeom* __restrict__ geo, int ngeo, int lsid, double diag,
+                             int state = 1) {
   const long n = (long)g.NX * g.NY * g.NZ;
   const double* gx = g_axis(g, 0);
   const double* gy = g_axis(g, 1);
@@ -1364,8 +1485,10 @@ __global__ void k_initialize(DevGrid g, F5 V, unsigned char* ID, double* phi, co
         else ph = G.interior ? dist : -dist;
       }
     }
-    store5(V, c, v);
-    ID[c] = (unsigned char)id;
+    if (state) {
+      store5(V, c, v);
+      ID[c] = (unsigned char)id;
+    }
     if (phi) phi[c] = ph;
   }
 }
@@ -1469,7 +1592,6 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
   k_levelset<<<blocks_for((long)g.NX * g.NY * g.NZ), 256, 0, st>>>(a);
   if (which == 0) {
     std::swap(B.phi, B.phitmp);
-    B.L.phi = B.phi;
   }
   if (which == 3) std::swap(B.phi1, B.phitmp);
   if (written) {
@@ -1481,13 +1603,12 @@ inline void mp_levelset(MPBuffers& B, const DevGrid& g, CF5 V, int which, DevCou
 }
 
 // UpdateMaterialIDByLevelSet(phin, phi) with the ghost IDs
-inline void mp_end_ids(MPBuffers& B, const DevGrid& g, const Scheme& s, unsigned char* ID, DevCounters* cnt,
-                       hipStream_t st) {
+inline void mp_end_ids(MPBuffers& B, const DevGrid& g, const Scheme& s, const LSPtrs& P, unsigned char* ID,
+                       DevCounters* cnt, hipStream_t st) {
   const long nin = (long)g.NX * g.NY * g.NZ;
   k_end_reset<<<1, 1, 0, st>>>(cnt);
-  k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, s, B.phin, B.phi, ID, B.tag, B.remaining, B.list_cap, B.lsid,
-                                              cnt);
-  k_id_vote<<<1, 1024, 0, st>>>(g, s, ID, B.tag, B.remaining, B.list_cap, B.lsid, cnt);
+  k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, s, P, ID, B.tag, B.remaining, B.list_cap, cnt);
+  k_id_vote<<<1, 1024, 0, st>>>(g, s, ID, B.tag, B.remaining, B.list_cap, P.tracked, cnt);
   long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
   k_ghost_ids<<<blocks_for(nf), 256, 0, st>>>(g, ID);
 }
@@ -1530,9 +1651,20 @@ inline void mp_end_fix(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, un
 }
 
 // the end-of-step update without a reinitializer
+// the level set of B alone (one level set per solver)
+inline LSPtrs mp_ls_single(const MPBuffers& B) {
+  LSPtrs P{};
+  P.phi[0] = B.phi;
+  P.phin[0] = B.phin;
+  P.mat[0] = B.lsid;
+  P.n = 1;
+  P.tracked = 1u << B.lsid;
+  return P;
+}
+
 inline void mp_end_step(MPBuffers& B, const DevGrid& g, const Scheme& s, F5 V, unsigned char* ID, DevCounters* cnt,
                         hipStream_t st) {
-  mp_end_ids(B, g, s, ID, cnt, st);
+  mp_end_ids(B, g, s, mp_ls_single(B), ID, cnt, st);
   mp_end_states(B, g, s, V, ID, cnt, st);
   mp_end_fix(B, g, s, V, ID, cnt, st);
 }

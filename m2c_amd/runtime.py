@@ -52,7 +52,13 @@ def lib():
         L.m2cx_download_ls_residual.argtypes = [P, D]
         L.m2cx_levelset_reinitialize.argtypes = [P, C.c_int]
         L.m2cx_levelset_construct_narrow_band.argtypes = [P]
-        L.m2cx_get_narrow_band.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), D]
+        L.m2cx_get_narrow_band.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
+                                           C.POINTER(C.c_uint8), D]
+        L.m2cx_create_levelsets.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int,
+                                            C.POINTER(abi.Scheme), C.POINTER(abi.Levelset), C.c_int, C.c_int,
+                                            C.POINTER(P)]
+        L.m2cx_upload_levelset.argtypes = [P, C.c_int, D]
+        L.m2cx_download_levelset.argtypes = [P, C.c_int, C.c_int, D]
         L.m2cx_set_overlap.argtypes = [P, C.c_int]
         L.m2cx_advance_one_time_step.argtypes = [P, C.c_double, C.c_double, C.c_int]
         L.m2cx_run_steps.argtypes = [P, C.c_int]
@@ -100,11 +106,18 @@ def _i(a):
 class Solver:
     """One device-resident explicit-step solver (one GPU)."""
 
-    def __init__(self, grid, mats, nmat, scheme, device=0):
+    def __init__(self, grid, mats, nmat, scheme, device=0, levelsets=None):
+        """levelsets: a sequence of abi.Levelset (several level sets,
+        m2cx_create_levelsets), or None for the scheme's level set 0"""
         self.L = lib()
         self.grid, self.mats, self.nmat, self.scheme = grid, mats, nmat, scheme
         h = P()
-        rc = self.L.m2cx_create(C.byref(grid), mats, nmat, C.byref(scheme), device, C.byref(h))
+        if levelsets is None:
+            rc = self.L.m2cx_create(C.byref(grid), mats, nmat, C.byref(scheme), device, C.byref(h))
+        else:
+            self._ls = abi.levelsets(*levelsets)
+            rc = self.L.m2cx_create_levelsets(C.byref(grid), mats, nmat, C.byref(scheme), self._ls, len(levelsets),
+                                              device, C.byref(h))
         self.h = h
         self._check(rc)
         self.shape = (grid.nz + 2, grid.ny + 2, grid.nx + 2)
@@ -199,15 +212,24 @@ class Solver:
         set_state; initialize() builds it itself)."""
         return self._check(self.L.m2cx_levelset_construct_narrow_band(self.h))
 
-    def narrow_band(self):
+    def upload_levelset(self, q, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(self.L.m2cx_upload_levelset(self.h, q, _d(phi)))
+
+    def download_levelset(self, q, what=abi.LS_PHI):
+        out = np.zeros(self.shape)
+        self._check(self.L.m2cx_download_levelset(self.h, q, what, _d(out)))
+        return out
+
+    def narrow_band(self, ls=0):
         """level (int32), useful and active (bool) per ghosted cell, and
-        (phi_max, phi_min, phi_out_pos, phi_out_neg)"""
+        (phi_max, phi_min, phi_out_pos, phi_out_neg) of level set `ls`"""
         level = np.zeros(self.shape, dtype=np.int32)
         useful = np.zeros(self.shape, dtype=np.uint8)
         active = np.zeros(self.shape, dtype=np.uint8)
         b = np.zeros(4)
         U8 = C.POINTER(C.c_uint8)
-        self._check(self.L.m2cx_get_narrow_band(self.h, level.ctypes.data_as(C.POINTER(C.c_int32)),
+        self._check(self.L.m2cx_get_narrow_band(self.h, ls, level.ctypes.data_as(C.POINTER(C.c_int32)),
                                                 useful.ctypes.data_as(U8), active.ctypes.data_as(U8), _d(b)))
         return level, useful.astype(bool), active.astype(bool), tuple(b)
 

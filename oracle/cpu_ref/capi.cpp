@@ -22,6 +22,48 @@ orc_solver* orc_create(const m2cx_grid* g, const m2cx_material* mats, int nmat, 
   return h;
 }
 
+// several level sets (m2cx_create_levelsets)
+orc_solver* orc_create_levelsets(const m2cx_grid* g, const m2cx_material* mats, int nmat, const m2cx_scheme* sch,
+                                 const m2cx_levelset* ls, int nls) {
+  auto* h = new orc_solver;
+  m2cx_scheme s = *sch;
+  s.n_levelsets = nls;
+  if (nls > 0) SchemeSetLS(s, ls[0]);
+  h->s.reset(new Solver(*g, mats, nmat, s));
+  h->s->SetLevelSets(ls, nls);
+  return h;
+}
+
+// level set q (what = 0) or its copy at the start of the step, Phi_tmp
+// (what = 2)
+int orc_set_levelset(orc_solver* h, int q, int what, const double* phi) {
+  Solver& s = *h->s;
+  if (q < 0 || q >= std::max(s.nls, 1)) return -2;
+  s.SwapLS(q);
+  std::memcpy((what == 2 ? s.Phi_tmp : s.Phi).data(), phi, s.m.ncells() * 8);
+  s.SwapLS(q);
+  return 0;
+}
+
+// MultiPhaseOperator::UpdateMaterialIDByLevelSet from Phi_tmp to Phi
+int orc_update_material_ids(orc_solver* h) {
+  Solver& s = *h->s;
+  s.UpdateMaterialIDByLevelSets();
+  return s.fatal ? -1 : 0;
+}
+
+// MultiPhaseOperator::ResolveConflictsInLevelSets; returns the nodes resolved
+int orc_resolve_conflicts(orc_solver* h) { return h->s->ResolveConflictsInLevelSets(); }
+
+int orc_get_levelset(orc_solver* h, int q, int what, double* out) {
+  Solver& s = *h->s;
+  if (q < 0 || q >= std::max(s.nls, 1)) return -2;
+  s.SwapLS(q);
+  copy_field(what == 1 ? s.Rls : s.Phi, out);
+  s.SwapLS(q);
+  return 0;
+}
+
 void orc_destroy(orc_solver* h) { delete h; }
 
 const char* orc_error(orc_solver* h) { return h->s->msg.c_str(); }
@@ -82,11 +124,16 @@ int orc_residual(orc_solver* h, double* R, int capture) {
 
 int orc_apply_bc(orc_solver* h) { h->s->ApplyBoundaryConditions(h->s->V); return 0; }
 int orc_update_ghost_ids(orc_solver* h) { h->s->UpdateMaterialIDAtGhostNodes(); return 0; }
-int orc_ls_apply_bc(orc_solver* h) { h->s->LSApplyBoundaryConditions(h->s->Phi); return 0; }
+int orc_ls_apply_bc(orc_solver* h) {
+  Solver& s = *h->s;
+  s.ForEachLS([&](int) { s.LSApplyBoundaryConditions(s.Phi); });
+  return 0;
+}
 
+// residual of every level set; `out` receives level set 0's
 int orc_ls_residual(orc_solver* h, double* out) {
   Solver& s = *h->s;
-  s.LSComputeResidual(s.V, s.Phi, s.Rls);
+  s.ForEachLS([&](int) { s.LSComputeResidual(s.V, s.Phi, s.Rls); });
   if (out) copy_field(s.Rls, out);
   return 0;
 }
@@ -138,7 +185,7 @@ int orc_bimaterial_normal(orc_solver* h, int d, int i, int j, int k, int myid, i
 // LevelSetOperator::Reinitialize with must_do = true
 int orc_reinitialize(orc_solver* h, int special_maxits) {
   Solver& s = *h->s;
-  s.Reinitialize(0.0, 1.0, 0, special_maxits, true);
+  s.ForEachLS([&](int) { s.Reinitialize(0.0, 1.0, 0, special_maxits, true); });
   return s.fatal ? -1 : 0;
 }
 
@@ -148,16 +195,24 @@ double orc_reinit_cfl(orc_solver* h) { return h->s->reinit_cfl; }
 // 1328-1334), after the level set and its boundary values are set
 int orc_construct_narrow_band(orc_solver* h) {
   Solver& s = *h->s;
-  if (!s.nb_bw) return -2;
-  s.ConstructNarrowBand();
+  int any = 0;
+  s.ForEachLS([&](int) {
+    if (!s.nb_bw) return;
+    s.ConstructNarrowBand();
+    any = 1;
+  });
+  if (!any) return -2;
   return s.fatal ? -1 : 0;
 }
 
 // The band of level set 0: level per ghosted cell (31415927 outside), the
 // useful and active masks, and phi_max, phi_min, phi_out_pos, phi_out_neg.
-int orc_get_narrow_band(orc_solver* h, int* level, unsigned char* useful, unsigned char* active,
+int orc_get_narrow_band(orc_solver* h, int ls, int* level, unsigned char* useful, unsigned char* active,
                         double* bounds) {
   Solver& s = *h->s;
+  if (ls < 0 || ls >= std::max(s.nls, 1)) return -2;
+  s.SwapLS(ls);
+  struct Back { Solver& s; int q; ~Back() { s.SwapLS(q); } } back{s, ls};
   if (!s.nb_built) return -2;
   const long n = s.m.ncells();
   if (level) std::memcpy(level, s.nbLevel.data(), n * 4);

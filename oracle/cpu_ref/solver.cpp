@@ -85,20 +85,12 @@ Solver::Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat_, const m
   ID.assign(n, 0);
   IDn.assign(n, 0);
   for (auto* f : {&Phi, &Phi1, &Phi_tmp, &Rls}) f->assign(n, 0.0);
-  reinit_cfl = s.reinit_cfl;  // cfl = iod_ls.reinit.cfl (LevelSetReinitializer.cpp:39)
   // domain_diagonal (Main.cpp:207-209)
   domain_diagonal = std::sqrt(std::pow(g.xmax - g.x0, 2) + std::pow(g.ymax - g.y0, 2) + std::pow(g.zmax - g.z0, 2));
   phi_max = -domain_diagonal;
   phi_min = domain_diagonal;
-  nb_bw = (s.n_levelsets > 0 && s.ls_bandwidth > 0 && s.ls_bandwidth < INT32_MAX) ? s.ls_bandwidth : 0;
-  if (nb_bw) {
-    // LevelSetOperator.cpp:84-99
-    if (nb_bw <= 1) Fail("in the narrow-band level set method, bandwidth should be at least 3");
-    else if (s.reinit_frequency <= 0 || s.reinit_frequency >= nb_bw)
-      Fail("the level set reinitialization frequency should be smaller than bandwidth");
-    else if (s.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME)
-      Fail("the narrow-band level set is built for the finite-difference level-set solver only");
-  }
+  nls = s.n_levelsets;
+  DeriveLevelSet();  // cfl = iod_ls.reinit.cfl (LevelSetReinitializer.cpp:39), the band
 }
 
 // MeshGenerator.cpp:161-173, SpaceOperator.cpp:173-350
@@ -545,8 +537,8 @@ static inline double central_diff3(double phi0, double phi1, double phi2, double
 // 2272-2415): grad phi at face d of cell (i, j, k) (between i-1 and i for
 // d = 0), normal part by the two cells, tangential parts from central
 // differences in the two columns weighted to the face, normalized.
-void Solver::GradPhiAtFace(int d, int i, int j, int k, double* dir) const {
-  auto P = [&](int a, int b, int c) { return Phi[m.idx(a, b, c)]; };
+void Solver::GradPhiAtFace(const std::vector<double>& phi, int d, int i, int j, int k, double* dir) const {
+  auto P = [&](int a, int b, int c) { return phi[m.idx(a, b, c)]; };
   auto X = [&](int a) { return m.X(a); };
   auto Y = [&](int a) { return m.Y(a); };
   auto Z = [&](int a) { return m.Z(a); };
@@ -586,18 +578,47 @@ void Solver::GradPhiAtFace(int d, int i, int j, int k, double* dir) const {
 }
 
 // SpaceOperator::GetNormalForBimaterialRiemann and CalculateGradPhiAtCellInterface
-// (SpaceOperator.cpp:2186-2270) with one level set: the face's direction of the
-// 1D Riemann problem (myid: this cell, neighborid: the cell on the minus side)
+// (SpaceOperator.cpp:2186-2270): the face's direction of the 1D Riemann
+// problem (myid: this cell, neighborid: the cell on the minus side).  One
+// level set is involved when one side is material 0 (its gradient, turned
+// when the neighbour is material 0), two otherwise (the difference of their
+// unit gradients, normalized).
 void Solver::BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const {
   dir[0] = dir[1] = dir[2] = 0.0;
   if (sch.riemann_normal == M2CX_RNORMAL_MESH) {
     dir[d] = 1.0;
     return;
   }
-  GradPhiAtFace(d, i, j, k, dir);
-  if (neighborid == 0)
-    for (int q = 0; q < 3; q++) dir[q] *= -1.0;
-  if (std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  const int L = std::max(nls, 1);
+  const int my_mat = (myid == 0) ? neighborid : myid;
+  const int nb_mat = (neighborid == 0) ? myid : neighborid;
+  int my_ls = -1, nb_ls = -1;
+  for (int q = 0; q < L; q++) {
+    if (my_ls < 0 && MatOf(q) == my_mat) my_ls = q;
+    if (nb_ls < 0 && MatOf(q) == nb_mat) nb_ls = q;
+  }
+  if (my_ls < 0 || nb_ls < 0) {  // the reference asserts
+    dir[d] = 1.0;
+    return;
+  }
+  if (my_ls == nb_ls) {
+    GradPhiAtFace(PhiOf(my_ls), d, i, j, k, dir);
+    if (neighborid == 0)
+      for (int q = 0; q < 3; q++) dir[q] *= -1.0;
+    if (std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]) == 0.0) dir[d] = 1.0;
+  } else {
+    double n1[3], n2[3];
+    GradPhiAtFace(PhiOf(my_ls), d, i, j, k, n1);
+    GradPhiAtFace(PhiOf(nb_ls), d, i, j, k, n2);
+    for (int q = 0; q < 3; q++) dir[q] = n2[q] - n1[q];
+    const double nr = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (nr != 0.0) {
+      const double inv = 1.0 / nr;  // Vec3D::operator/= multiplies by the reciprocal
+      dir[0] *= inv; dir[1] *= inv; dir[2] *= inv;
+    } else {
+      dir[d] = 1.0;
+    }
+  }
   if (dir[d] < 0.0) dir[d] *= -1.0;  // aligned with the positive grid axis
   if (sch.riemann_normal == M2CX_RNORMAL_AVERAGE) {
     dir[d] += 1.0;
@@ -888,56 +909,9 @@ void Solver::LSComputeResidual(const std::vector<double>& Vin, const std::vector
 }
 
 // ---------------------------------------------------------------------------
-// MultiPhaseOperator (MultiPhaseOperator.cpp:153-357, 919-942, 980-1150,
-// 1278-1525), one level set, no embedded surfaces.
+// MultiPhaseOperator (MultiPhaseOperator.cpp:919-942, 980-1150, 1278-1525),
+// no embedded surfaces; the material-ID update is in levelsets.cpp.
 // ---------------------------------------------------------------------------
-void Solver::UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi) {
-  const int lsid = sch.ls_materialid;
-  std::vector<char> tag(m.ncells(), 0);
-  // remaining nodes in std::set<Int3> order: i, then j, then k
-  std::set<std::array<int, 3>> remaining;
-  for (int k = 0; k < m.NZ; k++)
-    for (int j = 0; j < m.NY; j++)
-      for (int i = 0; i < m.NX; i++) {
-        long c = m.idx(i, j, k);
-        tag[c] = 0;
-        if (ID[c] == inactive_id) continue;
-        if (phi[c] * phi0[c] > 0.0) continue;
-        if (phi[c] < 0) ID[c] = lsid;
-        else { tag[c] = 1; remaining.insert({i, j, k}); }
-      }
-  for (int iter = 0; iter < 100; iter++) {
-    if (remaining.empty()) break;
-    int progress = 0;
-    std::set<std::array<int, 3>> tmp = remaining;
-    for (auto& ijk : tmp) {
-      int i = ijk[0], j = ijk[1], k = ijk[2];
-      std::set<int> cand;
-      for (int nk = k - 1; nk <= k + 1; nk++)
-        for (int nj = j - 1; nj <= j + 1; nj++)
-          for (int ni = i - 1; ni <= i + 1; ni++) {
-            if (!m.inside(ni, nj, nk)) continue;
-            long nc = m.idx(ni, nj, nk);
-            if (tag[nc] == 1) continue;
-            int nid = ID[nc];
-            if (nid == inactive_id || nid == lsid) continue;
-            cand.insert(nid);
-          }
-      if (cand.empty()) continue;
-      long c = m.idx(i, j, k);
-      ID[c] = *cand.begin();
-      tag[c] = 0;
-      remaining.erase(ijk);
-      progress++;
-    }
-    if (progress == 0) {
-      for (auto& ijk : tmp) ID[m.idx(ijk[0], ijk[1], ijk[2])] = 0;
-      break;
-    }
-  }
-  UpdateMaterialIDAtGhostNodes();
-}
-
 void Solver::UpdateMaterialIDAtGhostNodes() {
   for (auto& gp : ghosts) ID[m.idx(gp.i, gp.j, gp.k)] = ID[m.idx(gp.ii, gp.jj, gp.kk)];
 }
@@ -1088,9 +1062,11 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
   const bool ls = sch.n_levelsets > 0;
   // Phi_tmp = Phi(n) including ghosts; with the narrow band only the useful
   // nodes after the first step (TimeIntegrator.cpp:333-339)
-  if (ls && (!nb_bw || time_step == 1)) Phi_tmp = Phi;
-  else if (ls)
-    for (long c : useful_nodes) Phi_tmp[c] = 0.0 * Phi_tmp[c] + 1.0 * Phi[c];
+  if (ls) ForEachLS([&](int) {
+    if (!nb_bw || time_step == 1) Phi_tmp = Phi;
+    else
+      for (long c : useful_nodes) Phi_tmp[c] = 0.0 * Phi_tmp[c] + 1.0 * Phi[c];
+  });
   if (sch.integrator == M2CX_TI_FORWARD_EULER) {
     ComputeResidual(V, R, true);
     if (fatal) return;
@@ -1100,12 +1076,12 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     ClipDensityAndPressure(V);
     if (fatal) return;
     ApplyBoundaryConditions(V);
-    if (ls) {
+    if (ls) ForEachLS([&](int) {
       if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
       LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + dt * Rls[c]; });
       LSApplyBoundaryConditions(Phi);
-    }
+    });
   } else {
     const bool rk3 = (sch.integrator == M2CX_TI_RK3);
     auto interior = [&](auto fn) {
@@ -1124,13 +1100,13 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
     if (fatal) return;
     if (clipped) PrimitiveToConservative(V1, U1);
     ApplyBoundaryConditions(V1);
-    if (ls) {
+    if (ls) ForEachLS([&](int) {
       if (ls_test) PrescribeVelocityForTesting(V, Phi, time - dt);
       LSComputeResidual(V, Phi, Rls);
       LSInterior([&](long c) { Phi1[c] = 0.0 * Phi1[c] + 1.0 * Phi[c]; });
       LSInterior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + dt * Rls[c]; });
       LSApplyBoundaryConditions(Phi1);
-    }
+    });
     if (!rk3) {
       // stage 2 of RK2
       ComputeResidual(V1, R, false);
@@ -1142,13 +1118,13 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       ClipDensityAndPressure(V);
       if (fatal) return;
       ApplyBoundaryConditions(V);
-      if (ls) {
+      if (ls) ForEachLS([&](int) {
         if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
         LSInterior([&](long c) { Phi[c] = 0.5 * Phi[c] + 0.5 * Phi1[c]; });
         LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + hdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi);
-      }
+      });
     } else {
       // stage 2 of RK3
       ComputeResidual(V1, R, false);
@@ -1161,13 +1137,13 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       if (fatal) return;
       if (clipped) PrimitiveToConservative(V2, U1);
       ApplyBoundaryConditions(V2);
-      if (ls) {
+      if (ls) ForEachLS([&](int) {
         if (ls_test) PrescribeVelocityForTesting(V1, Phi1, time);
         LSComputeResidual(V1, Phi1, Rls);
         LSInterior([&](long c) { Phi1[c] = 0.25 * Phi1[c] + 0.75 * Phi[c]; });
         LSInterior([&](long c) { Phi1[c] = 1.0 * Phi1[c] + qdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi1);
-      }
+      });
       // stage 3
       ComputeResidual(V2, R, false);
       if (fatal) return;
@@ -1178,13 +1154,13 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
       ClipDensityAndPressure(V);
       if (fatal) return;
       ApplyBoundaryConditions(V);
-      if (ls) {
+      if (ls) ForEachLS([&](int) {
         if (ls_test) PrescribeVelocityForTesting(V2, Phi1, time - 0.5 * dt);
         LSComputeResidual(V2, Phi1, Rls);
         LSInterior([&](long c) { Phi[c] = 1.0 / 3.0 * Phi[c] + 2.0 / 3.0 * Phi1[c]; });
         LSInterior([&](long c) { Phi[c] = 1.0 * Phi[c] + tdt * Rls[c]; });
         LSApplyBoundaryConditions(Phi);
-      }
+      });
     }
   }
   if (ls) UpdateSolutionAfterTimeStepping();
@@ -1194,9 +1170,18 @@ void Solver::AdvanceOneTimeStep(double time, double dt) {
 // 720-795) for one level set and no embedded surfaces: ID update, the
 // level-set reinitialization at its frequency, then the state update.
 void Solver::UpdateSolutionAfterTimeStepping() {
+  // overlapping subdomains of several level sets (:737-741)
+  const int resolved = ResolveConflictsInLevelSets();
+  if (resolved) ForEachLS([&](int) { LSApplyBoundaryConditions(Phi); });
   IDn = ID;
-  if (!ls_test) UpdateMaterialIDByLevelSet(Phi_tmp, Phi);
-  Reinitialize(t, dts, time_step, 0, false);
+  if (!ls_test) UpdateMaterialIDByLevelSets();
+  if (fatal) return;
+  // reinitialization: all level sets once enough overlaps were resolved,
+  // otherwise each at its own frequency (:757-768)
+  ls_correction_bucket += resolved;
+  const bool must = ls_correction_bucket >= sch.mp_corrections_until_reinit;
+  if (must) ls_correction_bucket = 0;
+  ForEachLS([&](int) { Reinitialize(t, dts, time_step, 0, must); });
   if (fatal || ls_test) return;
   if (sch.mp_phase_change == M2CX_PHASECHANGE_EXTRAPOLATION)
     UpdateStateVariablesByExtrapolation();

@@ -58,6 +58,10 @@ struct Counters {
 
 // A first-layer node of the level-set reinitialization with its neighbours
 // across the interface (LevelSetReinitializer.h:55-81).
+// a LevelSet[i] block taken from / put into the LevelSet[0] fields of a scheme
+m2cx_levelset LSFromScheme(const m2cx_scheme& s);
+void SchemeSetLS(m2cx_scheme& s, const m2cx_levelset& L);
+
 struct FirstLayerNode {
   int i, j, k;
   bool s[6];
@@ -118,7 +122,7 @@ class Solver {
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   void LSComputeResidualFVM(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   // direction of the 1D Riemann problem at material interfaces
-  void GradPhiAtFace(int d, int i, int j, int k, double* dir) const;
+  void GradPhiAtFace(const std::vector<double>& phi, int d, int i, int j, int k, double* dir) const;
   void BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const;
   // LevelSetReinitializer, full domain (LevelSetReinitializer.cpp:76-176)
   bool ReinitEnabled() const { return sch.reinit_frequency > 0 || sch.reinit_frequency_dt > 0; }
@@ -138,8 +142,39 @@ class Solver {
   std::vector<long> useful_nodes, active_nodes, plus1_nodes;
   double domain_diagonal = 0.0, phi_max = 0.0, phi_min = 0.0, phi_out_pos = 0.0, phi_out_neg = 0.0;
   void ConstructNarrowBand();
+  // Several level sets: level set 0 lives in the members above; level set
+  // q >= 1 in lsx[q - 1], exchanged with them by SwapLS(q) while its own
+  // operator runs (swapping again puts it back).
+  struct LSState {
+    m2cx_levelset set{};
+    std::vector<double> Phi, Phi1, Phi_tmp, Rls, rTag, rSign, rPhibk, rPhi0, rPhi1, rR, rNormal, rPhiG2;
+    std::vector<int> LSK3, nbLevel;
+    std::vector<unsigned char> nbUseful, nbActive;
+    std::vector<long> useful_nodes, active_nodes, plus1_nodes;
+    int nb_bw = 0;
+    bool nb_built = false;
+    double phi_max = 0, phi_min = 0, phi_out_pos = 0, phi_out_neg = 0, reinit_cfl = 0.8;
+  };
+  int nls = 0;
+  std::vector<LSState> lsx;
+  int ls_correction_bucket = 0;  // TimeIntegrator.h:31
+  void SetLevelSets(const m2cx_levelset* ls, int n);
+  void DeriveLevelSet();
+  void SwapLS(int q);
+  template <class F>
+  void ForEachLS(F fn) {
+    for (int q = 0; q < nls; q++) {
+      SwapLS(q);
+      fn(q);
+      SwapLS(q);
+    }
+  }
+  const std::vector<double>& PhiOf(int q) const;
+  const std::vector<double>& PhiTmpOf(int q) const;
+  int MatOf(int q) const;
+  void UpdateMaterialIDByLevelSets();
+  int ResolveConflictsInLevelSets();
   // MultiPhaseOperator
-  void UpdateMaterialIDByLevelSet(const std::vector<double>& phi0, const std::vector<double>& phi);
   void UpdateMaterialIDAtGhostNodes();
   int UpdateStateVariablesByRiemannSolutions();
   int UpdateStateVariablesByExtrapolation();

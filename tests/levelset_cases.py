@@ -170,3 +170,45 @@ def doubled_plane(n=(24, 20, 16), flt=abi.REINIT_FIXED, maxits=400):
     V[..., 4] = 1.0
     ID = (phi < 0).astype(np.int32)
     return g, mats, 2, s, V, ID, phi, d
+
+
+def two_bubbles(n=(48, 16, 16), mat2=None, centres=((0.5, 0.5, 0.5), (1.5, 0.5, 0.5)), radii=(0.2, 0.2),
+                rnormal=abi.RNORMAL_MESH, integrator=abi.TI_RK3, reinit_frequency=-1, bandwidth=abi.INT32_MAX,
+                first_layer=abi.REINIT_FIXED, shocks=(0.2, 1.8)):
+    """Water (SG) on [0, 2] x [0, 1] x [0, 1] with an air bubble (material 1,
+    level set 0) and a second bubble of material 2 (level set 1; air again
+    unless `mat2` is given), hit by Mach-1.5 shocks from both ends.  Returns
+    grid, materials, nmat, scheme, V, ID, [phi_0, phi_1] (signed distances,
+    ghosts included), the two m2cx_levelset blocks and the signed distance to
+    the union of the bubbles."""
+    g = abi.grid(n, (0.0, 0.0, 0.0), (2.0, 1.0, 1.0))
+    air = abi.material("SG", **configs.AIR_SG)
+    m2 = air if mat2 is None else mat2
+    mats = abi.materials(abi.material("SG", **configs.WATER_SG), air, m2)
+    s = abi.default_scheme(flux=abi.FLUX_HLLC, limiter=abi.LIM_VANALBADA, integrator=integrator, cfl=0.4,
+                           n_levelsets=1, ls_materialid=1, riem_tol_main=1e-10, riemann_normal=rnormal,
+                           reinit_frequency=reinit_frequency, reinit_maxits=25, reinit_first_layer=first_layer,
+                           ls_bandwidth=bandwidth)
+    x = configs._axis(n[0], 0.0, 2.0)[None, None, :]
+    y = configs._axis(n[1], 0.0, 1.0)[None, :, None]
+    z = configs._axis(n[2], 0.0, 1.0)[:, None, None]
+    shape = (n[2] + 2, n[1] + 2, n[0] + 2)
+    rho_w, p_w = 1000.0, 1.0e5
+    rho2, u2, p2 = configs.sg_post_shock(rho_w, p_w, configs.WATER_SG["gamma"], configs.WATER_SG["pc"], 1.5)
+    V = np.empty(shape + (5,))
+    V[...] = [rho_w, 0.0, 0.0, 0.0, p_w]
+    xb = np.broadcast_to(x, shape)
+    V[xb < shocks[0]] = [rho2, u2, 0.0, 0.0, p2]
+    V[xb > shocks[1]] = [rho2, -u2, 0.0, 0.0, p2]
+    ID = np.zeros(shape, dtype=np.int32)
+    phis = []
+    for q, (c, r) in enumerate(zip(centres, radii)):
+        d = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - r
+        phis.append(np.array(np.broadcast_to(d, shape), dtype=np.float64))
+    for q in (1, 0):  # the first bubble wins where they overlap
+        inside = phis[q] < 0
+        V[inside] = [1.2, 0.0, 0.0, 0.0, p_w]
+        ID[inside] = q + 1
+    union = np.minimum(phis[0], phis[1])
+    ls = [abi.levelset_from_scheme(s, materialid=1), abi.levelset_from_scheme(s, materialid=2)]
+    return g, mats, 3, s, V, ID, phis, ls, union

@@ -66,7 +66,14 @@ def lib():
         L.orc_reinit_cfl.argtypes = [P]
         L.orc_set_ls_test.argtypes = [P, C.c_int]
         L.orc_construct_narrow_band.argtypes = [P]
-        L.orc_get_narrow_band.argtypes = [P, I, C.POINTER(C.c_ubyte), C.POINTER(C.c_ubyte), D]
+        L.orc_get_narrow_band.argtypes = [P, C.c_int, I, C.POINTER(C.c_ubyte), C.POINTER(C.c_ubyte), D]
+        L.orc_create_levelsets.restype = P
+        L.orc_create_levelsets.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int,
+                                           C.POINTER(abi.Scheme), C.POINTER(abi.Levelset), C.c_int]
+        L.orc_set_levelset.argtypes = [P, C.c_int, C.c_int, D]
+        L.orc_update_material_ids.argtypes = [P]
+        L.orc_resolve_conflicts.argtypes = [P]
+        L.orc_get_levelset.argtypes = [P, C.c_int, C.c_int, D]
         L.orc_bimaterial_normal.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D]
         _lib = L
     return _lib
@@ -83,10 +90,17 @@ def ip(a):
 class Oracle:
     """One CPU restatement instance on a grid (single rank)."""
 
-    def __init__(self, grid, mats, nmat, scheme):
+    def __init__(self, grid, mats, nmat, scheme, levelsets=None):
+        """levelsets: a sequence of abi.Levelset (several level sets), or None
+        for the scheme's own level set 0"""
         self.L = lib()
         self.grid, self.mats, self.nmat, self.scheme = grid, mats, nmat, scheme
-        self.h = self.L.orc_create(C.byref(grid), mats, nmat, C.byref(scheme))
+        if levelsets is None:
+            self.h = self.L.orc_create(C.byref(grid), mats, nmat, C.byref(scheme))
+        else:
+            self._ls = abi.levelsets(*levelsets)
+            self.h = self.L.orc_create_levelsets(C.byref(grid), mats, nmat, C.byref(scheme), self._ls,
+                                                 len(levelsets))
         self.shape = (grid.nz + 2, grid.ny + 2, grid.nx + 2)
 
     def __del__(self):
@@ -185,17 +199,34 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("construct_narrow_band: %d %s" % (rc, self.error()))
 
-    def narrow_band(self):
+    def narrow_band(self, ls=0):
         """level (int32), useful and active (bool) per ghosted cell, and
-        (phi_max, phi_min, phi_out_pos, phi_out_neg)"""
+        (phi_max, phi_min, phi_out_pos, phi_out_neg) of level set `ls`"""
         level = np.zeros(self.shape, dtype=np.int32)
         useful = np.zeros(self.shape, dtype=np.uint8)
         active = np.zeros(self.shape, dtype=np.uint8)
         b = np.zeros(4)
         U8 = C.POINTER(C.c_ubyte)
-        assert self.L.orc_get_narrow_band(self.h, ip(level), useful.ctypes.data_as(U8), active.ctypes.data_as(U8),
-                                          dp(b)) == 0
+        assert self.L.orc_get_narrow_band(self.h, ls, ip(level), useful.ctypes.data_as(U8),
+                                          active.ctypes.data_as(U8), dp(b)) == 0
         return level, useful.astype(bool), active.astype(bool), tuple(b)
+
+    def set_levelset(self, q, phi, tmp=False):
+        """level set q, or (tmp) its copy at the start of the step"""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert self.L.orc_set_levelset(self.h, q, 2 if tmp else 0, dp(phi)) == 0
+
+    def update_material_ids(self):
+        if self.L.orc_update_material_ids(self.h) < 0:
+            raise RuntimeError(self.error())
+
+    def resolve_conflicts(self):
+        return self.L.orc_resolve_conflicts(self.h)
+
+    def levelset(self, q, what=abi.LS_PHI):
+        out = np.zeros(self.shape)
+        assert self.L.orc_get_levelset(self.h, q, what, dp(out)) == 0
+        return out
 
     def set_ls_test(self, test):
         """Prescribed-velocity level-set test 1 (slotted disk), 2 (vortex

@@ -54,7 +54,7 @@ def test_argument_errors_need_no_gpu():
 
 _PREFIXES = ("EOS_", "BC_", "LSBC_", "FLUX_", "REC_", "LIM_", "SLOPE_", "VAR_", "TI_", "MPFLUX_", "MPREC_", "GUESS_",
              "FIELD_", "E_", "GEOM_", "RIEMANN_", "REINIT_", "RNORMAL_", "PHASECHANGE_", "LSSOLVER_",
-             "LSFLUX_")
+             "LSFLUX_", "LS_", "MAX_LEVELSETS")
 
 
 def _consts():
@@ -70,7 +70,8 @@ def _c_layout(tmp_path):
     as the C compiler sees them."""
     checks = []
     for cname, py in (("m2cx_material", abi.Material), ("m2cx_grid", abi.Grid), ("m2cx_scheme", abi.Scheme),
-                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry)):
+                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry),
+                      ("m2cx_levelset", abi.Levelset)):
         checks.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             checks.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
@@ -88,7 +89,8 @@ def _c_layout(tmp_path):
 def test_ctypes_mirror_matches_header(tmp_path):
     got = _c_layout(tmp_path)
     for cname, py in (("m2cx_material", abi.Material), ("m2cx_grid", abi.Grid), ("m2cx_scheme", abi.Scheme),
-                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry)):
+                      ("m2cx_counters", abi.Counters), ("m2cx_geometry", abi.Geometry),
+                      ("m2cx_levelset", abi.Levelset)):
         assert got[f"{cname} size"] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[f"{cname}.{f}"] == getattr(py, f).offset, (cname, f)
