@@ -453,6 +453,8 @@ static int create_impl(const m2cx_grid* grid, const m2cx_material* mats, int nma
   if (sch->var_type < M2CX_VAR_PRIMITIVE || sch->var_type > M2CX_VAR_CONSERVATIVE_CHARACTERISTIC) return M2CX_E_ARG;
   if (sch->n_levelsets < 0 || sch->n_levelsets > (several ? M2CX_MAX_LEVELSETS : 1)) return M2CX_E_UNSUPPORTED;
   if (sch->n_levelsets > 0 && (sch->ls_materialid <= 0 || sch->ls_materialid >= nmat)) return M2CX_E_ARG;
+  // MultiPhase.CorrectionsUntilReinitialization at least 1 (TimeIntegrator.cpp:46-49)
+  if (sch->n_levelsets > 0 && sch->mp_corrections_until_reinit < 1) return M2CX_E_ARG;
   if (sch->reinit_first_layer < M2CX_REINIT_FIXED || sch->reinit_first_layer > M2CX_REINIT_ITERATIVE_CONSTRAINED2)
     return M2CX_E_ARG;
   if (sch->riemann_normal < M2CX_RNORMAL_LEVEL_SET || sch->riemann_normal > M2CX_RNORMAL_AVERAGE) return M2CX_E_ARG;

@@ -90,6 +90,9 @@ Solver::Solver(const m2cx_grid& g, const m2cx_material* mats, int nmat_, const m
   phi_max = -domain_diagonal;
   phi_min = domain_diagonal;
   nls = s.n_levelsets;
+  // TimeIntegrator.cpp:46-49
+  if (nls > 0 && s.mp_corrections_until_reinit < 1)
+    Fail("MultiPhase::CorrectionsUntilReinitialization must be greater than 0");
   DeriveLevelSet();  // cfl = iod_ls.reinit.cfl (LevelSetReinitializer.cpp:39), the band
 }
 

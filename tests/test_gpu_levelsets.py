@@ -153,3 +153,11 @@ def test_levelset_settings_refused():
     ls[1].materialid = 0  # material 0 cannot have a level set
     with pytest.raises(runtime.M2CError):
         runtime.Solver(g, mats, nmat, s, levelsets=ls)
+    g, mats, nmat, s, V, ID, phis, ls, _ = lc.two_bubbles(n=(16, 8, 8))
+    s.mp_corrections_until_reinit = 0  # CorrectionsUntilReinitialization >= 1
+    with pytest.raises(runtime.M2CError):
+        runtime.Solver(g, mats, nmat, s, levelsets=ls)
+    s.mp_corrections_until_reinit = 1
+    s.mp_ls_correction_frequency = 5  # the optional corrections are not built
+    with pytest.raises(runtime.M2CError):
+        runtime.Solver(g, mats, nmat, s, levelsets=ls)
