@@ -175,13 +175,13 @@ typedef struct {
   int32_t ls_rec_type;         /* M2CX_REC_* (Linear)                          */
   int32_t ls_limiter;          /* M2CX_LIM_* (GeneralizedMinMod)               */
   double ls_gmm_coeff;         /* GeneralizedMinModCoefficient (1.2)           */
-  /* MultiPhase (IoData.cpp:1706-1708): LevelSetCorrectionFrequency (-1; only
-   * -1 / 0 are supported: the optional gap corrections of
-   * MultiPhaseOperator.cpp:1764-1830 are not built) and
-   * CorrectionsUntilReinitialization (1): with several level sets, the
-   * overlaps ResolveConflictsInLevelSets resolves are counted and every
-   * level set is reinitialized once the count reaches this number
-   * (TimeIntegrator.cpp:737-768).                                           */
+  /* MultiPhase (IoData.cpp:1706-1708): LevelSetCorrectionFrequency (-1:
+   * off; at the steps it divides, ResolveConflictsInLevelSets also runs its
+   * optional corrections, MultiPhaseOperator.cpp:1766-1968) and
+   * CorrectionsUntilReinitialization (1, at least 1): the nodes
+   * ResolveConflictsInLevelSets resolves are counted and every level set is
+   * reinitialized once the count reaches this number (TimeIntegrator.cpp:
+   * 731-768).                                                               */
   int32_t mp_ls_correction_frequency;
   int32_t mp_corrections_until_reinit;
 } m2cx_scheme;

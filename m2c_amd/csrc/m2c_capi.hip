@@ -401,8 +401,6 @@ int m2cx_create_levelsets(const m2cx_grid* grid, const m2cx_material* mats, int 
     if (bw && (bw <= 1 || L.reinit_frequency <= 0 || L.reinit_frequency >= bw)) return M2CX_E_ARG;
     if (bw && L.solver == M2CX_LSSOLVER_FINITE_VOLUME) return M2CX_E_UNSUPPORTED;
   }
-  // the optional corrections of ResolveConflictsInLevelSets are not built
-  if (sch->mp_ls_correction_frequency > 0) return M2CX_E_UNSUPPORTED;
   m2cx_scheme s0 = *sch;
   s0.n_levelsets = nls;
   if (nls > 0) scheme_set_ls(s0, ls[0]);
@@ -1306,31 +1304,62 @@ static bool reinit_due(const m2cx_solver* h, double time, double dt, int time_st
 // 720-800), one level set, no embedded surfaces.  FixUnresolvedNodes with the
 // failsafe density leaves no cell unresolved, so the level-set correction of
 // :771-793 never runs.
-// MultiPhaseOperator::ResolveConflictsInLevelSets, part I (MultiPhaseOperator
-// .cpp:1705-1762), with its boundary fill (TimeIntegrator.cpp:737-741): the
-// candidates (nodes inside two or more subdomains) are collected on the
-// device, sorted into the reference's (k, j, i) order here, and resolved in
-// that order by one device thread.
+// MultiPhaseOperator::ResolveConflictsInLevelSets (MultiPhaseOperator.cpp:
+// 1705-1990), with its boundary fill (TimeIntegrator.cpp:737-741).  Part I
+// alone (the default): the candidates (nodes inside two or more subdomains,
+// the only ones it can change) are collected on the device, sorted into the
+// reference's (k, j, i) order here, and resolved in that order by one device
+// thread.  At the steps LevelSetCorrectionFrequency divides, part I with its
+// optional branches and part II run as sweeps of planes i + j + k = t (one
+// launch per plane, the sequential order's results), since the optional
+// branches can change any node near an interface.
 static int resolve_conflicts(m2cx_solver* h, int* resolved) {
   *resolved = 0;
-  if (h->nls < 2) return 0;
-  const LSPtrs P = ls_ptrs(h);
-  const int cap = h->mp.list_cap;
+  if (h->nls == 0) return 0;
   hipStream_t st = h->stream;
-  k_lsr_reset<<<1, 1, 0, st>>>(h->cnt);
-  k_lsr_mark<<<blocks_for(h->ncells), 256, 0, st>>>(h->g, P, h->lsr_list, cap, h->cnt);
+  bool opt = false;
+  if (h->sch.mp_ls_correction_frequency > 0) {
+    long ts = h->host_time_step;
+    if (ts < 0) {
+      DevCounters c;
+      HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+      HIPCHK(h, hipStreamSynchronize(st));
+      ts = c.time_step;
+    }
+    opt = ts % h->sch.mp_ls_correction_frequency == 0;
+  }
+  if (!opt && h->nls < 2) return 0;
+  const LSPtrs P = ls_ptrs(h);
   DevCounters c;
-  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
-  HIPCHK(h, hipStreamSynchronize(st));
-  if (c.n_lsr == 0) return 0;
-  if (c.n_lsr > cap) return fail(h, M2CX_E_CAPACITY, "overlapping level-set subdomains exceed the list capacity");
-  std::vector<int> list(c.n_lsr);
-  HIPCHK(h, hipMemcpy(list.data(), h->lsr_list, list.size() * 4, hipMemcpyDeviceToHost));
-  std::sort(list.begin(), list.end());  // linear index order is (k, j, i) order
-  HIPCHK(h, hipMemcpy(h->lsr_list, list.data(), list.size() * 4, hipMemcpyHostToDevice));
-  k_lsr_resolve<<<1, 64, 0, st>>>(h->g, P, h->lsr_list, c.n_lsr, h->cnt);
-  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
-  HIPCHK(h, hipStreamSynchronize(st));
+  if (opt) {
+    const DevGrid& g = h->g;
+    k_lsr_reset<<<1, 1, 0, st>>>(h->cnt);
+    if (h->nls >= 2) {
+      const long m1 = (long)(g.NY + 2) * (g.NZ + 2);
+      for (int t = -3; t <= g.NX + g.NY + g.NZ; t++)
+        k_lsr_plane1<<<blocks_for(m1), 256, 0, st>>>(g, P, t, h->cnt);
+    }
+    const long m2 = (long)g.NY * g.NZ;
+    for (int t = 0; t <= g.NX + g.NY + g.NZ - 3; t++) k_lsr_plane2<<<blocks_for(m2), 256, 0, st>>>(g, P, t, h->cnt);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+  } else {
+    const int cap = h->mp.list_cap;
+    k_lsr_reset<<<1, 1, 0, st>>>(h->cnt);
+    k_lsr_mark<<<blocks_for(h->ncells), 256, 0, st>>>(h->g, P, h->lsr_list, cap, h->cnt);
+    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    if (c.n_lsr == 0) return 0;
+    if (c.n_lsr > cap) return fail(h, M2CX_E_CAPACITY, "overlapping level-set subdomains exceed the list capacity");
+    std::vector<int> list(c.n_lsr);
+    HIPCHK(h, hipMemcpy(list.data(), h->lsr_list, list.size() * 4, hipMemcpyDeviceToHost));
+    std::sort(list.begin(), list.end());  // linear index order is (k, j, i) order
+    HIPCHK(h, hipMemcpy(h->lsr_list, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+    k_lsr_resolve<<<1, 64, 0, st>>>(h->g, P, h->lsr_list, c.n_lsr, h->cnt);
+    HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+  }
   *resolved = c.n_resolved;
   if (*resolved) {
     const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
@@ -1360,7 +1389,7 @@ static int end_of_step(m2cx_solver* h) {
     any_reinit |= reinit_enabled(h);
     swap_ls(h, q);
   }
-  if (h->nls == 1 && !any_reinit && h->sch.mp_corrections_until_reinit > 0) {
+  if (h->nls == 1 && !any_reinit && h->sch.mp_ls_correction_frequency <= 0) {
     tic(h, "endstep", &e0);
     mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
     toc(h, "endstep", e0);
@@ -2230,6 +2259,9 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
     if (reinit_enabled(h))
       return efail(e, M2CX_E_UNSUPPORTED, "level-set reinitialization needs host control between steps; "
                                           "step such members alone");
+    if (h->nls > 1 || (h->nls > 0 && h->sch.mp_ls_correction_frequency > 0))
+      return efail(e, M2CX_E_UNSUPPORTED, "several level sets and level-set corrections need host control "
+                                          "between steps; step such members alone");
     if (h->sch.flux == M2CX_FLUX_GODUNOV && h->multimat && h->sch.mp_flux == M2CX_MPFLUX_NUMERICAL)
       return efail(e, M2CX_E_UNSUPPORTED, "Godunov side fluxes at interfaces (Flux = Godunov with "
                                           "MultiPhase.Flux = Numerical): step such members alone");

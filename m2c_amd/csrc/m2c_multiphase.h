@@ -1063,6 +1063,65 @@ __global__ void k_lsr_mark(DevGrid g, LSPtrs P, int* list, int cap, DevCounters*
   }
 }
 
+// Part I at one node of the ghosted brick; returns 1 when it resolved a node
+// inside several subdomains (counted).  With the optional corrections (opt)
+// a node next to two or more boundaries inside one subdomain, or on the
+// boundary (phi = 0) of some, has all of those level sets set to
+// +-mean|phi| over them, uncounted (MultiPhaseOperator.cpp:1766-1799).
+__device__ int lsr_part1_node(const DevGrid& g, const LSPtrs& P, long c, int i, int j, int k, bool opt) {
+  int bnd[M2CX_MAX_LEVELSETS];
+  int nb = 0, no = 0, ni = 0, owner0 = -1, inter0 = -1;
+  for (int l = 0; l < P.n; l++) {
+    const double* ph = P.phi[l];
+    const double p = ph[c];
+    const bool b = (i - 1 >= -1 && p * ph[c - 1] <= 0) || (i + 1 <= g.NX && p * ph[c + 1] <= 0) ||
+                   (j - 1 >= -1 && p * ph[c - g.SX] <= 0) || (j + 1 <= g.NY && p * ph[c + g.SX] <= 0) ||
+                   (k - 1 >= -1 && p * ph[c - g.S] <= 0) || (k + 1 <= g.NZ && p * ph[c + g.S] <= 0);
+    if (!b) continue;
+    bnd[nb++] = l;
+    if (p < 0) {
+      if (no++ == 0) owner0 = l;
+    } else if (p == 0) {
+      if (ni++ == 0) inter0 = l;
+    }
+  }
+  if (nb <= 1 || (no == 0 && ni == 0)) return 0;
+  if (no <= 1) {
+    if (!opt) return 0;
+    double new_phi = 0.0;
+    for (int w = 0; w < nb; w++) new_phi += fabs(P.phi[bnd[w]][c]);
+    new_phi /= (double)nb;
+    for (int w = 0; w < nb; w++) {
+      double* ph = P.phi[bnd[w]];
+      const bool own = (no == 0) ? bnd[w] == inter0 : ph[c] < 0;
+      ph[c] = own ? -new_phi : new_phi;
+    }
+    return 0;
+  }
+  // several owners: the largest |phi| (the first on ties) keeps the node
+  int new_owner = owner0;
+  double max_phi = fabs(P.phi[owner0][c]);
+  double new_phi = 0.0;
+  for (int w = 0; w < nb; w++) {
+    const double p = P.phi[bnd[w]][c];
+    if (!(p < 0)) continue;
+    if (fabs(p) > max_phi) {
+      new_owner = bnd[w];
+      max_phi = fabs(p);
+    }
+  }
+  for (int w = 0; w < nb; w++) {
+    const double p = P.phi[bnd[w]][c];
+    if (p < 0) new_phi += fabs(p);
+  }
+  new_phi /= (double)no;
+  for (int w = 0; w < nb; w++) {
+    double* ph = P.phi[bnd[w]];
+    if (ph[c] < 0) ph[c] = (bnd[w] == new_owner) ? -new_phi : new_phi;
+  }
+  return 1;
+}
+
 __global__ void k_lsr_resolve(DevGrid g, LSPtrs P, const int* __restrict__ list, int n, DevCounters* cnt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int resolved = 0;
@@ -1073,33 +1132,64 @@ __global__ void k_lsr_resolve(DevGrid g, LSPtrs P, const int* __restrict__ list,
     const int i = (int)(u - rr * sx) - 1;
     const int k = (int)(rr / sy) - 1;
     const int j = (int)(rr - (unsigned)(k + 1) * sy) - 1;
-    int owner[M2CX_MAX_LEVELSETS];
-    int nb = 0, no = 0;
-    for (int l = 0; l < P.n; l++) {
-      const double* ph = P.phi[l];
-      const double p = ph[c];
-      const bool bnd = (i - 1 >= -1 && p * ph[c - 1] <= 0) || (i + 1 <= g.NX && p * ph[c + 1] <= 0) ||
-                       (j - 1 >= -1 && p * ph[c - g.SX] <= 0) || (j + 1 <= g.NY && p * ph[c + g.SX] <= 0) ||
-                       (k - 1 >= -1 && p * ph[c - g.S] <= 0) || (k + 1 <= g.NZ && p * ph[c + g.S] <= 0);
-      if (!bnd) continue;
-      nb++;
-      if (p < 0) owner[no++] = l;
-    }
-    if (nb <= 1 || no <= 1) continue;  // the other branches act only with LevelSetCorrectionFrequency
-    int new_owner = owner[0];
-    double max_phi = fabs(P.phi[owner[0]][c]);
-    for (int w = 1; w < no; w++)
-      if (fabs(P.phi[owner[w]][c]) > max_phi) {
-        new_owner = owner[w];
-        max_phi = fabs(P.phi[owner[w]][c]);
-      }
-    double new_phi = 0.0;
-    for (int w = 0; w < no; w++) new_phi += fabs(P.phi[owner[w]][c]);
-    new_phi /= (double)no;
-    for (int w = 0; w < no; w++) P.phi[owner[w]][c] = (owner[w] == new_owner) ? -new_phi : new_phi;
-    resolved++;
+    resolved += lsr_part1_node(g, P, c, i, j, k, false);
   }
   cnt->n_resolved = resolved;
+}
+
+// The optional corrections in the reference's (k, j, i) order, one launch per
+// plane i + j + k = t: a node reads only itself and its six face neighbours;
+// those before it in (k, j, i) order lie on plane t - 1 (updated by the
+// previous launch), those after it on plane t + 1 (not yet), and no two nodes
+// of one plane are face neighbours, so a plane's nodes run in parallel with
+// the results of the sequential sweep.  Part I over the ghosted brick:
+__global__ void k_lsr_plane1(DevGrid g, LSPtrs P, int t, DevCounters* cnt) {
+  const long m = (long)(g.NY + 2) * (g.NZ + 2);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < m; q += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(q % (g.NY + 2)) - 1, k = (int)(q / (g.NY + 2)) - 1;
+    const int i = t - j - k;
+    if (i < -1 || i > g.NX) continue;
+    if (lsr_part1_node(g, P, gidx(g, i, j, k), i, j, k, true)) atomicAdd(&cnt->n_resolved, 1);
+  }
+}
+
+// Part II over the interior (MultiPhaseOperator.cpp:1836-1968): a background
+// node (phi >= 0 for every level set) with at most one background face
+// neighbour inside the domain goes to the level set with the smallest phi
+// (the first on ties), which becomes -phi.
+__device__ __forceinline__ bool lsr_background(const LSPtrs& P, long c) {
+  for (int l = 0; l < P.n; l++)
+    if (P.phi[l][c] < 0) return false;
+  return true;
+}
+
+__global__ void k_lsr_plane2(DevGrid g, LSPtrs P, int t, DevCounters* cnt) {
+  const long m = (long)g.NY * g.NZ;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < m; q += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(q % g.NY), k = (int)(q / g.NY);
+    const int i = t - j - k;
+    if (i < 0 || i >= g.NX) continue;
+    const long c = gidx(g, i, j, k);
+    if (!lsr_background(P, c)) continue;
+    int qi = 0;
+    if (i - 1 >= 0 && lsr_background(P, c - 1)) qi++;
+    if (i + 1 < g.NX && lsr_background(P, c + 1)) qi++;
+    if (j - 1 >= 0 && lsr_background(P, c - g.SX)) qi++;
+    if (j + 1 < g.NY && lsr_background(P, c + g.SX)) qi++;
+    if (k - 1 >= 0 && lsr_background(P, c - g.S)) qi++;
+    if (k + 1 < g.NZ && lsr_background(P, c + g.S)) qi++;
+    if (qi >= 2) continue;
+    double min_phi = DBL_MAX;
+    int new_owner = -1;
+    for (int l = 0; l < P.n; l++)
+      if (P.phi[l][c] < min_phi) {
+        new_owner = l;
+        min_phi = P.phi[l][c];
+      }
+    if (new_owner < 0) continue;
+    P.phi[new_owner][c] = -min_phi;
+    atomicAdd(&cnt->n_resolved, 1);
+  }
 }
 
 // UpdateMaterialIDAtGhostNodes (MultiPhaseOperator.cpp:919-942)

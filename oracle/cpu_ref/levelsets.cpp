@@ -6,11 +6,12 @@
 // work runs, so every single-level-set routine serves each level set.  The
 // routines that look at all of them restate
 //   MultiPhaseOperator::UpdateMaterialIDByLevelSet (MultiPhaseOperator.cpp:153-357)
-//   MultiPhaseOperator::ResolveConflictsInLevelSets, part I (:1705-1762)
+//   MultiPhaseOperator::ResolveConflictsInLevelSets (:1705-1990)
 //   SpaceOperator::CalculateGradPhiAtCellInterface (SpaceOperator.cpp:2211-2268)
 // and TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
 // 720-800) for several level sets.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <set>
 #include "solver.h"
@@ -126,7 +127,6 @@ void Solver::SetLevelSets(const m2cx_levelset* ls, int n) {
     if (seen.count(mid)) Fail("cannot initialize multiple level sets for the same material");
     seen.insert(mid);
   }
-  if (sch.mp_ls_correction_frequency > 0) Fail("LevelSetCorrectionFrequency > 0 is not supported");
   if (fatal) return;
   sch.n_levelsets = n;
   nls = n;
@@ -226,48 +226,105 @@ void Solver::UpdateMaterialIDByLevelSets() {
   UpdateMaterialIDAtGhostNodes();
 }
 
-// ResolveConflictsInLevelSets, part I (MultiPhaseOperator.cpp:1705-1762):
-// over the ghosted brick in (k, j, i) order, a node next to the boundary of
-// two or more subdomains and inside more than one of them goes to the one
-// with the largest |phi|; the others get +mean|phi| of the owners, the new
-// owner -mean|phi|.  Nodes are updated in place, so later nodes see earlier
-// results.  Returns the number of nodes resolved.
+// ResolveConflictsInLevelSets (MultiPhaseOperator.cpp:1705-1990).
+// Part I, over the ghosted brick in (k, j, i) order: a node next to the
+// boundary of two or more subdomains and inside more than one of them goes
+// to the one with the largest |phi| (the first on ties); the others get
+// +mean|phi| of the owners, the new owner -mean|phi|.  With the optional
+// corrections (LevelSetCorrectionFrequency, at the steps it divides) a node
+// next to two or more boundaries inside one subdomain, or on the boundary
+// (phi = 0) of some, has all of them set to +-mean|phi| over those level
+// sets, uncounted.  Part II (optional only), over the interior in (k, j, i)
+// order: a background node (phi >= 0 for every level set) with at most one
+// background face neighbour inside the domain goes to the level set with the
+// smallest phi, which becomes -phi.  Nodes are updated in place, so later
+// nodes see earlier results.  Returns the number of nodes resolved (part I's
+// multi-owner nodes and part II's nodes).
 int Solver::ResolveConflictsInLevelSets() {
-  if (nls < 2) return 0;
+  if (nls == 0) return 0;
+  const bool opt = sch.mp_ls_correction_frequency > 0 && time_step % sch.mp_ls_correction_frequency == 0;
   std::vector<std::vector<double>*> phi(nls);
   phi[0] = &Phi;
   for (int q = 1; q < nls; q++) phi[q] = &lsx[q - 1].Phi;
   int resolved = 0;
-  std::vector<int> boundaries, owner;
-  for (int k = -1; k <= m.NZ; k++)
-    for (int j = -1; j <= m.NY; j++)
-      for (int i = -1; i <= m.NX; i++) {
-        const long c = m.idx(i, j, k);
-        boundaries.clear();
-        for (int q = 0; q < nls; q++) {
-          const std::vector<double>& P = *phi[q];
-          const double p = P[c];
-          if ((i - 1 >= -1 && p * P[m.idx(i - 1, j, k)] <= 0) || (i + 1 <= m.NX && p * P[m.idx(i + 1, j, k)] <= 0) ||
-              (j - 1 >= -1 && p * P[m.idx(i, j - 1, k)] <= 0) || (j + 1 <= m.NY && p * P[m.idx(i, j + 1, k)] <= 0) ||
-              (k - 1 >= -1 && p * P[m.idx(i, j, k - 1)] <= 0) || (k + 1 <= m.NZ && p * P[m.idx(i, j, k + 1)] <= 0))
-            boundaries.push_back(q);
-        }
-        if (boundaries.size() <= 1) continue;
-        owner.clear();
-        for (int q : boundaries)
-          if ((*phi[q])[c] < 0) owner.push_back(q);
-        if (owner.size() <= 1) continue;  // the other branches act only with LevelSetCorrectionFrequency
-        int new_owner = owner[0];
-        double max_phi = std::fabs((*phi[owner[0]])[c]);
-        for (size_t w = 1; w < owner.size(); w++)
-          if (std::fabs((*phi[owner[w]])[c]) > max_phi) {
-            new_owner = owner[w];
-            max_phi = std::fabs((*phi[owner[w]])[c]);
+  std::vector<int> boundaries, owner, inter;
+  if (nls >= 2) {
+    for (int k = -1; k <= m.NZ; k++)
+      for (int j = -1; j <= m.NY; j++)
+        for (int i = -1; i <= m.NX; i++) {
+          const long c = m.idx(i, j, k);
+          boundaries.clear();
+          for (int q = 0; q < nls; q++) {
+            const std::vector<double>& P = *phi[q];
+            const double p = P[c];
+            if ((i - 1 >= -1 && p * P[m.idx(i - 1, j, k)] <= 0) || (i + 1 <= m.NX && p * P[m.idx(i + 1, j, k)] <= 0) ||
+                (j - 1 >= -1 && p * P[m.idx(i, j - 1, k)] <= 0) || (j + 1 <= m.NY && p * P[m.idx(i, j + 1, k)] <= 0) ||
+                (k - 1 >= -1 && p * P[m.idx(i, j, k - 1)] <= 0) || (k + 1 <= m.NZ && p * P[m.idx(i, j, k + 1)] <= 0))
+              boundaries.push_back(q);
           }
-        double new_phi = 0.0;
-        for (int q : owner) new_phi += std::fabs((*phi[q])[c]);
-        new_phi /= owner.size();
-        for (int q : owner) (*phi[q])[c] = (q == new_owner) ? -new_phi : new_phi;
+          if (boundaries.size() <= 1) continue;
+          owner.clear();
+          inter.clear();
+          for (int q : boundaries) {
+            const double p = (*phi[q])[c];
+            if (p < 0) owner.push_back(q);
+            else if (p == 0) inter.push_back(q);
+          }
+          if (owner.empty() && inter.empty()) continue;
+          if (owner.size() <= 1) {
+            // one owner, or none with some on their boundary: optional only
+            if (!opt) continue;
+            double new_phi = 0.0;
+            for (int q : boundaries) new_phi += std::fabs((*phi[q])[c]);
+            new_phi /= boundaries.size();
+            for (int q : boundaries) {
+              const bool own = owner.empty() ? q == inter[0] : (*phi[q])[c] < 0;
+              (*phi[q])[c] = own ? -new_phi : new_phi;
+            }
+            continue;
+          }
+          int new_owner = owner[0];
+          double max_phi = std::fabs((*phi[owner[0]])[c]);
+          for (size_t w = 1; w < owner.size(); w++)
+            if (std::fabs((*phi[owner[w]])[c]) > max_phi) {
+              new_owner = owner[w];
+              max_phi = std::fabs((*phi[owner[w]])[c]);
+            }
+          double new_phi = 0.0;
+          for (int q : owner) new_phi += std::fabs((*phi[q])[c]);
+          new_phi /= owner.size();
+          for (int q : owner) (*phi[q])[c] = (q == new_owner) ? -new_phi : new_phi;
+          resolved++;
+        }
+  }
+  if (!opt) return resolved;
+  // part II: isolated background nodes of the interior
+  auto background = [&](long c) {
+    for (int q = 0; q < nls; q++)
+      if ((*phi[q])[c] < 0) return false;
+    return true;
+  };
+  for (int k = 0; k < m.NZ; k++)
+    for (int j = 0; j < m.NY; j++)
+      for (int i = 0; i < m.NX; i++) {
+        const long c = m.idx(i, j, k);
+        if (!background(c)) continue;
+        int qi = 0;
+        const bool nb[6] = {i - 1 >= 0, i + 1 < m.NX, j - 1 >= 0, j + 1 < m.NY, k - 1 >= 0, k + 1 < m.NZ};
+        const long nc[6] = {m.idx(i - 1, j, k), m.idx(i + 1, j, k), m.idx(i, j - 1, k),
+                            m.idx(i, j + 1, k), m.idx(i, j, k - 1), m.idx(i, j, k + 1)};
+        for (int f = 0; f < 6 && qi < 2; f++)
+          if (nb[f] && background(nc[f])) qi++;
+        if (qi >= 2) continue;
+        double min_phi = DBL_MAX;
+        int new_owner = -1;
+        for (int q = 0; q < nls; q++)
+          if ((*phi[q])[c] < min_phi) {
+            new_owner = q;
+            min_phi = (*phi[q])[c];
+          }
+        if (new_owner < 0) continue;
+        (*phi[new_owner])[c] = -min_phi;
         resolved++;
       }
   return resolved;

@@ -146,3 +146,17 @@ def test_ensemble_with_learned_guess_matches_members_alone():
     assert rc == sum(a[3] for a in alone)
     _same_as_alone(alone, members)
     ens.close()
+
+
+def test_ensemble_rejects_several_levelsets_and_corrections():
+    from m2c_amd import runtime
+    import levelset_cases as lc
+    g, mats, nmat, s, V, ID, phis, ls, _ = lc.two_bubbles(n=(16, 8, 8), mat2=abi.material("SG", gamma=1.667))
+    m = runtime.Solver(g, mats, nmat, s, levelsets=ls)
+    with pytest.raises(runtime.M2CError, match="error -7"):
+        runtime.Ensemble([m])
+    (_, (g, mats, nmat, s, V, ID, Phi)), = configs.c4_instances(count=1, seed=1, edges=(12,))
+    s.mp_ls_correction_frequency = 2
+    m = runtime.Solver(g, mats, nmat, s)
+    with pytest.raises(runtime.M2CError, match="error -7"):
+        runtime.Ensemble([m])

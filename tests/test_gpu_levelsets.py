@@ -3,7 +3,9 @@ oracle, bit for bit: two bubbles of two materials, each tracked by its own
 level set, with full-domain or narrow-band reinitialization, Mesh or
 LevelSet normals; bubbles that overlap at the start (ResolveConflictsInLevelSets)
 and touch (interface faces between two tracked materials, whose Riemann
-normal uses both level sets); the initial condition built on the device."""
+normal uses both level sets); the initial condition built on the device;
+the optional corrections of ResolveConflictsInLevelSets
+(LevelSetCorrectionFrequency)."""
 import numpy as np
 import pytest
 
@@ -157,7 +159,98 @@ def test_levelset_settings_refused():
     s.mp_corrections_until_reinit = 0  # CorrectionsUntilReinitialization >= 1
     with pytest.raises(runtime.M2CError):
         runtime.Solver(g, mats, nmat, s, levelsets=ls)
-    s.mp_corrections_until_reinit = 1
-    s.mp_ls_correction_frequency = 5  # the optional corrections are not built
-    with pytest.raises(runtime.M2CError):
-        runtime.Solver(g, mats, nmat, s, levelsets=ls)
+
+
+def _planted(phis):
+    """an isolated background node inside each bubble (part II resolves it)"""
+    out = []
+    for q, p in enumerate(phis):
+        p = p.copy()
+        p[np.unravel_index(np.argmin(p), p.shape)] = 0.01 * (q + 1)
+        out.append(p)
+    return out
+
+
+@pytest.mark.parametrize("freq,until", [(1, 1), (2, 3)])
+def test_levelset_corrections_bitwise(freq, until):
+    """LevelSetCorrectionFrequency: part I with its optional branches and
+    part II as plane sweeps on the device, against the oracle's sequential
+    sweeps; the planted nodes are resolved at the first correction step, which
+    forces a reinitialization of both level sets when CorrectionsUntil
+    Reinitialization is 1."""
+    print("corrections every", freq, "steps, reinit after", until, flush=True)
+    g, mats, nmat, s, V, ID, phis, ls, _ = lc.two_bubbles(n=(32, 12, 12), mat2=HELIUM, rnormal=abi.RNORMAL_LEVEL_SET,
+                                                          reinit_frequency=4, shocks=(0.29, 1.71))
+    s.mp_ls_correction_frequency = freq
+    s.mp_corrections_until_reinit = until
+    gpu, o = _pair(g, mats, nmat, s, V, ID, _planted(phis), ls)
+    for step in range(10):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        _same(gpu, o, 2, "step %d" % step)
+    cg, co = gpu.counters(), o.counters()
+    assert cg["reinit_calls"] == co["reinit_calls"] == (6 if until == 1 else 4)
+    assert cg["id_changed_cells"] == co["id_changed"] > 0
+
+
+def test_levelset_corrections_overlap_bitwise():
+    """Overlapping bubbles with the corrections on every step: part I's
+    counted branch runs in the plane sweep, next to its optional branches."""
+    print("overlapping bubbles with corrections", flush=True)
+    g, mats, nmat, s, V, ID, phis, ls, _ = lc.two_bubbles(n=(32, 12, 12), mat2=HELIUM,
+                                                          centres=((0.8, 0.5, 0.5), (1.15, 0.5, 0.5)),
+                                                          radii=(0.2, 0.2), rnormal=abi.RNORMAL_LEVEL_SET,
+                                                          reinit_frequency=4)
+    s.mp_ls_correction_frequency = 1
+    gpu, o = _pair(g, mats, nmat, s, V, ID, phis, ls)
+    for step in range(6):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        _same(gpu, o, 2, "step %d" % step)
+    cg, co = gpu.counters(), o.counters()
+    assert cg["reinit_calls"] == co["reinit_calls"] >= 2
+
+
+def test_levelset_corrections_one_levelset_bitwise():
+    """One level set: part I does not run, part II does (the reference calls
+    ResolveConflictsInLevelSets for any number of level sets)."""
+    from m2c_amd import runtime
+    print("one level set with corrections", flush=True)
+    g, mats, nmat, s, V, ID, phis, ls, union = lc.two_bubbles(n=(32, 12, 12), mat2=HELIUM, shocks=(0.29, 1.71))
+    mats = abi.materials(mats[0], mats[1])
+    s.mp_ls_correction_frequency = 3
+    ID = np.where(ID == 2, 1, ID).astype(ID.dtype)
+    phi = _planted([union])[0]
+
+    def start(s):
+        o = ol.Oracle(g, mats, 2, s)
+        o.set_state(V, ID, phi)
+        o.apply_bc()
+        o.update_ghost_ids()
+        o.ls_apply_bc()
+        V0, ID0, P0 = o.get_state()
+        gpu = runtime.Solver(g, mats, 2, s)
+        gpu.set_state(V0, ID0, P0)
+        return gpu, o
+
+    # the step-3 correction forces a reinitialization: without a
+    # reinitializer that is an error, as in the reference
+    # (LevelSetOperator.cpp:1341-1345)
+    gpu, o = start(s)
+    o.run_steps(2)
+    gpu.run_steps(2)
+    with pytest.raises(RuntimeError, match="not specified"):
+        o.run_steps(1)
+    with pytest.raises(runtime.M2CError, match="not specified"):
+        gpu.run_steps(1)
+    s.reinit_frequency = 5
+    gpu, o = start(s)
+    u = _used(ID.shape)
+    for step in range(6):
+        gpu.run_steps(1)
+        o.run_steps(1)
+        Vg, IDg, Pg = gpu.get_state()
+        Vo, IDo, Po = o.get_state()
+        assert np.array_equal(Pg[u], Po[u]) and np.array_equal(IDg[u], IDo[u]) and np.array_equal(Vg[u], Vo[u]), step
+    # forced at step 3, at its frequency at step 5
+    assert gpu.counters()["reinit_calls"] == o.counters()["reinit_calls"] == 2

@@ -18,6 +18,10 @@ inside the built EOS set (the one that exists, HVImpactShafquatIslam/
   is resolved too);
 - the Riemann normal between two tracked materials: for two planar level
   sets the difference of their unit normals, normalized (closed form);
+- the optional corrections (LevelSetCorrectionFrequency): part I's branches
+  for one owner and for a node on a boundary (phi = 0), uncounted, and part
+  II's isolated background nodes in (k, j, i) order, on hand-made cases
+  whose results follow from the rules by hand;
 - the material-ID update: a node leaving one subdomain for another takes
   the other's material; entering none, its untracked neighbours' material;
   entering two is an error.
@@ -65,11 +69,11 @@ def test_two_levelsets_match_one_levelset_over_both_bubbles(rnormal):
     assert c2["riemann_calls"] == c1["riemann_calls"] > 0
 
 
-def _line(nx=8, n2=None):
+def _line(nx=8, n2=None, correction_frequency=-1):
     """nx x 1 x 1 cells on [0, nx]: centres 0.5, 1.5, ..."""
     g = abi.grid((nx, 1, 1), (0.0, 0.0, 0.0), (float(nx), 1.0, 1.0))
     mats = abi.materials(abi.material("SG"), abi.material("SG"), abi.material("SG"), abi.material("SG"))
-    s = abi.default_scheme()
+    s = abi.default_scheme(mp_ls_correction_frequency=correction_frequency)
     ls = [abi.levelset_from_scheme(s, materialid=m) for m in (1, 2, 3)[:n2 or 2]]
     o = ol.Oracle(g, mats, 4, s, levelsets=ls)
     x = configs._axis(nx, 0.0, float(nx))
@@ -182,3 +186,97 @@ def test_material_ids_two_subdomains_is_an_error():
     o.set_levelset(1, 3.0 - x)       # ... and subdomain 1
     with pytest.raises(RuntimeError, match="two material subdomains"):
         o.update_material_ids()
+
+
+def test_optional_correction_one_owner():
+    """Cell 2 lies inside subdomain 0 only, next to both boundaries: with the
+    corrections both level sets take +-mean|phi| there (uncounted); cell 3,
+    taken after it, then sees the updated cell 2 and is corrected too."""
+    o, x = _line(correction_frequency=1)
+    phi0, phi1 = x - 2.8, 2.9 - x
+    o.set_levelset(0, phi0)
+    o.set_levelset(1, phi1)
+    assert o.resolve_conflicts() == 0  # the optional branches are not counted
+    p0, p1 = o.levelset(0), o.levelset(1)
+    m2 = (abs(2.5 - 2.8) + abs(2.9 - 2.5)) / 2
+    m3 = (abs(3.5 - 2.8) + abs(2.9 - 3.5)) / 2
+    assert np.all(p0[..., 3] == -m2) and np.all(p1[..., 3] == m2)
+    assert np.all(p0[..., 4] == m3) and np.all(p1[..., 4] == -m3)
+    rest = np.ones(x.shape, bool)
+    rest[..., 3:5] = False
+    assert np.array_equal(p0[rest], phi0[rest]) and np.array_equal(p1[rest], phi1[rest])
+    # without the corrections nothing changes
+    o, x = _line(correction_frequency=-1)
+    o.set_levelset(0, phi0)
+    o.set_levelset(1, phi1)
+    assert o.resolve_conflicts() == 0
+    assert np.array_equal(o.levelset(0), phi0) and np.array_equal(o.levelset(1), phi1)
+
+
+def test_optional_correction_on_a_boundary():
+    """phi0 = 0 at cell 2, which is outside subdomain 1 next to its boundary:
+    level set 0 (the first on the boundary) takes it with -mean|phi|."""
+    o, x = _line(correction_frequency=1)
+    phi0, phi1 = x - 2.5, 2.9 - x
+    o.set_levelset(0, phi0)
+    o.set_levelset(1, phi1)
+    assert o.resolve_conflicts() == 0
+    p0, p1 = o.levelset(0), o.levelset(1)
+    m2 = (0.0 + abs(2.9 - 2.5)) / 2
+    m3 = (abs(3.5 - 2.5) + abs(2.9 - 3.5)) / 2  # cell 3 then has one owner (level set 1)
+    assert np.all(p0[..., 3] == -m2) and np.all(p1[..., 3] == m2)
+    assert np.all(p0[..., 4] == m3) and np.all(p1[..., 4] == -m3)
+
+
+def _cube_case(nls):
+    n = 8
+    g = abi.grid((n, n, n), (0.0, 0.0, 0.0), (8.0, 8.0, 8.0))
+    mats = abi.materials(abi.material("SG"), abi.material("SG"), abi.material("SG"))
+    s = abi.default_scheme(mp_ls_correction_frequency=3)
+    ls = [abi.levelset_from_scheme(s, materialid=m) for m in (1, 2)[:nls]]
+    o = ol.Oracle(g, mats, 3, s, levelsets=ls)
+    return o
+
+
+def test_optional_correction_isolated_background_nodes_in_order():
+    """One level set covering everything but three pockets of background
+    nodes (phi > 0): a single node, a line of three along x, and a 2x2x2
+    block.  Part II gives a background node with at most one background
+    neighbour to the level set (phi -> -phi), in (k, j, i) order: the line's
+    middle node has two background neighbours at the start, but its first
+    node has gone by its turn, so the whole line goes; the block stays."""
+    o = _cube_case(1)
+    phi = -np.ones(o.shape)
+    planted = {(2, 2, 2): 0.3, (4, 5, 5): 0.5, (5, 5, 5): 0.25, (6, 5, 5): 0.75}
+    block = [(i, j, k) for i in (5, 6) for j in (1, 2) for k in (1, 2)]
+    for (i, j, k), v in planted.items():
+        phi[k + 1, j + 1, i + 1] = v
+    for (i, j, k) in block:
+        phi[k + 1, j + 1, i + 1] = 1.0
+    o.set_levelset(0, phi)
+    assert o.resolve_conflicts() == 4
+    p = o.levelset(0)
+    want = phi.copy()
+    for (i, j, k), v in planted.items():
+        want[k + 1, j + 1, i + 1] = -v
+    assert np.array_equal(p, want)
+
+
+def test_optional_correction_background_goes_to_the_smallest_phi():
+    """Two level sets: an isolated background node goes to the level set with
+    the smaller phi (the first on ties); the other keeps its value."""
+    o = _cube_case(2)
+    phi0 = -np.ones(o.shape)
+    phi1 = np.ones(o.shape)
+    phi0[3, 3, 3] = 0.7   # node (2, 2, 2): outside both, phi1 = 0.4 smaller
+    phi1[3, 3, 3] = 0.4
+    phi0[6, 6, 6] = 0.5   # node (5, 5, 5): a tie, level set 0 takes it
+    phi1[6, 6, 6] = 0.5
+    o.set_levelset(0, phi0)
+    o.set_levelset(1, phi1)
+    # part I: no node lies next to both boundaries (phi1 > 0 everywhere);
+    # part II: the two isolated nodes (step 0: the frequency 3 divides it)
+    assert o.resolve_conflicts() == 2
+    p0, p1 = o.levelset(0), o.levelset(1)
+    assert p1[3, 3, 3] == -0.4 and p0[3, 3, 3] == 0.7
+    assert p0[6, 6, 6] == -0.5 and p1[6, 6, 6] == 0.5
