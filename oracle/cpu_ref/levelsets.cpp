@@ -74,8 +74,6 @@ void Solver::DeriveLevelSet() {
     if (nb_bw <= 1) Fail("in the narrow-band level set method, bandwidth should be at least 3");
     else if (sch.reinit_frequency <= 0 || sch.reinit_frequency >= nb_bw)
       Fail("the level set reinitialization frequency should be smaller than bandwidth");
-    else if (sch.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME)
-      Fail("the narrow-band level set is built for the finite-difference level-set solver only");
   }
 }
 
@@ -104,6 +102,9 @@ void Solver::SwapLS(int q) {
   useful_nodes.swap(S.useful_nodes);
   active_nodes.swap(S.active_nodes);
   plus1_nodes.swap(S.plus1_nodes);
+  fvPf.swap(S.fvPf);
+  fvUf.swap(S.fvUf);
+  fvSl.swap(S.fvSl);
   std::swap(nb_bw, S.nb_bw);
   std::swap(nb_built, S.nb_built);
   std::swap(phi_max, S.phi_max);

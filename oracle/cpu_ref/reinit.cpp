@@ -653,6 +653,107 @@ void Solver::LSComputeResidualFVM(const std::vector<double>& Vin, const std::vec
       }
 }
 
+// LevelSetOperator::ComputeResidualFVM in a narrow band (LevelSetOperator.cpp:
+// 966-1062, 1160-1232, 1288-1306): ReconstructInBand reconstructs the
+// velocity components and phi at the ACTIVE nodes only (Reconstructor::
+// ReconstructIn1D and Reconstruct with Active as the selection, :922-1054,
+// 335-790), face ghosts by the boundary conditions where active; the
+// outputs live in buffers kept between calls, so a flux through a face of a
+// useful node that is not active reads what the buffers held (zero before
+// the first write, else that node's values from the last call in which it
+// was active).  The residual is zeroed on the useful nodes and the fluxes
+// through the minus faces of each useful node are added in the order of the
+// useful-node list (ComputeAdvectionFluxInBand, :1160-1232), which fixes the
+// summation order of each node's terms; the source term on the active nodes
+// inside the domain.
+void Solver::LSComputeResidualFVMNarrowBand(const std::vector<double>& Vin, const std::vector<double>& phi,
+                                            std::vector<double>& res) {
+  const long n = m.ncells();
+  if ((long)fvPf.size() != 6 * n) {
+    fvPf.assign(6 * n, 0.0);
+    fvUf.assign(6 * n, 0.0);
+    fvSl.assign(3 * n, 0.0);
+  }
+  std::vector<double>&pf = fvPf, &uf = fvUf, &sl = fvSl;
+  // the scalar field of ReconstructInBand holds the velocity component on the
+  // useful nodes; the active nodes' stencils lie on useful nodes, so it is
+  // read here from V itself
+  for (long c : active_nodes) {
+    int i, j, k;
+    m.ijk(c, i, j, k);
+    if (i < 0 || i >= m.NX || j < 0 || j >= m.NY || k < 0 || k >= m.NZ) continue;
+    const long st[3] = {1, m.PX, (long)m.PX * m.PY};
+    const double dd[3] = {m.DX(i), m.DY(j), m.DZ(k)};
+    for (int a = 0; a < 3; a++) {
+      const double A = A3[c * 3 + a], B = B3[c * 3 + a];
+      const int K = LSK3[c * 3 + a];
+      double u0 = Vin[(c - st[a]) * 5 + 1 + a], u1 = Vin[c * 5 + 1 + a], u2 = Vin[(c + st[a]) * 5 + 1 + a];
+      double sig = ls_sigma(sch, A, B, K, u1 - u0, u2 - u1);
+      uf[(2 * a) * n + c] = u1 - 0.5 * sig;
+      uf[(2 * a + 1) * n + c] = u1 + 0.5 * sig;
+      sl[a * n + c] = sig / dd[a];
+      double p0 = phi[c - st[a]], p1 = phi[c], p2 = phi[c + st[a]];
+      double sp = ls_sigma(sch, A, B, K, p1 - p0, p2 - p1);
+      pf[(2 * a) * n + c] = p1 - 0.5 * sp;
+      pf[(2 * a + 1) * n + c] = p1 + 0.5 * sp;
+    }
+  }
+  for (auto& gp : ghosts) {
+    const long c = m.idx(gp.i, gp.j, gp.k), im = m.idx(gp.ii, gp.jj, gp.kk);
+    if (!nbActive[c]) continue;
+    const int a = gp.side / 2;
+    const bool lo = (gp.side % 2) == 0;
+    const int gface = lo ? 2 * a + 1 : 2 * a, iface = lo ? 2 * a : 2 * a + 1;
+    switch (gp.bc) {
+      case M2CX_BC_INLET: case M2CX_BC_INLET2: case M2CX_BC_OVERSET:
+        uf[gface * n + c] = Vin[c * 5 + 1 + a];
+        pf[gface * n + c] = phi[c];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = 0.0;
+        break;
+      case M2CX_BC_OUTLET:
+        uf[gface * n + c] = uf[iface * n + im];
+        pf[gface * n + c] = pf[iface * n + im];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = sl[e * n + im];
+        break;
+      default:  // symmetry, slip and stick walls
+        uf[gface * n + c] = -uf[iface * n + im];
+        pf[gface * n + c] = pf[iface * n + im];
+        for (int e = 0; e < 3; e++) sl[e * n + c] = -sl[e * n + im];
+        break;
+    }
+  }
+  for (long c : useful_nodes) res[c] = 0.0;
+  for (long c : useful_nodes) {
+    int i, j, k;
+    m.ijk(c, i, j, k);
+    if (i == -1 || j == -1 || k == -1) continue;
+    if (k != m.NZ && j != m.NY) {
+      const long cm = m.idx(i - 1, j, k);
+      double f = ls_face_flux(sch, pf[1 * n + cm], pf[0 * n + c], uf[1 * n + cm], uf[0 * n + c]) / m.DX(i);
+      if (nbActive[cm]) res[cm] -= f;
+      if (nbActive[c]) res[c] += f;
+    }
+    if (k != m.NZ && i != m.NX) {
+      const long cm = m.idx(i, j - 1, k);
+      double f = ls_face_flux(sch, pf[3 * n + cm], pf[2 * n + c], uf[3 * n + cm], uf[2 * n + c]) / m.DY(j);
+      if (nbActive[cm]) res[cm] -= f;
+      if (nbActive[c]) res[c] += f;
+    }
+    if (j != m.NY && i != m.NX) {
+      const long cm = m.idx(i, j, k - 1);
+      double f = ls_face_flux(sch, pf[5 * n + cm], pf[4 * n + c], uf[5 * n + cm], uf[4 * n + c]) / m.DZ(k);
+      if (nbActive[cm]) res[cm] -= f;
+      if (nbActive[c]) res[c] += f;
+    }
+  }
+  for (long c : active_nodes) {
+    int i, j, k;
+    m.ijk(c, i, j, k);
+    if (i < 0 || i >= m.NX || j < 0 || j >= m.NY || k < 0 || k >= m.NZ) continue;
+    res[c] += phi[c] * (sl[0 * n + c] + sl[1 * n + c] + sl[2 * n + c]);
+  }
+}
+
 // MultiPhaseOperator::UpdateStateVariablesByExtrapolation (MultiPhaseOperator.
 // cpp:1155-1272): a cell that changed material takes the weighted average of
 // its neighbours (3x3x3) of the same material that did not change, each

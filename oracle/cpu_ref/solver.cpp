@@ -870,15 +870,18 @@ void Solver::LSApplyBoundaryConditions(std::vector<double>& phi, const std::vect
 
 void Solver::LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi,
                                std::vector<double>& res) {
+  if (nb_bw && !nb_built) {
+    Fail("the narrow band of the level set has not been constructed");
+    return;
+  }
   if (sch.ls_solver == M2CX_LSSOLVER_FINITE_VOLUME) {
-    LSComputeResidualFVM(Vin, phi, res);
+    if (nb_bw)
+      LSComputeResidualFVMNarrowBand(Vin, phi, res);
+    else
+      LSComputeResidualFVM(Vin, phi, res);
     return;
   }
   if (nb_bw) {
-    if (!nb_built) {
-      Fail("the narrow band of the level set has not been constructed");
-      return;
-    }
     LSComputeResidualFDMNarrowBand(Vin, phi, res);
     return;
   }

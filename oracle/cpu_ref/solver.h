@@ -121,6 +121,8 @@ class Solver {
   void LSApplyBoundaryConditions(std::vector<double>& phi, const std::vector<unsigned char>* useful = nullptr);
   void LSComputeResidual(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
   void LSComputeResidualFVM(const std::vector<double>& Vin, const std::vector<double>& phi, std::vector<double>& res);
+  void LSComputeResidualFVMNarrowBand(const std::vector<double>& Vin, const std::vector<double>& phi,
+                                      std::vector<double>& res);
   // direction of the 1D Riemann problem at material interfaces
   void GradPhiAtFace(const std::vector<double>& phi, int d, int i, int j, int k, double* dir) const;
   void BimaterialNormal(int d, int i, int j, int k, int myid, int neighborid, double* dir) const;
@@ -140,6 +142,11 @@ class Solver {
   std::vector<int> nbLevel;
   std::vector<unsigned char> nbUseful, nbActive;
   std::vector<long> useful_nodes, active_nodes, plus1_nodes;
+  // the finite-volume level set's reconstruction outputs (Phil..Phif, ul..wf,
+  // dudx..dwdz of LevelSetOperator), kept between calls: in the band only the
+  // active nodes are rewritten, and the fluxes read the others as left
+  // (zero before a first write)
+  std::vector<double> fvPf, fvUf, fvSl;
   double domain_diagonal = 0.0, phi_max = 0.0, phi_min = 0.0, phi_out_pos = 0.0, phi_out_neg = 0.0;
   void ConstructNarrowBand();
   // Several level sets: level set 0 lives in the members above; level set
@@ -150,6 +157,7 @@ class Solver {
     std::vector<double> Phi, Phi1, Phi_tmp, Rls, rTag, rSign, rPhibk, rPhi0, rPhi1, rR, rNormal, rPhiG2;
     std::vector<int> LSK3, nbLevel;
     std::vector<unsigned char> nbUseful, nbActive;
+    std::vector<double> fvPf, fvUf, fvSl;
     std::vector<long> useful_nodes, active_nodes, plus1_nodes;
     int nb_bw = 0;
     bool nb_built = false;
