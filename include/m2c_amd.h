@@ -162,7 +162,8 @@ typedef struct {
   int32_t reinit_first_layer;  /* FirstLayerTreatment, M2CX_REINIT_* (Fixed)   */
   /* LevelSet[0].Bandwidth (IoData.cpp:1519, INT_MAX): the narrow-band level
    * set of LevelSetOperator.cpp:84-99 when 2 <= ls_bandwidth < INT32_MAX
-   * (requires 1 <= reinit_frequency < ls_bandwidth, as the reference does);
+   * (requires 1 <= reinit_frequency < ls_bandwidth, as the reference does;
+   * finite-difference or finite-volume level set);
    * 0 or INT32_MAX: the level set over the full domain.  The band is built
    * by m2cx_initialize, or by m2cx_levelset_construct_narrow_band after the
    * level set is uploaded.                                                   */

@@ -16,6 +16,7 @@
 // writes exactly the nodes the reference writes, so values left over from
 // earlier calls are read where the reference reads them.
 #pragma once
+#include <climits>
 #include "m2c_reinit.h"
 
 namespace m2c {
@@ -38,6 +39,20 @@ struct NBBuf {
   double* phig2 = nullptr;                 // PhiG2 of the reinitializer
   double* r = nullptr;                     // R of the reinitializer
   NBState* st = nullptr;
+  // finite-volume level set in the band: each useful node's position in the
+  // reference's useful-node list (it orders the sums of the face fluxes), and
+  // the reconstruction outputs kept between calls (only the active nodes are
+  // rewritten; faces of the others read what is left, zero at first)
+  bool fv = false;
+  int* pos = nullptr;
+  int* key = nullptr;    // discovery key of the nodes of the layer being ordered
+  int* slot = nullptr;   // scan workspace
+  long slot_cap = 0;
+  int* bsum = nullptr;   // tile sums of the scan
+  long bsum_cap = 0;
+  int* total = nullptr;  // scan total
+  int n_list = 0;        // useful nodes (host copy, set with the positions)
+  double *fvpf = nullptr, *fvuf = nullptr, *fvsl = nullptr;  // phi faces [6][n], velocity faces [6][n], slopes [3][n]
 };
 
 // doubles as unsigned keys with the order of their values (-0 below +0)
@@ -584,10 +599,255 @@ __global__ void k_nb_copy_useful(NBBuf nb, double* __restrict__ dst, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// list order of the useful nodes (finite-volume level set in the band)
+// ---------------------------------------------------------------------------
+// ConstructNarrowBand pushes levels 0 and 1 in a (k, j, i) sweep of the
+// ghosted brick (:1500-1533); UpdateNarrowBand pushes them in the order of the
+// old list, which TagFirstLayerNodesInBand walks (:440-647, 1805-1821); then
+// PropagateNarrowBand appends each level b in the order its nodes are first
+// reached from the list: by the reaching node's position, then the direction
+// (-x, +x, -y, +y, -z, +z) (:1596-1728).  On the device a node's position is
+// a rank: in index order, in old-position order, or by the smallest key
+// (parent position - layer base) * 6 + direction over its parents, each
+// taken by an exclusive scan of 0/1 slots.
+constexpr int NB_SCAN_TILE = 1024;  // 256 threads x 4 items
+
+__global__ void __launch_bounds__(256) k_scan_tiles(int* d, long n, int* bsum) {
+  __shared__ int s[256];
+  const long base = (long)blockIdx.x * NB_SCAN_TILE + threadIdx.x * 4;
+  int v[4], sum = 0;
+  for (int e = 0; e < 4; e++) {
+    v[e] = (base + e < n) ? d[base + e] : 0;
+    sum += v[e];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int tv = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += tv;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - sum;
+  for (int e = 0; e < 4; e++) {
+    if (base + e < n) d[base + e] = run;
+    run += v[e];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+
+// one work-group: exclusive scan of the tile sums, 256 at a time
+__global__ void __launch_bounds__(256) k_scan_sums(int* bsum, long nt, int* total) {
+  __shared__ int s[256];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long b0 = 0; b0 < nt; b0 += 256) {
+    const long q = b0 + threadIdx.x;
+    const int v = (q < nt) ? bsum[q] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int tv = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += tv;
+      __syncthreads();
+    }
+    if (q < nt) bsum[q] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 255) carry += s[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_add(int* d, long n, const int* __restrict__ bsum) {
+  const long q0 = (long)blockIdx.x * NB_SCAN_TILE;
+  const int add = bsum[blockIdx.x];
+  for (int e = threadIdx.x; e < NB_SCAN_TILE; e += blockDim.x)
+    if (q0 + e < n) d[q0 + e] += add;
+}
+
+__global__ void k_pos_flag_first(DevGrid g, NBBuf nb) {
+  NB_GHOSTED_LOOP(q) nb.slot[q] = nb.level[q] <= 1;
+}
+__global__ void k_pos_set_first(DevGrid g, NBBuf nb) {
+  NB_GHOSTED_LOOP(q) if (nb.level[q] <= 1) nb.pos[q] = nb.slot[q];
+}
+// levels 0 and 1 of an update are nodes of the old band (k_nb_reseed), whose
+// positions are still in pos
+// (positions outside [0, m) cannot occur; the test only keeps a broken
+// invariant from writing outside the slots)
+__global__ void k_pos_mark_update(DevGrid g, NBBuf nb, long m) {
+  NB_GHOSTED_LOOP(q) {
+    if (!(nb.old_useful[q] && nb.level[q] <= 1)) continue;
+    const int p = nb.pos[q];
+    if (p >= 0 && p < m) nb.slot[p] = 1;
+  }
+}
+__global__ void k_pos_set_update(DevGrid g, NBBuf nb, long m) {
+  NB_GHOSTED_LOOP(q) {
+    if (!(nb.old_useful[q] && nb.level[q] <= 1)) continue;
+    const int p = nb.pos[q];
+    nb.pos[q] = (p >= 0 && p < m) ? nb.slot[p] : 0;
+  }
+}
+__global__ void k_pos_key_init(DevGrid g, NBBuf nb, int band) {
+  NB_GHOSTED_LOOP(q) if (nb.level[q] == band) nb.key[q] = INT_MAX;
+}
+__global__ void k_pos_keys(DevGrid g, NBBuf nb, int band, int base) {
+  NB_GHOSTED_LOOP(q) {
+    const int lp = nb.level[q];
+    if (!(band == 2 ? lp <= 1 : lp == band - 1)) continue;
+    int i, j, k;
+    ghost_ijk(g, q, i, j, k);
+    const int r = (nb.pos[q] - base) * 6;
+    const int ni[6] = {i - 1, i + 1, i, i, i, i}, nj[6] = {j, j, j - 1, j + 1, j, j};
+    const int nk[6] = {k, k, k, k, k - 1, k + 1};
+    const long off[6] = {-1, 1, -(long)g.SX, (long)g.SX, -g.S, g.S};
+    for (int d = 0; d < 6; d++) {
+      if (!populated(g, ni[d], nj[d], nk[d])) continue;
+      const long m = q + off[d];
+      if (nb.level[m] == band) atomicMin(&nb.key[m], r + d);
+    }
+  }
+}
+// every node of level b has a parent of level b - 1, so its key is below m
+__global__ void k_pos_mark_layer(DevGrid g, NBBuf nb, int band, long m) {
+  NB_GHOSTED_LOOP(q) if (nb.level[q] == band && nb.key[q] < m) nb.slot[nb.key[q]] = 1;
+}
+__global__ void k_pos_set_layer(DevGrid g, NBBuf nb, int band, int base, long m) {
+  NB_GHOSTED_LOOP(q) if (nb.level[q] == band) nb.pos[q] = base + ((nb.key[q] < m) ? nb.slot[nb.key[q]] : 0);
+}
+
+// ---------------------------------------------------------------------------
+// finite-volume level set in the band (LevelSetOperator::ReconstructInBand,
+// ComputeAdvectionFluxInBand, AddSourceTermInBand, LevelSetOperator.cpp:
+// 1031-1062, 1160-1232, 1288-1306)
+// ---------------------------------------------------------------------------
+// reconstruction at the active nodes inside the domain (the stencils lie on
+// useful nodes, where the reference's scalar field holds the velocity)
+__global__ void k_nb_fv_recon(LSArgs a, NBBuf nb) {
+  const DevGrid& g = a.g;
+  const double* __restrict__ P = a.phi_st;
+  const long n = g.ncells;
+  NB_LIST_LOOP(t, nb.st->n_useful) {
+    const long c = nb.ulist[t];
+    if (!nb.active[c]) continue;
+    int i, j, k;
+    ghost_ijk(g, c, i, j, k);
+    if (out_count(g, i, j, k)) continue;
+    const int ijk[3] = {i, j, k};
+    for (int D = 0; D < 3; D++) {
+      const LSFaces f = ls_faces(a, P, D, c, ijk[D]);
+      nb.fvuf[(2 * D) * n + c] = f.um;
+      nb.fvuf[(2 * D + 1) * n + c] = f.up;
+      nb.fvpf[(2 * D) * n + c] = f.pm;
+      nb.fvpf[(2 * D + 1) * n + c] = f.pp;
+      nb.fvsl[D * n + c] = f.slope;
+    }
+  }
+}
+
+// the face of an active face ghost towards the domain, by the mesh boundary
+// condition, from its image's kept face (Reconstructor.cpp:807-880, 1048-1140)
+__global__ void k_nb_fv_ghost(LSArgs a, NBBuf nb) {
+  const DevGrid& g = a.g;
+  const long n = g.ncells;
+  const long nyz = (long)g.NY * g.NZ, nxz = (long)g.NX * g.NZ, nxy = (long)g.NX * g.NY;
+  const long nfaces = 2 * (nyz + nxz + nxy);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nfaces; q += (long)gridDim.x * blockDim.x) {
+    long r = q;
+    int side, i, j, k;
+    if (r < 2 * nyz) { side = (int)(r / nyz); r %= nyz; j = r % g.NY; k = r / g.NY; i = side == 0 ? -1 : g.NX; }
+    else if ((r -= 2 * nyz) < 2 * nxz) { side = 2 + (int)(r / nxz); r %= nxz; i = r % g.NX; k = r / g.NX; j = side == 2 ? -1 : g.NY; }
+    else { r -= 2 * nxz; side = 4 + (int)(r / nxy); r %= nxy; i = r % g.NX; j = r / g.NX; k = side == 4 ? -1 : g.NZ; }
+    const long c = gidx(g, i, j, k);
+    if (!nb.active[c]) continue;
+    const int D = side / 2;
+    const bool lo = (side % 2) == 0;
+    const long im = c + (lo ? 1 : -1) * axis_stride(g, D);
+    const int gf = lo ? 2 * D + 1 : 2 * D, imf = lo ? 2 * D : 2 * D + 1;
+    double u, p;
+    ls_ghost_face(a, a.phi_st, side, c, nb.fvuf[imf * n + im], nb.fvpf[imf * n + im], u, p);
+    nb.fvuf[gf * n + c] = u;
+    nb.fvpf[gf * n + c] = p;
+  }
+}
+
+// the flux through the minus face of node d along axis D, from the kept faces
+__device__ __forceinline__ double nb_fv_flux(const LSArgs& a, const NBBuf& nb, int D, long d, int idx) {
+  const long n = a.g.ncells, m = d - axis_stride(a.g, D);
+  return ls_face_flux(a, nb.fvpf[(2 * D + 1) * n + m], nb.fvpf[(2 * D) * n + d], nb.fvuf[(2 * D + 1) * n + m],
+                      nb.fvuf[(2 * D) * n + d]) /
+         g_axis(a.g, 3 + D)[idx + 1];
+}
+
+// R zero on the useful nodes; at an active node the fluxes the reference's
+// list loop adds to it -- its own minus faces (x, y, z) at its own turn, each
+// plus face at the turn of the neighbour across it -- in list order, then
+// the source term inside the domain
+__global__ void k_nb_fv_flux(LSArgs a, NBBuf nb) {
+  const DevGrid& g = a.g;
+  const double* __restrict__ P = a.phi_st;
+  const long n = g.ncells;
+  NB_LIST_LOOP(t, nb.st->n_useful) {
+    const long c = nb.ulist[t];
+    if (!nb.active[c]) {
+      a.rls[c] = 0.0;
+      continue;
+    }
+    int i, j, k;
+    ghost_ijk(g, c, i, j, k);
+    int tp[4], tk[4], nt = 0;
+    if (i != -1 && j != -1 && k != -1) { tp[nt] = nb.pos[c]; tk[nt++] = 0; }
+    if (i + 1 <= g.NX && j != -1 && k != -1 && k != g.NZ && j != g.NY && nb.useful[c + 1]) {
+      tp[nt] = nb.pos[c + 1]; tk[nt++] = 1;
+    }
+    if (j + 1 <= g.NY && i != -1 && k != -1 && k != g.NZ && i != g.NX && nb.useful[c + g.SX]) {
+      tp[nt] = nb.pos[c + g.SX]; tk[nt++] = 2;
+    }
+    if (k + 1 <= g.NZ && i != -1 && j != -1 && j != g.NY && i != g.NX && nb.useful[c + g.S]) {
+      tp[nt] = nb.pos[c + g.S]; tk[nt++] = 3;
+    }
+    for (int x = 1; x < nt; x++)  // list order
+      for (int y = x; y > 0 && tp[y - 1] > tp[y]; y--) {
+        const int a0 = tp[y], a1 = tk[y];
+        tp[y] = tp[y - 1]; tk[y] = tk[y - 1];
+        tp[y - 1] = a0; tk[y - 1] = a1;
+      }
+    double r = 0.0;
+    for (int x = 0; x < nt; x++) {
+      switch (tk[x]) {
+        case 0:
+          if (k != g.NZ && j != g.NY) r += nb_fv_flux(a, nb, 0, c, i);
+          if (k != g.NZ && i != g.NX) r += nb_fv_flux(a, nb, 1, c, j);
+          if (j != g.NY && i != g.NX) r += nb_fv_flux(a, nb, 2, c, k);
+          break;
+        case 1: r -= nb_fv_flux(a, nb, 0, c + 1, i + 1); break;
+        case 2: r -= nb_fv_flux(a, nb, 1, c + g.SX, j + 1); break;
+        default: r -= nb_fv_flux(a, nb, 2, c + g.S, k + 1); break;
+      }
+    }
+    if (!out_count(g, i, j, k)) r += P[c] * (nb.fvsl[0 * n + c] + nb.fvsl[1 * n + c] + nb.fvsl[2 * n + c]);
+    a.rls[c] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 inline int nb_alloc(NBBuf& nb, long ncells, hipStream_t st) {
   if (nb.level) return 0;
+  if (nb.fv) {
+    if (hipMalloc(&nb.pos, ncells * 4) != hipSuccess || hipMalloc(&nb.key, ncells * 4) != hipSuccess ||
+        hipMalloc(&nb.total, 4) != hipSuccess || hipMalloc(&nb.fvpf, 6 * ncells * 8) != hipSuccess ||
+        hipMalloc(&nb.fvuf, 6 * ncells * 8) != hipSuccess || hipMalloc(&nb.fvsl, 3 * ncells * 8) != hipSuccess)
+      return M2CX_E_OOM;
+    for (void* p : {(void*)nb.fvpf, (void*)nb.fvuf})
+      if (hipMemsetAsync(p, 0, 6 * ncells * 8, st) != hipSuccess) return M2CX_E_HIP;
+    if (hipMemsetAsync(nb.fvsl, 0, 3 * ncells * 8, st) != hipSuccess) return M2CX_E_HIP;
+  }
   if (hipMalloc(&nb.level, ncells * 4) != hipSuccess) return M2CX_E_OOM;
   for (unsigned char** p : {&nb.useful, &nb.active, &nb.plus1, &nb.old_useful})
     if (hipMalloc(p, ncells) != hipSuccess) return M2CX_E_OOM;
@@ -606,13 +866,86 @@ inline int nb_alloc(NBBuf& nb, long ncells, hipStream_t st) {
 
 inline void nb_free(NBBuf& nb) {
   for (void* p : {(void*)nb.level, (void*)nb.useful, (void*)nb.active, (void*)nb.plus1, (void*)nb.old_useful,
-                  (void*)nb.ulist, (void*)nb.plist, (void*)nb.phig2, (void*)nb.r, (void*)nb.st})
+                  (void*)nb.ulist, (void*)nb.plist, (void*)nb.phig2, (void*)nb.r, (void*)nb.st, (void*)nb.pos,
+                  (void*)nb.key, (void*)nb.slot, (void*)nb.bsum, (void*)nb.total, (void*)nb.fvpf, (void*)nb.fvuf,
+                  (void*)nb.fvsl})
     if (p) hipFree(p);
   const int bw = nb.bw;
   const double diag = nb.diag;
+  const bool fv = nb.fv;
   nb = NBBuf{};
   nb.bw = bw;
   nb.diag = diag;
+  nb.fv = fv;
+}
+
+// exclusive scan of d[0, n) in place; *total_host gets the sum
+inline int nb_scan(NBBuf& nb, int* d, long n, hipStream_t st, int* total_host) {
+  const long tiles = (n + NB_SCAN_TILE - 1) / NB_SCAN_TILE;
+  if (tiles == 0) {
+    *total_host = 0;
+    return 0;
+  }
+  if (tiles > nb.bsum_cap) {
+    if (nb.bsum) hipFree(nb.bsum);
+    nb.bsum = nullptr;
+    nb.bsum_cap = 0;
+    if (hipMalloc(&nb.bsum, tiles * 4) != hipSuccess) return M2CX_E_OOM;
+    nb.bsum_cap = tiles;
+  }
+  k_scan_tiles<<<(unsigned)tiles, 256, 0, st>>>(d, n, nb.bsum);
+  k_scan_sums<<<1, 256, 0, st>>>(nb.bsum, tiles, nb.total);
+  k_scan_add<<<(unsigned)tiles, 256, 0, st>>>(d, n, nb.bsum);
+  if (hipMemcpyAsync(total_host, nb.total, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return M2CX_E_HIP;
+  if (hipStreamSynchronize(st) != hipSuccess) return M2CX_E_HIP;
+  return 0;
+}
+
+inline int nb_slots(NBBuf& nb, long m, hipStream_t st) {
+  if (m > nb.slot_cap) {
+    if (nb.slot) hipFree(nb.slot);
+    nb.slot = nullptr;
+    nb.slot_cap = 0;
+    if (hipMalloc(&nb.slot, (m > 0 ? m : 1) * 4) != hipSuccess) return M2CX_E_OOM;
+    nb.slot_cap = m;
+  }
+  if (m > 0 && hipMemsetAsync(nb.slot, 0, m * 4, st) != hipSuccess) return M2CX_E_HIP;
+  return 0;
+}
+
+// the useful nodes' positions in the reference's list (after the levels of a
+// construction or an update are in place)
+inline int nb_positions(NBBuf& nb, const DevGrid& g, bool update, hipStream_t st) {
+  const unsigned B = (unsigned)blocks_for(g.ncells);
+  int cnt = 0;
+  if (!update) {
+    if (int rc = nb_slots(nb, g.ncells, st)) return rc;
+    k_pos_flag_first<<<B, 256, 0, st>>>(g, nb);
+    if (int rc = nb_scan(nb, nb.slot, g.ncells, st, &cnt)) return rc;
+    k_pos_set_first<<<B, 256, 0, st>>>(g, nb);
+  } else {
+    if (int rc = nb_slots(nb, nb.n_list, st)) return rc;
+    const long m = nb.n_list;
+    k_pos_mark_update<<<B, 256, 0, st>>>(g, nb, m);
+    if (int rc = nb_scan(nb, nb.slot, m, st, &cnt)) return rc;
+    k_pos_set_update<<<B, 256, 0, st>>>(g, nb, m);
+  }
+  int base = 0;
+  for (int band = 2; band <= nb.bw; band++) {
+    const long m = 6L * cnt;
+    if (m >= INT_MAX) return M2CX_E_CAPACITY;
+    if (int rc = nb_slots(nb, m, st)) return rc;
+    k_pos_key_init<<<B, 256, 0, st>>>(g, nb, band);
+    k_pos_keys<<<B, 256, 0, st>>>(g, nb, band, base);
+    k_pos_mark_layer<<<B, 256, 0, st>>>(g, nb, band, m);
+    int next = 0;
+    if (int rc = nb_scan(nb, nb.slot, m, st, &next)) return rc;
+    k_pos_set_layer<<<B, 256, 0, st>>>(g, nb, band, base + cnt, m);
+    base += cnt;
+    cnt = next;
+  }
+  nb.n_list = base + cnt;
+  return 0;
 }
 
 // the band from levels 0 and 1 already in place: the higher levels, the
@@ -625,14 +958,17 @@ inline void nb_propagate(NBBuf& nb, const DevGrid& g, hipStream_t st) {
 }
 
 // ConstructNarrowBand (LevelSetReinitializer.cpp:1479-1592)
-inline void nb_construct(NBBuf& nb, const DevGrid& g, double* phi, hipStream_t st) {
+inline int nb_construct(NBBuf& nb, const DevGrid& g, double* phi, hipStream_t st) {
   k_nb_seed<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, nb);
   nb_propagate(nb, g, st);
+  if (nb.fv)
+    if (int rc = nb_positions(nb, g, false, st)) return rc;
   k_nb_keys<<<1, 1, 0, st>>>(nb, 1);
   k_nb_minmax<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, nb, 0);
   k_nb_bounds<<<1, 1, 0, st>>>(nb, 1);
   k_nb_cutoff<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, nb);
   nb.built = true;
+  return 0;
 }
 
 }  // namespace m2c

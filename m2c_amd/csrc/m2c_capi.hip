@@ -399,7 +399,6 @@ int m2cx_create_levelsets(const m2cx_grid* grid, const m2cx_material* mats, int 
       return M2CX_E_ARG;
     const int bw = (L.bandwidth > 0 && L.bandwidth < INT32_MAX) ? L.bandwidth : 0;
     if (bw && (bw <= 1 || L.reinit_frequency <= 0 || L.reinit_frequency >= bw)) return M2CX_E_ARG;
-    if (bw && L.solver == M2CX_LSSOLVER_FINITE_VOLUME) return M2CX_E_UNSUPPORTED;
   }
   m2cx_scheme s0 = *sch;
   s0.n_levelsets = nls;
@@ -433,6 +432,7 @@ int m2cx_create_levelsets(const m2cx_grid* grid, const m2cx_material* mats, int 
     S.reinit_cfl = ls[q].reinit_cfl;
     S.nb.diag = diag;
     S.nb.bw = (ls[q].bandwidth > 0 && ls[q].bandwidth < INT32_MAX) ? ls[q].bandwidth : 0;
+    S.nb.fv = ls[q].solver == M2CX_LSSOLVER_FINITE_VOLUME;
     if (S.nb.bw)
       if (int rc = nb_alloc(S.nb, n, h->stream)) return fail(h, rc, "narrow-band buffers");
     h->lsx.push_back(S);
@@ -467,11 +467,10 @@ static int create_impl(const m2cx_grid* grid, const m2cx_material* mats, int nma
   // asserts there, CalculateGradPhiAtCellInterface)
   if (sch->riemann_normal != M2CX_RNORMAL_MESH && sch->n_levelsets < 1) return M2CX_E_UNSUPPORTED;
   // narrow band (LevelSetOperator.cpp:84-99): bandwidth at least 2 and the
-  // reinitialization frequency, in steps, below it; finite differences only
+  // reinitialization frequency, in steps, below it
   const int bw = (sch->n_levelsets > 0 && sch->ls_bandwidth > 0 && sch->ls_bandwidth < INT32_MAX)
                      ? sch->ls_bandwidth : 0;
   if (bw && (bw <= 1 || sch->reinit_frequency <= 0 || sch->reinit_frequency >= bw)) return M2CX_E_ARG;
-  if (bw && sch->ls_solver == M2CX_LSSOLVER_FINITE_VOLUME) return M2CX_E_UNSUPPORTED;
   auto* h = new m2cx_solver;
   *out = h;
   h->grid = *grid;
@@ -604,6 +603,7 @@ static int create_impl(const m2cx_grid* grid, const m2cx_material* mats, int nma
       const m2cx_grid& gr = *grid;
       h->nb.diag = std::sqrt((gr.xmax - gr.x0) * (gr.xmax - gr.x0) + (gr.ymax - gr.y0) * (gr.ymax - gr.y0) +
                              (gr.zmax - gr.z0) * (gr.zmax - gr.z0));
+      h->nb.fv = sch->ls_solver == M2CX_LSSOLVER_FINITE_VOLUME;
       if (int rc2 = nb_alloc(h->nb, h->ncells, h->stream)) return fail(h, rc2, "narrow-band buffers");
     }
     LSFvm& fv = h->mp.fv;
@@ -979,7 +979,14 @@ static void levelset_band(m2cx_solver* h, CF5 V, int which) {
   MPBuffers& B = h->mp;
   const int LB = nb_blocks(h);
   LSArgs a = mp_ls_args(B, h->g, V, which, h->cnt);
-  k_nb_ls_residual<<<LB, 256, 0, h->stream>>>(a, h->nb);
+  if (B.fv.on) {
+    const long nf = 2L * ((long)h->g.NY * h->g.NZ + (long)h->g.NX * h->g.NZ + (long)h->g.NX * h->g.NY);
+    k_nb_fv_recon<<<LB, 256, 0, h->stream>>>(a, h->nb);
+    k_nb_fv_ghost<<<blocks_for(nf), 256, 0, h->stream>>>(a, h->nb);
+    k_nb_fv_flux<<<LB, 256, 0, h->stream>>>(a, h->nb);
+  } else {
+    k_nb_ls_residual<<<LB, 256, 0, h->stream>>>(a, h->nb);
+  }
   if (which < 0) return;
   k_nb_ls_combine<<<LB, 256, 0, h->stream>>>(h->g, h->nb, which, B.phi, B.phi1, B.rls, h->cnt);
   double* tgt = (which == 1 || which == 3) ? B.phi1 : B.phi;
@@ -1176,7 +1183,7 @@ static int reinit_full(m2cx_solver* h, int special_maxits) {
 // LevelSetOperator::ConstructNarrowBandInReinitializer (LevelSetOperator.cpp:
 // 1328-1334) with ConstructNarrowBand (LevelSetReinitializer.cpp:1479-1592)
 static int nb_build(m2cx_solver* h) {
-  nb_construct(h->nb, h->g, h->mp.phi, h->stream);
+  if (int rc = nb_construct(h->nb, h->g, h->mp.phi, h->stream)) return fail(h, rc, "narrow band: list order");
   HIPCHK(h, hipGetLastError());
   return 0;
 }
@@ -1222,6 +1229,8 @@ static int reinit_band(m2cx_solver* h, int special_maxits) {
   // UpdateNarrowBand (:1777-1886)
   k_nb_reseed<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, b.tag, nb);
   nb_propagate(nb, g, st);
+  if (nb.fv)
+    if (int rc = nb_positions(nb, g, true, st)) return fail(h, rc, "narrow band: list order");
   k_nb_leave_enter<<<blocks_for(g.ncells), 256, 0, st>>>(g, phi, nb);
   k_nb_rsign<<<LB, 256, 0, st>>>(g, phi, b.sign, nb);
   const int lb = blocks_for(hs.nfl);

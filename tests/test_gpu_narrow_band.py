@@ -199,8 +199,7 @@ def test_band_settings_refused():
     g, mats, nmat, s, V, ID, phi = _sphere(bw=4, freq=4)  # frequency not below the bandwidth
     with pytest.raises(runtime.M2CError):
         runtime.Solver(g, mats, nmat, s)
-    g, mats, nmat, s, V, ID, phi = _sphere(bw=4, freq=2)
-    s.ls_solver = abi.LSSOLVER_FINITE_VOLUME
+    g, mats, nmat, s, V, ID, phi = _sphere(bw=1, freq=1)  # bandwidth below 2
     with pytest.raises(runtime.M2CError):
         runtime.Solver(g, mats, nmat, s)
     g, mats, nmat, s, V, ID, phi = _sphere(bw=4, freq=2)
