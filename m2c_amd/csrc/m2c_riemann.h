@@ -384,15 +384,10 @@ struct DevRiemann {
     return true;
   }
 
-  // ComputeRhoUStar (:1213-1420 adaptive, :2523-2690 non-adaptive).  The body
-  // is inlined into the main secant loop, where nearly every call of a long
-  // solve happens (a call saves and restores live registers through scratch:
-  // ~1.5k cycles of a ~4-10k-cycle call on the stalled faces), and kept out of
-  // line once (rhoustar) for the other call sites, to bound the code size.
-  __device__ __attribute__((always_inline)) bool rhoustar_body(int wave, Path& path, double rho, double u, double p,
-                                                               double ps, int id, double rhos0, double rhos1,
-                                                               double& rhos, double& us, bool* trans_rare,
-                                                               double* Vrare, bool count) {
+  // ComputeRhoUStar (:1213-1420 adaptive, :2523-2690 non-adaptive)
+  __device__ __noinline__ bool rhoustar(int wave, Path& path, double rho, double u, double p, double ps, int id, double rhos0,
+                           double rhos1, double& rhos, double& us, bool* trans_rare = nullptr,
+                           double* Vrare = nullptr, bool count = true) {
     if (count && wave == 1) st.fevals++;
     rhos = rho;
     us = u;
@@ -530,12 +525,6 @@ struct DevRiemann {
       rhos_0 = rhos_1; us_0 = us_1; ps_0 = ps_1; xi_0 = xi_1;
     }
     return done;
-  }
-
-  __device__ __noinline__ bool rhoustar(int wave, Path& path, double rho, double u, double p, double ps, int id,
-                                        double rhos0, double rhos1, double& rhos, double& us,
-                                        bool* trans_rare = nullptr, double* Vrare = nullptr, bool count = true) {
-    return rhoustar_body(wave, path, rho, u, p, ps, id, rhos0, rhos1, rhos, us, trans_rare, Vrare, count);
   }
 
   // ----------------------------------------------------------- bracketing
@@ -833,14 +822,14 @@ struct DevRiemann {
       }
       bool ok;
       while (true) {
-        ok = rhoustar_body(1, path1, rhol, ul, pl, p2, idl, rhol0, rhol1, rhol2, ul2, &trans_rare, Vrare, true);
+        ok = rhoustar(1, path1, rhol, ul, pl, p2, idl, rhol0, rhol1, rhol2, ul2, &trans_rare, Vrare);
         if (!ok) {
           p2 = 0.5 * (p2 + p0);
           iter++;
           if (iter < P.maxIts_main) { st.it_main++; continue; }
           break;
         }
-        ok = rhoustar_body(3, path3, rhor, ur, pr, p2, idr, rhor0, rhor1, rhor2, ur2, &trans_rare, Vrare, true);
+        ok = rhoustar(3, path3, rhor, ur, pr, p2, idr, rhor0, rhor1, rhor2, ur2, &trans_rare, Vrare);
         if (!ok) {
           p2 = 0.5 * (p2 + pp1);
           iter++;
