@@ -363,6 +363,13 @@ int m2cx_levelset_reinitialize(m2cx_solver* h, int special_maxits);
  * m2cx_levelset_residual and m2cx_levelset_reinitialize need it.
  * M2CX_E_UNSUPPORTED without a band. */
 int m2cx_levelset_construct_narrow_band(m2cx_solver* h);
+/* One level set's operator alone, as the reference calls it per LevelSet[i]
+ * (LevelSetOperator::Reinitialize, LevelSetOperator.cpp:1338-1363;
+ * ConstructNarrowBandInReinitializer, :1328-1334): level set `ls` reinitialized
+ * (must_do), or its band built; m2cx_levelset_reinitialize and
+ * m2cx_levelset_construct_narrow_band do this for every level set. */
+int m2cx_levelset_reinitialize_ls(m2cx_solver* h, int ls, int special_maxits);
+int m2cx_levelset_construct_narrow_band_ls(m2cx_solver* h, int ls);
 /* The band of level set `ls` for inspection: per ghosted cell its level (31415927 outside),
  * useful (level <= bandwidth) and active (level < bandwidth) flags, and
  * bounds = {phi_max, phi_min, phi_out_pos, phi_out_neg}; any may be null.

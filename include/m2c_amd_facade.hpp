@@ -62,20 +62,31 @@ class SpaceVariable3D;
 class Context {
  public:
   Context(const m2cx_grid& grid, const m2cx_material* mats, int nmat, const m2cx_scheme& scheme, int device = 0);
+  // several LevelSet[i] blocks (Main.cpp:285-330): level set i tracks
+  // levelsets[i].materialid; level set 0's block overrides the scheme's
+  Context(const m2cx_grid& grid, const m2cx_material* mats, int nmat, const m2cx_scheme& scheme,
+          const std::vector<m2cx_levelset>& levelsets, int device = 0);
   ~Context();
   Context(const Context&) = delete;
   Context& operator=(const Context&) = delete;
   m2cx_solver* handle() const { return h_; }
   const m2cx_grid& grid() const { return grid_; }
   const m2cx_scheme& scheme() const { return scheme_; }
+  int num_levelsets() const { return (int)levelsets_.size(); }
+  const m2cx_levelset& levelset(int i) const { return levelsets_.at(i); }
+  // the mirror id of level set i: M2CX_FIELD_PHI for 0, past the C ABI's
+  // fields for the others
+  static constexpr int kFields = 6 + M2CX_MAX_LEVELSETS - 1;
+  static int LevelSetField(int i) { return i == 0 ? M2CX_FIELD_PHI : 5 + i; }
   // device field -> host variable that mirrors it (nullptr: none)
-  SpaceVariable3D* holder[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  SpaceVariable3D* holder[kFields] = {};
   int check(int rc) const;  // throws on a negative return code
 
  private:
   m2cx_solver* h_ = nullptr;
   m2cx_grid grid_;
   m2cx_scheme scheme_;
+  std::vector<m2cx_levelset> levelsets_;
 };
 
 // SpaceVariable.h:50-208 (single rank, one ghost layer)
@@ -146,20 +157,21 @@ class SpaceOperator {
   std::shared_ptr<Context> ctx_;
 };
 
-// LevelSetOperator.h:75-110 (level set 0)
+// LevelSetOperator.h:75-110, one per level set (Main.cpp:285-330)
 class LevelSetOperator {
  public:
-  explicit LevelSetOperator(std::shared_ptr<Context> ctx) : ctx_(std::move(ctx)) {}
+  explicit LevelSetOperator(std::shared_ptr<Context> ctx, int ls = 0) : ctx_(std::move(ctx)), ls_(ls) {}
   void ComputeResidual(SpaceVariable3D& V, SpaceVariable3D& Phi, SpaceVariable3D& R, double time);
   bool Reinitialize(double time, double dt, int time_step, SpaceVariable3D& Phi, int special_maxIts = 0,
                     bool must_do = false);
   // LevelSetOperator.cpp:1328-1334: the narrow band around Phi's zero level
   // set (scheme.ls_bandwidth), Phi cut off outside it
   void ConstructNarrowBandInReinitializer(SpaceVariable3D& Phi);
-  int GetMaterialID() const { return ctx_->scheme().ls_materialid; }
+  int GetMaterialID() const { return ctx_->levelset(ls_).materialid; }
 
  private:
   std::shared_ptr<Context> ctx_;
+  int ls_;
 };
 
 // TimeIntegrator.h:60-216.  One step of the scheme's integrator, with the

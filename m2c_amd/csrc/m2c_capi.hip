@@ -2491,6 +2491,32 @@ int m2cx_levelset_construct_narrow_band(m2cx_solver* h) {
   return 0;
 }
 
+int m2cx_levelset_construct_narrow_band_ls(m2cx_solver* h, int ls) {
+  if (!h) return M2CX_E_ARG;
+  if (ls < 0 || ls >= h->nls) return fail(h, M2CX_E_ARG, "no such level set");
+  HIPCHK(h, hipSetDevice(h->device));
+  swap_ls(h, ls);
+  int rc = h->nb.bw ? nb_build(h) : fail(h, M2CX_E_UNSUPPORTED, "no narrow band: this level set's bandwidth is not set");
+  swap_ls(h, ls);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int m2cx_levelset_reinitialize_ls(m2cx_solver* h, int ls, int special_maxits) {
+  if (!h) return M2CX_E_ARG;
+  if (ls < 0 || ls >= h->nls) return fail(h, M2CX_E_ARG, "no such level set");
+  HIPCHK(h, hipSetDevice(h->device));
+  swap_ls(h, ls);
+  int rc = reinit_enabled(h)
+               ? reinitialize(h, special_maxits)
+               : fail(h, M2CX_E_UNSUPPORTED, "reinitialization requested but not specified (reinit_frequency)");
+  swap_ls(h, ls);
+  if (rc) return rc;
+  HIPCHK(h, hipGetLastError());
+  return check_error(h);
+}
+
 int m2cx_get_narrow_band(m2cx_solver* h, int ls, int32_t* level, uint8_t* useful, uint8_t* active,
                          double* bounds) {
   if (!h) return M2CX_E_ARG;

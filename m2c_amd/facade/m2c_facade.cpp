@@ -20,11 +20,46 @@ Context::Context(const m2cx_grid& grid, const m2cx_material* mats, int nmat, con
     h_ = nullptr;
     throw Error(rc, msg);
   }
+  if (scheme.n_levelsets > 0) {
+    // the scheme's own LevelSet[0] block
+    m2cx_levelset L{};
+    L.materialid = scheme.ls_materialid;
+    for (int q = 0; q < 6; q++) L.bc[q] = scheme.ls_bc[q];
+    L.solver = scheme.ls_solver;
+    L.flux = scheme.ls_flux;
+    L.delta = scheme.ls_delta;
+    L.rec_type = scheme.ls_rec_type;
+    L.limiter = scheme.ls_limiter;
+    L.gmm_coeff = scheme.ls_gmm_coeff;
+    L.reinit_frequency = scheme.reinit_frequency;
+    L.reinit_maxits = scheme.reinit_maxits;
+    L.reinit_frequency_dt = scheme.reinit_frequency_dt;
+    L.reinit_cfl = scheme.reinit_cfl;
+    L.reinit_tol = scheme.reinit_tol;
+    L.reinit_first_layer = scheme.reinit_first_layer;
+    L.bandwidth = scheme.ls_bandwidth;
+    levelsets_.push_back(L);
+  }
+}
+
+Context::Context(const m2cx_grid& grid, const m2cx_material* mats, int nmat, const m2cx_scheme& scheme,
+                 const std::vector<m2cx_levelset>& levelsets, int device)
+    : grid_(grid), scheme_(scheme), levelsets_(levelsets) {
+  const int rc = m2cx_create_levelsets(&grid, mats, nmat, &scheme, levelsets.data(), (int)levelsets.size(), device,
+                                       &h_);
+  if (rc < 0) {
+    const std::string msg = std::string("m2cx_create_levelsets: ") + m2cx_last_error(h_);
+    if (h_) m2cx_destroy(h_);
+    h_ = nullptr;
+    throw Error(rc, msg);
+  }
+  scheme_.n_levelsets = (int)levelsets.size();
+  if (!levelsets.empty()) scheme_.ls_materialid = levelsets[0].materialid;
 }
 
 Context::~Context() {
   // variables mirrored on the device keep their latest values on the host
-  for (int f = 0; f < 6; f++) {
+  for (int f = 0; f < kFields; f++) {
     if (!holder[f]) continue;
     try {
       holder[f]->SyncToHost();
@@ -76,6 +111,8 @@ void SpaceVariable3D::SyncToHost() {
     std::vector<int32_t> tmp(data_.size());
     ctx_->check(m2cx_download(ctx_->handle(), field_, tmp.data()));
     for (size_t q = 0; q < tmp.size(); q++) data_[q] = (double)tmp[q];
+  } else if (field_ >= 6) {
+    ctx_->check(m2cx_download_levelset(ctx_->handle(), field_ - 5, M2CX_LS_PHI, data_.data()));
   } else {
     ctx_->check(m2cx_download(ctx_->handle(), field_, data_.data()));
   }
@@ -101,6 +138,8 @@ void SpaceVariable3D::ToDevice(Context& ctx, int field) {
     std::vector<int32_t> tmp(data_.size());
     for (size_t q = 0; q < tmp.size(); q++) tmp[q] = (int32_t)data_[q];  // IDs are integer-valued doubles
     ctx.check(m2cx_upload(ctx.handle(), field, tmp.data()));
+  } else if (field >= 6) {
+    ctx.check(m2cx_upload_levelset(ctx.handle(), field - 5, data_.data()));
   } else {
     ctx.check(m2cx_upload(ctx.handle(), field, data_.data()));
   }
@@ -225,7 +264,9 @@ void SpaceOperator::ComputeResidual(SpaceVariable3D& V, SpaceVariable3D& ID, Spa
   Context& c = *ctx_;
   V.ToDevice(c, M2CX_FIELD_V);
   ID.ToDevice(c, M2CX_FIELD_ID);
-  if (Phi && !Phi->empty() && (*Phi)[0]) (*Phi)[0]->ToDevice(c, M2CX_FIELD_PHI);
+  if (Phi)
+    for (int i = 0; i < (int)Phi->size() && i < c.num_levelsets(); i++)
+      if ((*Phi)[i]) (*Phi)[i]->ToDevice(c, Context::LevelSetField(i));
   SpaceVariable3D::Evict(c, M2CX_FIELD_R);
   const int nfail = c.check(m2cx_compute_residual(c.handle(), M2CX_FIELD_V, riemann_solutions ? 1 : 0));
   R.ProducedOnDevice(c, M2CX_FIELD_R);
@@ -239,18 +280,18 @@ void LevelSetOperator::ComputeResidual(SpaceVariable3D& V, SpaceVariable3D& Phi,
   Context& c = *ctx_;
   if (R.NumDOF() != 1) throw Error(M2CX_E_ARG, "LevelSetOperator::ComputeResidual: R has one component");
   V.ToDevice(c, M2CX_FIELD_V);
-  Phi.ToDevice(c, M2CX_FIELD_PHI);
+  Phi.ToDevice(c, Context::LevelSetField(ls_));
   c.check(m2cx_levelset_residual(c.handle(), M2CX_FIELD_V));
   // the level-set residual has no field of its own: it comes to R's host copy
   R.SyncToHost();
   R.Unbind();
-  c.check(m2cx_download_ls_residual(c.handle(), R.HostData().data()));
+  c.check(m2cx_download_levelset(c.handle(), ls_, M2CX_LS_RESIDUAL, R.HostData().data()));
   R.RestoreDataPointerAndInsert();
 }
 
 bool LevelSetOperator::Reinitialize(double time, double dt, int time_step, SpaceVariable3D& Phi, int special_maxIts,
                                     bool must_do) {
-  const m2cx_scheme& s = ctx_->scheme();
+  const m2cx_levelset& s = ctx_->levelset(ls_);
   if (!must_do) {
     // isTimeToWrite (Utils.cpp:337-357) with the reinitialization frequency
     bool due = false;
@@ -261,17 +302,19 @@ bool LevelSetOperator::Reinitialize(double time, double dt, int time_step, Space
     if (!due) return false;
   }
   Context& c = *ctx_;
-  Phi.ToDevice(c, M2CX_FIELD_PHI);
-  c.check(m2cx_levelset_reinitialize(c.handle(), special_maxIts));
-  Phi.ProducedOnDevice(c, M2CX_FIELD_PHI);
+  const int f = Context::LevelSetField(ls_);
+  Phi.ToDevice(c, f);
+  c.check(m2cx_levelset_reinitialize_ls(c.handle(), ls_, special_maxIts));
+  Phi.ProducedOnDevice(c, f);
   return true;
 }
 
 void LevelSetOperator::ConstructNarrowBandInReinitializer(SpaceVariable3D& Phi) {
   Context& c = *ctx_;
-  Phi.ToDevice(c, M2CX_FIELD_PHI);
-  c.check(m2cx_levelset_construct_narrow_band(c.handle()));
-  Phi.ProducedOnDevice(c, M2CX_FIELD_PHI);
+  const int f = Context::LevelSetField(ls_);
+  Phi.ToDevice(c, f);
+  c.check(m2cx_levelset_construct_narrow_band_ls(c.handle(), ls_));
+  Phi.ProducedOnDevice(c, f);
 }
 
 // ---------------------------------------------------------- TimeIntegrator
@@ -289,11 +332,13 @@ void TimeIntegratorBase::AdvanceOneTimeStep(SpaceVariable3D& V, SpaceVariable3D&
   if (L || Xi || Vturb || LocalDt || subcycle != 0)
     throw Error(M2CX_E_UNSUPPORTED, "AdvanceOneTimeStep: laser, ionization, turbulence, local time stepping and "
                                     "subcycling are not on this path");
-  if (s.n_levelsets > 0 && (Phi.empty() || !Phi[0]))
-    throw Error(M2CX_E_ARG, "AdvanceOneTimeStep: the scheme has a level set and Phi is empty");
+  const int nls = c.num_levelsets();
+  if ((int)Phi.size() < nls) throw Error(M2CX_E_ARG, "AdvanceOneTimeStep: one Phi per level set");
+  for (int i = 0; i < nls; i++)
+    if (!Phi[i]) throw Error(M2CX_E_ARG, "AdvanceOneTimeStep: one Phi per level set");
   V.ToDevice(c, M2CX_FIELD_V);
   ID.ToDevice(c, M2CX_FIELD_ID);
-  if (s.n_levelsets > 0) Phi[0]->ToDevice(c, M2CX_FIELD_PHI);
+  for (int i = 0; i < nls; i++) Phi[i]->ToDevice(c, Context::LevelSetField(i));
   // the step works in the stage buffers and the residual buffer
   SpaceVariable3D::Evict(c, M2CX_FIELD_V1);
   SpaceVariable3D::Evict(c, M2CX_FIELD_V2);
@@ -301,7 +346,7 @@ void TimeIntegratorBase::AdvanceOneTimeStep(SpaceVariable3D& V, SpaceVariable3D&
   const int nfail = c.check(m2cx_advance_one_time_step(c.handle(), time, dt, time_step));
   V.ProducedOnDevice(c, M2CX_FIELD_V);
   ID.ProducedOnDevice(c, M2CX_FIELD_ID);
-  if (s.n_levelsets > 0) Phi[0]->ProducedOnDevice(c, M2CX_FIELD_PHI);
+  for (int i = 0; i < nls; i++) Phi[i]->ProducedOnDevice(c, Context::LevelSetField(i));
   riemann_warning(nfail);
 }
 

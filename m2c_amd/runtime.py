@@ -52,6 +52,8 @@ def lib():
         L.m2cx_download_ls_residual.argtypes = [P, D]
         L.m2cx_levelset_reinitialize.argtypes = [P, C.c_int]
         L.m2cx_levelset_construct_narrow_band.argtypes = [P]
+        L.m2cx_levelset_construct_narrow_band_ls.argtypes = [P, C.c_int]
+        L.m2cx_levelset_reinitialize_ls.argtypes = [P, C.c_int, C.c_int]
         L.m2cx_get_narrow_band.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
                                            C.POINTER(C.c_uint8), D]
         L.m2cx_create_levelsets.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int,

@@ -2,7 +2,9 @@
 façades of include/m2c_amd_facade.hpp (SpaceVariable3D with GetDataPointer /
 RestoreDataPointerAndInsert, SpaceOperator::ComputeTimeStepSize and
 ComputeResidual, TimeIntegratorRK2/RK3::AdvanceOneTimeStep), driven on the
-reference's fixture decks and compared with the oracle bit for bit."""
+reference's fixture decks and compared with the oracle bit for bit; with
+several level sets, one LevelSetOperator and one Phi per LevelSet[i] block,
+bands built through ConstructNarrowBandInReinitializer."""
 import ctypes as C
 import os
 
@@ -27,6 +29,10 @@ def driver():
     L.facade_run.restype = C.c_int
     L.facade_run.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int, C.POINTER(abi.Scheme), D, I, D,
                              C.c_int, D, D, D, D, D]
+    L.facade_run_levelsets.restype = C.c_int
+    L.facade_run_levelsets.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int,
+                                       C.POINTER(abi.Scheme), C.POINTER(abi.Levelset), C.c_int, D, I, D, C.c_int,
+                                       D, D, D, D]
     return L
 
 
@@ -65,3 +71,40 @@ def test_reference_time_loop_through_facades(fixtures, name):
     assert np.array_equal(Vf[inner], Vo[inner])
     assert np.array_equal(IDf[inner], IDo[inner].astype(np.float64))
     assert np.array_equal(Pf[inner], Po[inner])
+
+
+def test_several_levelsets_through_facades():
+    """two bubbles of two tracked materials (full-domain reinitialization
+    every 2 steps for one, a band of 4 for the other), stepped by the
+    reference's loop through the façades"""
+    import levelset_cases as lc
+    print("façade loop with two level sets", flush=True)
+    g, mats, nmat, s, V, ID, phis, ls, _ = lc.two_bubbles(n=(32, 12, 12), mat2=abi.material("SG", gamma=1.667),
+                                                          rnormal=abi.RNORMAL_LEVEL_SET, reinit_frequency=2,
+                                                          shocks=(0.29, 1.71))
+    ls[1].bandwidth = 4
+    o = ol.Oracle(g, mats, nmat, s, levelsets=ls)
+    o.set_state(V, ID, phis[0])
+    o.set_levelset(1, phis[1])
+    o.apply_bc()
+    o.update_ghost_ids()
+    o.ls_apply_bc()
+    V0, ID0, _ = o.get_state()
+    P0 = np.stack([o.levelset(q) for q in range(2)])
+    o.construct_narrow_band()
+    steps = 9
+    o.run_steps(steps)
+    Vc, IDc, P0c = np.ascontiguousarray(V0), np.ascontiguousarray(ID0, dtype=np.int32), np.ascontiguousarray(P0)
+    Vout, IDout, Pout, tout = np.zeros_like(Vc), np.zeros(ID0.shape), np.zeros_like(P0c), np.zeros(1)
+    arr = abi.levelsets(*ls)
+    st = driver().facade_run_levelsets(C.byref(g), mats, nmat, C.byref(s), arr, 2, _dp(Vc), IDc.ctypes.data_as(I),
+                                       _dp(P0c), steps, _dp(Vout), _dp(IDout), _dp(Pout), _dp(tout))
+    assert st == steps
+    Vo, IDo, _ = o.get_state()
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(Vout[inner], Vo[inner])
+    assert np.array_equal(IDout[inner].astype(np.int32), IDo[inner])
+    for q in range(2):
+        assert np.array_equal(Pout[q][inner], o.levelset(q)[inner]), q
+    assert tout[0] == o.time()[0]
+    assert o.counters()["id_changed"] > 0
