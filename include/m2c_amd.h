@@ -369,6 +369,13 @@ int m2cx_levelset_construct_narrow_band(m2cx_solver* h);
  * (must_do), or its band built; m2cx_levelset_reinitialize and
  * m2cx_levelset_construct_narrow_band do this for every level set. */
 int m2cx_levelset_reinitialize_ls(m2cx_solver* h, int ls, int special_maxits);
+/* MultiPhaseOperator::UpdateMaterialIDByLevelSet (MultiPhaseOperator.h:
+ * 82-83, MultiPhaseOperator.cpp:153-357) with the ID ghost fill: the material
+ * IDs from every level set's sign changes between phi_tmp (the level sets at
+ * the start of the step: nls ghosted fields, level set 0 first; it replaces
+ * the solver's copy) and the device level sets.  Error (M2CX_E_LEVELSETS)
+ * where a node enters two subdomains, as the reference exits. */
+int m2cx_update_material_id_by_levelset(m2cx_solver* h, const double* phi_tmp);
 int m2cx_levelset_construct_narrow_band_ls(m2cx_solver* h, int ls);
 /* The band of level set `ls` for inspection: per ghosted cell its level (31415927 outside),
  * useful (level <= bandwidth) and active (level < bandwidth) flags, and

@@ -1382,6 +1382,55 @@ static int resolve_conflicts(m2cx_solver* h, int* resolved) {
   return 0;
 }
 
+// materials that no level set tracks
+static int untracked_count(const m2cx_solver* h) {
+  const LSPtrs P = ls_ptrs(h);
+  int n = 0;
+  for (int m = 0; m < (int)h->mats.size(); m++) n += !((P.tracked >> m) & 1u);
+  return n;
+}
+
+// UpdateMaterialIDByLevelSet (MultiPhaseOperator.cpp:153-357) over every
+// level set, with Phi_tmp in each level set's phin: the sweep, then the
+// neighbour vote -- Jacobi sweeps in one work-group when at most one
+// material (material 0) has no level set, where the order cannot change the
+// result, else the reference's in-order sweeps (the remaining nodes sorted
+// here, one device thread) -- and the ghost IDs
+static int update_ids(m2cx_solver* h) {
+  MPBuffers& B = h->mp;
+  const DevGrid& g = h->g;
+  hipStream_t st = h->stream;
+  const LSPtrs P = ls_ptrs(h);
+  if (untracked_count(h) < 2) {
+    mp_end_ids(B, g, h->s, P, h->ID, h->cnt, st);
+    return 0;
+  }
+  const long nin = (long)g.NX * g.NY * g.NZ;
+  k_end_reset<<<1, 1, 0, st>>>(h->cnt);
+  k_id_sweep<<<blocks_for(nin), 256, 0, st>>>(g, h->s, P, h->ID, B.tag, B.remaining, B.list_cap, h->cnt);
+  DevCounters c;
+  HIPCHK(h, hipMemcpyAsync(&c, h->cnt, sizeof c, hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  const int n = std::min(c.n_remaining, B.list_cap);
+  if (n > 0) {
+    std::vector<int> list(n);
+    HIPCHK(h, hipMemcpy(list.data(), B.remaining, n * 4L, hipMemcpyDeviceToHost));
+    auto key = [&](int q) {  // (i, j, k), i most significant
+      long r = q;
+      const long i = r % g.SX;
+      r /= g.SX;
+      const long j = r % g.SY, k = r / g.SY;
+      return (i * g.SY + j) * g.SZ + k;
+    };
+    std::sort(list.begin(), list.end(), [&](int a, int b) { return key(a) < key(b); });
+    HIPCHK(h, hipMemcpy(B.remaining, list.data(), n * 4L, hipMemcpyHostToDevice));
+    k_id_vote_seq<<<1, 1, 0, st>>>(g, h->s, h->ID, B.tag, B.remaining, n, P.tracked);
+  }
+  const long nf = 2L * ((long)g.NY * g.NZ + (long)g.NX * g.NZ + (long)g.NX * g.NY);
+  k_ghost_ids<<<blocks_for(nf), 256, 0, st>>>(g, h->ID);
+  return 0;
+}
+
 // TimeIntegratorBase::UpdateSolutionAfterTimeStepping (TimeIntegrator.cpp:
 // 720-800), no embedded surfaces: overlaps of several level sets resolved,
 // the material IDs from every level set, each level set reinitialized at
@@ -1398,7 +1447,7 @@ static int end_of_step(m2cx_solver* h) {
     any_reinit |= reinit_enabled(h);
     swap_ls(h, q);
   }
-  if (h->nls == 1 && !any_reinit && h->sch.mp_ls_correction_frequency <= 0) {
+  if (h->nls == 1 && !any_reinit && h->sch.mp_ls_correction_frequency <= 0 && untracked_count(h) < 2) {
     tic(h, "endstep", &e0);
     mp_end_step(h->mp, h->g, h->s, f5(h, h->V[0]), h->ID, h->cnt, h->stream);
     toc(h, "endstep", e0);
@@ -1408,7 +1457,7 @@ static int end_of_step(m2cx_solver* h) {
   F5 V = f5(h, h->V[0]);
   int resolved = 0;
   if (int rc = resolve_conflicts(h, &resolved)) return rc;
-  mp_end_ids(B, h->g, h->s, ls_ptrs(h), h->ID, h->cnt, h->stream);
+  if (int rc = update_ids(h)) return rc;
   h->ls_bucket += resolved;
   const bool must = h->ls_bucket >= h->sch.mp_corrections_until_reinit;
   if (must) h->ls_bucket = 0;
@@ -2268,9 +2317,11 @@ int m2cx_ensemble_create(m2cx_solver* const* members, int n, m2cx_ensemble** out
     if (reinit_enabled(h))
       return efail(e, M2CX_E_UNSUPPORTED, "level-set reinitialization needs host control between steps; "
                                           "step such members alone");
-    if (h->nls > 1 || (h->nls > 0 && h->sch.mp_ls_correction_frequency > 0))
-      return efail(e, M2CX_E_UNSUPPORTED, "several level sets and level-set corrections need host control "
-                                          "between steps; step such members alone");
+    if (h->nls > 1 || (h->nls > 0 && h->sch.mp_ls_correction_frequency > 0) ||
+        (h->nls > 0 && untracked_count(h) >= 2))
+      return efail(e, M2CX_E_UNSUPPORTED, "several level sets, level-set corrections and two or more materials "
+                                          "without a level set need host control between steps; step such "
+                                          "members alone");
     if (h->sch.flux == M2CX_FLUX_GODUNOV && h->multimat && h->sch.mp_flux == M2CX_MPFLUX_NUMERICAL)
       return efail(e, M2CX_E_UNSUPPORTED, "Godunov side fluxes at interfaces (Flux = Godunov with "
                                           "MultiPhase.Flux = Numerical): step such members alone");
@@ -2489,6 +2540,23 @@ int m2cx_levelset_construct_narrow_band(m2cx_solver* h) {
   if (!any) return fail(h, M2CX_E_UNSUPPORTED, "no narrow band: ls_bandwidth is not set");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
+}
+
+int m2cx_update_material_id_by_levelset(m2cx_solver* h, const double* phi_tmp) {
+  if (!h || !phi_tmp) return M2CX_E_ARG;
+  if (!h->multimat || h->nls == 0) return fail(h, M2CX_E_UNSUPPORTED, "no level set");
+  HIPCHK(h, hipSetDevice(h->device));
+  for (int q = 0; q < h->nls; q++) {
+    swap_ls(h, q);
+    const hipError_t e = hipMemcpyAsync(h->mp.phin, phi_tmp + (long)q * h->ncells, h->ncells * 8,
+                                        hipMemcpyHostToDevice, h->stream);
+    swap_ls(h, q);
+    if (e != hipSuccess) return fail(h, M2CX_E_HIP, std::string("Phi_tmp upload: ") + hipGetErrorString(e));
+  }
+  if (int rc = update_ids(h)) return rc;
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_error(h);
 }
 
 int m2cx_levelset_construct_narrow_band_ls(m2cx_solver* h, int ls) {

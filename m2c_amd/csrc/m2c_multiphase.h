@@ -1043,6 +1043,53 @@ __global__ void __launch_bounds__(1024) k_id_vote(DevGrid g, Scheme s, unsigned 
   for (int q = threadIdx.x; q < nrem; q += blockDim.x) tag[remaining[q]] = 0;
 }
 
+// The same vote in the reference's order (MultiPhaseOperator.cpp:236-338),
+// for when two or more materials have no level set and the order can change
+// the result: the remaining nodes sorted as its std::set<Int3> (i, then j,
+// then k), taken by one thread; a node decided in a sweep is seen by the
+// nodes after it in that sweep.
+__global__ void k_id_vote_seq(DevGrid g, Scheme s, unsigned char* ID, unsigned char* tag, const int* __restrict__ list,
+                              int n, unsigned tracked) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int iter = 0; iter < 100; iter++) {
+    int left = 0;
+    for (int t = 0; t < n; t++) left += tag[list[t]] & 1;
+    if (left == 0) break;
+    int progress = 0;
+    for (int t = 0; t < n; t++) {
+      const long c = list[t];
+      if (!(tag[c] & 1)) continue;
+      long r = c;
+      const int i = (int)(r % g.SX) - 1;
+      r /= g.SX;
+      const int j = (int)(r % g.SY) - 1;
+      const int k = (int)(r / g.SY) - 1;
+      int best = 256;
+      for (int dk = -1; dk <= 1; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+          for (int di = -1; di <= 1; di++) {
+            const int ni = i + di, nj = j + dj, nk = k + dk;
+            if (ni < 0 || ni >= g.NX || nj < 0 || nj >= g.NY || nk < 0 || nk >= g.NZ) continue;
+            const long nc = gidx(g, ni, nj, nk);
+            if (tag[nc] & 1) continue;
+            const int nid = ID[nc];
+            if (nid == s.inactive || ((tracked >> nid) & 1u)) continue;
+            if (nid < best) best = nid;
+          }
+      if (best == 256) continue;
+      ID[c] = (unsigned char)best;
+      tag[c] = 0;
+      progress++;
+    }
+    if (progress == 0) {
+      for (int t = 0; t < n; t++)
+        if (tag[list[t]] & 1) ID[list[t]] = 0;  // orphans (MultiPhaseOperator.cpp:323-333)
+      break;
+    }
+  }
+  for (int t = 0; t < n; t++) tag[list[t]] = 0;
+}
+
 // ResolveConflictsInLevelSets, part I (MultiPhaseOperator.cpp:1705-1762).
 // Only a node inside two or more subdomains can be changed, so those are
 // collected first (any order; the host sorts them); one thread then takes

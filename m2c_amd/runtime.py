@@ -54,6 +54,7 @@ def lib():
         L.m2cx_levelset_construct_narrow_band.argtypes = [P]
         L.m2cx_levelset_construct_narrow_band_ls.argtypes = [P, C.c_int]
         L.m2cx_levelset_reinitialize_ls.argtypes = [P, C.c_int, C.c_int]
+        L.m2cx_update_material_id_by_levelset.argtypes = [P, D]
         L.m2cx_get_narrow_band.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
                                            C.POINTER(C.c_uint8), D]
         L.m2cx_create_levelsets.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Material), C.c_int,
@@ -217,6 +218,12 @@ class Solver:
     def upload_levelset(self, q, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         self._check(self.L.m2cx_upload_levelset(self.h, q, _d(phi)))
+
+    def update_material_ids(self, phi_tmp):
+        """MultiPhaseOperator::UpdateMaterialIDByLevelSet with Phi_tmp = the
+        given level sets (one ghosted field per level set)"""
+        arr = np.ascontiguousarray(np.stack([np.asarray(p, dtype=np.float64) for p in phi_tmp]))
+        self._check(self.L.m2cx_update_material_id_by_levelset(self.h, _d(arr)))
 
     def download_levelset(self, q, what=abi.LS_PHI):
         out = np.zeros(self.shape)

@@ -254,3 +254,21 @@ def test_levelset_corrections_one_levelset_bitwise():
         assert np.array_equal(Pg[u], Po[u]) and np.array_equal(IDg[u], IDo[u]) and np.array_equal(Vg[u], Vo[u]), step
     # forced at step 3, at its frequency at step 5
     assert gpu.counters()["reinit_calls"] == o.counters()["reinit_calls"] == 2
+
+
+def test_vote_in_the_reference_order_bitwise():
+    """two materials without a level set: the device's in-order vote against
+    the oracle on the line whose result depends on the order"""
+    from m2c_amd import runtime
+    import test_oracle_levelsets as tol
+    g, mats, s, V, ID, phi, phi_tmp = tol._vote_line()
+    o = ol.Oracle(g, mats, 3, s)
+    o.set_state(V, ID, phi)
+    o.set_levelset(0, phi_tmp, tmp=True)
+    o.update_material_ids()
+    gpu = runtime.Solver(g, mats, 3, s)
+    gpu.set_state(V, ID, phi)
+    gpu.update_material_ids([phi_tmp])
+    IDg, IDo = gpu.get_state()[1], o.get_state()[1]
+    assert IDg[1, 1, 1:-1].tolist() == [1, 1, 1, 1, 0, 0, 0, 0]
+    assert np.array_equal(IDg, IDo)

@@ -280,3 +280,36 @@ def test_optional_correction_background_goes_to_the_smallest_phi():
     p0, p1 = o.levelset(0), o.levelset(1)
     assert p1[3, 3, 3] == -0.4 and p0[3, 3, 3] == 0.7
     assert p0[6, 6, 6] == -0.5 and p1[6, 6, 6] == 0.5
+
+
+def _vote_line():
+    """8 x 1 x 1 cells; materials 0 and 1 have no level set, 2 is tracked.
+    Cells 0, 1 hold material 1, cells 2-4 material 2, cells 5-7 material 0;
+    cells 2-4 leave subdomain 2 for none (phi_tmp < 0, phi > 0)."""
+    g = abi.grid((8, 1, 1), (0.0, 0.0, 0.0), (8.0, 1.0, 1.0))
+    mats = abi.materials(abi.material("SG"), abi.material("SG"), abi.material("SG"))
+    s = abi.default_scheme(n_levelsets=1, ls_materialid=2)
+    V = np.zeros((3, 3, 10, 5))
+    V[..., 0] = 1.0
+    V[..., 4] = 1.0
+    ID = np.zeros((3, 3, 10), dtype=np.int32)
+    ID[..., 1:3] = 1
+    ID[..., 3:6] = 2
+    phi = np.ones((3, 3, 10))
+    phi_tmp = np.ones((3, 3, 10))
+    phi_tmp[..., 3:6] = -1.0
+    return g, mats, s, V, ID, phi, phi_tmp
+
+
+def test_vote_in_the_reference_order_with_two_untracked_materials():
+    """The neighbour vote in (i, j, k) order, a node decided in a sweep seen
+    by the later ones: cell 2 takes 1 (cell 1), cell 3 then 1 (cell 2), cell
+    4 the smaller of 1 (cell 3) and 0 (cell 5).  Sweeps that see only the
+    previous sweep's decisions would leave cell 3 for the second sweep, where
+    it takes the smaller of 1 and 0."""
+    g, mats, s, V, ID, phi, phi_tmp = _vote_line()
+    o = ol.Oracle(g, mats, 3, s)
+    o.set_state(V, ID, phi)
+    o.set_levelset(0, phi_tmp, tmp=True)
+    o.update_material_ids()
+    assert o.get_state()[1][1, 1, 1:-1].tolist() == [1, 1, 1, 1, 0, 0, 0, 0]
