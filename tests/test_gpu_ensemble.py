@@ -160,3 +160,34 @@ def test_ensemble_rejects_several_levelsets_and_corrections():
     m = runtime.Solver(g, mats, nmat, s)
     with pytest.raises(runtime.M2CError, match="error -7"):
         runtime.Ensemble([m])
+
+
+def test_ensemble_at_the_c4_member_sizes():
+    """members drawn from C4's own edge sizes (64, 96, 128 cells), with the
+    learned guess as in the benchmark, 3 steps: each member bit for bit as
+    run alone"""
+    import os
+    from m2c_amd import runtime
+    print("ensemble at C4 member sizes", flush=True)
+    theta = np.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "m2c_amd", "data",
+                                 "guess_params.npy"))
+    cases = configs.c4_instances(count=3, seed=5, edges=(64, 96, 128))
+    alone, members = [], []
+    for _, (g, mats, nmat, s, V, ID, Phi) in cases:
+        s.riem_guess = abi.GUESS_MLP
+        a = runtime.Solver(g, mats, nmat, s)
+        a.mlp_set_params(theta)
+        a.set_state(V, ID, Phi)
+        rc = a.run_steps(3)
+        alone.append((a.get_state(), a.counters(), a.time(), rc))
+        a.close()
+        m = runtime.Solver(g, mats, nmat, s)
+        m.mlp_set_params(theta)
+        m.set_state(V, ID, Phi)
+        members.append(m)
+    assert len({m.shape for m in members}) > 1  # uneven sizes
+    ens = runtime.Ensemble(members)
+    rc = ens.run_steps(3)
+    assert rc == sum(a[3] for a in alone)
+    _same_as_alone(alone, members)
+    ens.close()
